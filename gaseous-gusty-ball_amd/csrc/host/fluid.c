@@ -1,0 +1,833 @@
+/*
+ * Fluid solver, host side: owns the device state of a Fluid and sequences the gfx950
+ * kernels of one step (reference src/fluid.c).  No per-cell work happens here.
+ *
+ * Step order (reference fluid.c:424-478): departures -> absolute vorticity ->
+ * BFECC advect/advect/source/limited advect -> relax + un-Coriolis -> remove mean ->
+ * warm-started MG-PCG -> velocity (+ max speed).  The PCG loop runs on the device
+ * with device-resident scalars; the host enqueues iterations ahead (as many as the
+ * previous solve needed) and reads the loop flag back once per batch.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "fluid.h"
+#include "ggb_ext.h"
+#include "ggb_host.h"
+
+#define GGB_VERSION "ggb-amd 0.1 (gfx950)"
+
+/* ---------------- runtime helpers ---------------- */
+
+void ggb_die(const char *fmt, ...)
+{
+	va_list ap;
+	va_start(ap, fmt);
+	fprintf(stderr, "ggb: ");
+	vfprintf(stderr, fmt, ap);
+	fprintf(stderr, "\n");
+	va_end(ap);
+	exit(1);
+}
+
+int ggb_ensure_device(void) { return ggbk_init(); }
+
+int ggb_device_init(void) { return ggbk_init(); }
+
+const char *ggb_version(void) { return GGB_VERSION; }
+
+void ggb_sync(void)
+{
+	if (ggbk_sync())
+		ggb_die("device error: %s", ggbk_last_error());
+}
+
+void *ggb_dmalloc(size_t bytes, const char *what)
+{
+	void *p = ggbk_malloc(bytes);
+	if (!p)
+		ggb_die("%s: out of device memory (%zu bytes)", what, bytes);
+	return p;
+}
+
+static void upload(void *dst, const void *src, size_t bytes)
+{
+	if (ggbk_h2d(dst, src, bytes))
+		ggb_die("upload failed: %s", ggbk_last_error());
+}
+
+static void download(void *dst, const void *src, size_t bytes)
+{
+	if (ggbk_d2h(dst, src, bytes))
+		ggb_die("download failed: %s", ggbk_last_error());
+}
+
+/* ---------------- grid tables on the device ---------------- */
+
+static void *grid_buf(struct ggb_grid_dev *gd, size_t bytes, const void *host)
+{
+	void *p = ggb_dmalloc(bytes, "grid tables");
+	gd->bufs[gd->nbufs++] = p;
+	if (host)
+		upload(p, host, bytes);
+	return p;
+}
+
+ggbk_grid ggb_grid_dev(const CubeSphere *csc)
+{
+	CubeSphere *cs = (CubeSphere *) csc; /* the device cache is library-managed */
+	if (cs->dev)
+		return cs->dev->g;
+	if (ggb_ensure_device())
+		ggb_die("no GPU available");
+	int n = cs->ncells, N = cs->N;
+	int maxnb = 0;
+	for (int i = 0; i < n; i++)
+		if (cs->nnb[i] > maxnb)
+			maxnb = cs->nnb[i];
+	if (maxnb < 1)
+		maxnb = 1;
+	struct ggb_grid_dev *gd = calloc(1, sizeof(*gd));
+	if (!gd)
+		ggb_die("out of memory");
+	size_t nk = (size_t) maxnb * n;
+	int *nbr = malloc(sizeof(int) * nk);
+	float *w = malloc(sizeof(float) * nk);
+	float *el = malloc(sizeof(float) * nk);
+	float *gc = malloc(sizeof(float) * 3 * nk);
+	float *dinv = malloc(sizeof(float) * (size_t) n);
+	if (!nbr || !w || !el || !gc || !dinv)
+		ggb_die("out of memory");
+	for (int i = 0; i < n; i++) {
+		double s = 0;
+		for (int k = 0; k < maxnb; k++) {
+			int in = k < cs->nnb[i];
+			size_t src = (size_t) CS_MAXNB * i + k;
+			nbr[(size_t) k * n + i] = in ? cs->nbr[src] : i;
+			w[(size_t) k * n + i] = in ? cs->w[src] : 0.0f;
+			el[(size_t) k * n + i] = in ? cs->elen[src] : 0.0f;
+			for (int c = 0; c < 3; c++)
+				gc[((size_t) k * 3 + c) * n + i] = in ? cs->gc[3 * src + c] : 0.0f;
+		}
+		/* dinv = 1 / sum w (reference fluid.c:182-187, 58-63) */
+		for (int k = 0; k < cs->nnb[i]; k++)
+			s += cs->w[(size_t) CS_MAXNB * i + k];
+		dinv[i] = (float) (1.0 / s);
+	}
+	ggbk_grid g;
+	memset(&g, 0, sizeof(g));
+	g.N = N;
+	g.n = n;
+	g.maxnb = maxnb;
+	g.h = cs->h;
+	g.total_area = cs->total_area;
+	g.pos = grid_buf(gd, sizeof(float) * 3 * (size_t) n, cs->pos);
+	g.area = grid_buf(gd, sizeof(float) * (size_t) n, cs->area);
+	g.nnb = grid_buf(gd, (size_t) n, cs->nnb);
+	g.nbr = grid_buf(gd, sizeof(int) * nk, nbr);
+	g.w = grid_buf(gd, sizeof(float) * nk, w);
+	g.elen = grid_buf(gd, sizeof(float) * nk, el);
+	g.gc = grid_buf(gd, sizeof(float) * 3 * nk, gc);
+	g.ghost = grid_buf(gd, sizeof(int) * 6 * (size_t) (N + 2) * (N + 2), cs->ghost);
+	g.dinv = grid_buf(gd, sizeof(float) * (size_t) n, dinv);
+	free(nbr);
+	free(w);
+	free(el);
+	free(gc);
+	free(dinv);
+	gd->g = g;
+	cs->dev = gd;
+	return g;
+}
+
+void ggb_grid_dev_release(CubeSphere *cs)
+{
+	if (!cs->dev)
+		return;
+	ggb_sync();
+	for (int k = 0; k < cs->dev->nbufs; k++)
+		ggbk_free(cs->dev->bufs[k]);
+	free(cs->dev);
+	cs->dev = NULL;
+}
+
+/* ---------------- fluid device state ---------------- */
+
+struct ggb_fluid_dev {
+	int prec;
+	int T, TM, TZ;          /* state, multigrid and preconditioned-residual types */
+	size_t tsz, tmsz, tzsz;
+	int n;
+	ggbk_grid g0;
+	int nlev;
+	CubeSphere *coarse;     /* [nlev]; [0] unused */
+	ggbk_level *levels;     /* [nlev] */
+	void **xbufs;
+	int nxbufs;
+	void *omega, *omega_band, *psi, *vel, *q0, *q1, *q2, *dep_b, *dep_f, *r, *p, *Ap, *z;
+	ggbk_scalars *s;        /* device */
+	ggbk_scalars *hs;       /* pinned host copy */
+	double *partials;
+	int predict;            /* iterations the previous solve needed */
+	/* last-synced copies of the host mirrors (detects caller writes) */
+	float *shadow_omega, *shadow_band, *shadow_psi, *shadow_vel;
+};
+
+#define MAX_LIVE 64
+static const Fluid *g_live[MAX_LIVE];
+static pthread_mutex_t g_live_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static void live_add(const Fluid *F)
+{
+	pthread_mutex_lock(&g_live_mu);
+	for (int k = 0; k < MAX_LIVE; k++)
+		if (!g_live[k]) {
+			g_live[k] = F;
+			break;
+		}
+	pthread_mutex_unlock(&g_live_mu);
+}
+
+static void live_remove(const Fluid *F)
+{
+	pthread_mutex_lock(&g_live_mu);
+	for (int k = 0; k < MAX_LIVE; k++)
+		if (g_live[k] == F)
+			g_live[k] = NULL;
+	pthread_mutex_unlock(&g_live_mu);
+}
+
+const Fluid *ggb_fluid_of_vel(const float *vel)
+{
+	const Fluid *r = NULL;
+	pthread_mutex_lock(&g_live_mu);
+	for (int k = 0; k < MAX_LIVE; k++)
+		if (g_live[k] && g_live[k]->vel == vel)
+			r = g_live[k];
+	pthread_mutex_unlock(&g_live_mu);
+	return r;
+}
+
+const void *ggb_fluid_dev_vel(const Fluid *F, int *T)
+{
+	*T = F->dev->T;
+	return F->dev->vel;
+}
+
+static void *xbuf(struct ggb_fluid_dev *D, size_t bytes, const char *what)
+{
+	void *p = ggb_dmalloc(bytes, what);
+	D->xbufs[D->nxbufs++] = p;
+	return p;
+}
+
+static float *falloc(size_t n)
+{
+	float *f = calloc(n ? n : 1, sizeof(float));
+	if (!f)
+		ggb_die("fluid: out of memory");
+	return f;
+}
+
+/* reference fluid.c:44-47 */
+static int mg_levels(int N)
+{
+	int nlev = 1;
+	while ((N >> nlev) >= 4 && ((N >> (nlev - 1)) % 2) == 0)
+		nlev++;
+	return nlev;
+}
+
+/* Bilinear transfer from coarse level C to fine level Fg: taps of every fine centre on
+ * the coarse grid, and R = P^T as CSR with fine indices ascending (fluid.c:64-96). */
+static ggbk_xfer build_transfer(struct ggb_fluid_dev *D, const CubeSphere *fine, const CubeSphere *coarse)
+{
+	int nf = fine->ncells, nc = coarse->ncells;
+	int *ptap = malloc(sizeof(int) * 4 * (size_t) nf);
+	float *pw = malloc(sizeof(float) * 4 * (size_t) nf);
+	int *rstart = calloc((size_t) nc + 1, sizeof(int));
+	int *ridx = malloc(sizeof(int) * 4 * (size_t) nf);
+	float *rw = malloc(sizeof(float) * 4 * (size_t) nf);
+	int *cur = malloc(sizeof(int) * (size_t) nc);
+	int *pt_cm = malloc(sizeof(int) * 4 * (size_t) nf);
+	float *pw_cm = malloc(sizeof(float) * 4 * (size_t) nf);
+	if (!ptap || !pw || !rstart || !ridx || !rw || !cur || !pt_cm || !pw_cm)
+		ggb_die("fluid: out of memory");
+	#pragma omp parallel for schedule(static)
+	for (int i = 0; i < nf; i++) {
+		int idx[4];
+		double wt[4];
+		cs_taps(coarse, cs_pos(fine, i), idx, wt);
+		for (int k = 0; k < 4; k++) {
+			ptap[4 * i + k] = idx[k];
+			pw[4 * i + k] = (float) wt[k];
+			pt_cm[(size_t) k * nf + i] = idx[k];
+			pw_cm[(size_t) k * nf + i] = (float) wt[k];
+		}
+	}
+	for (int i = 0; i < 4 * nf; i++)
+		rstart[ptap[i] + 1]++;
+	for (int c = 0; c < nc; c++)
+		rstart[c + 1] += rstart[c];
+	memcpy(cur, rstart, sizeof(int) * (size_t) nc);
+	for (int i = 0; i < nf; i++)
+		for (int k = 0; k < 4; k++) {
+			int c = ptap[4 * i + k];
+			ridx[cur[c]] = i;
+			rw[cur[c]] = pw[4 * i + k];
+			cur[c]++;
+		}
+	ggbk_xfer x;
+	x.nf = nf;
+	x.nc = nc;
+	void *d;
+	d = xbuf(D, sizeof(int) * 4 * (size_t) nf, "transfer");
+	upload(d, pt_cm, sizeof(int) * 4 * (size_t) nf);
+	x.ptap = d;
+	d = xbuf(D, sizeof(float) * 4 * (size_t) nf, "transfer");
+	upload(d, pw_cm, sizeof(float) * 4 * (size_t) nf);
+	x.pw = d;
+	d = xbuf(D, sizeof(int) * ((size_t) nc + 1), "transfer");
+	upload(d, rstart, sizeof(int) * ((size_t) nc + 1));
+	x.rstart = d;
+	d = xbuf(D, sizeof(int) * 4 * (size_t) nf, "transfer");
+	upload(d, ridx, sizeof(int) * 4 * (size_t) nf);
+	x.ridx = d;
+	d = xbuf(D, sizeof(float) * 4 * (size_t) nf, "transfer");
+	upload(d, rw, sizeof(float) * 4 * (size_t) nf);
+	x.rw = d;
+	free(ptap); free(pw); free(rstart); free(ridx); free(rw); free(cur); free(pt_cm); free(pw_cm);
+	return x;
+}
+
+int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
+{
+	memset(F, 0, sizeof(*F));
+	if (ggb_ensure_device())
+		ggb_die("fluid_init: no GPU available (%s)", ggbk_last_error());
+	F->cs = cs;
+	F->prm = *prm;
+	if (F->prm.precision < GGB_PREC_FP64 || F->prm.precision > GGB_PREC_REF32)
+		ggb_die("fluid_init: unknown precision %d", F->prm.precision);
+	size_t n = cs->ncells;
+	F->omega = falloc(n);
+	F->omega_band = falloc(n);
+	F->psi = falloc(n);
+	F->vel = falloc(3 * n);
+
+	struct ggb_fluid_dev *D = calloc(1, sizeof(*D));
+	if (!D)
+		ggb_die("fluid: out of memory");
+	F->dev = D;
+	D->prec = F->prm.precision;
+	D->T = D->prec == GGB_PREC_REF32 ? GGBK_F32 : GGBK_F64;
+	D->TM = D->prec == GGB_PREC_FP64 ? GGBK_F64 : GGBK_F32;
+	D->tsz = D->T == GGBK_F64 ? 8 : 4;
+	D->tmsz = D->TM == GGBK_F64 ? 8 : 4;
+	D->n = (int) n;
+	D->g0 = ggb_grid_dev(cs);
+	D->nlev = prm->multigrid ? mg_levels(cs->N) : 0;
+	D->TZ = prm->multigrid ? D->TM : D->T;
+	D->tzsz = D->TZ == GGBK_F64 ? 8 : 4;
+	F->nlev = D->nlev;
+	D->xbufs = calloc(64 + 8 * (size_t) (D->nlev + 1), sizeof(void *));
+	if (!D->xbufs)
+		ggb_die("fluid: out of memory");
+
+	size_t ts = D->tsz;
+	D->omega = xbuf(D, ts * n, "fluid");
+	D->omega_band = xbuf(D, ts * n, "fluid");
+	D->psi = xbuf(D, ts * n, "fluid");
+	D->vel = xbuf(D, 3 * ts * n, "fluid");
+	D->q0 = xbuf(D, ts * n, "fluid");
+	D->q1 = xbuf(D, ts * n, "fluid");
+	D->q2 = xbuf(D, ts * n, "fluid");
+	D->dep_b = xbuf(D, 3 * ts * n, "fluid");
+	D->dep_f = xbuf(D, 3 * ts * n, "fluid");
+	D->r = xbuf(D, ts * n, "fluid");
+	D->p = xbuf(D, ts * n, "fluid");
+	D->Ap = xbuf(D, ts * n, "fluid");
+	D->z = xbuf(D, D->tzsz * n, "fluid");
+	D->s = xbuf(D, sizeof(ggbk_scalars), "fluid");
+	D->hs = ggbk_host_alloc(sizeof(ggbk_scalars));
+	if (!D->hs)
+		ggb_die("fluid: pinned allocation failed");
+	size_t nblk = (n + 255) / 256 + 1;
+	D->partials = xbuf(D, sizeof(double) * 2 * nblk, "fluid");
+	ggbk_zero(D->s, sizeof(ggbk_scalars));
+	ggbk_zero(D->omega, ts * n);
+	ggbk_zero(D->omega_band, ts * n);
+	ggbk_zero(D->psi, ts * n);
+	ggbk_zero(D->vel, 3 * ts * n);
+
+	if (D->nlev > 0) {
+		D->coarse = calloc((size_t) D->nlev, sizeof(CubeSphere));
+		D->levels = calloc((size_t) D->nlev, sizeof(ggbk_level));
+		if (!D->coarse || !D->levels)
+			ggb_die("fluid: out of memory");
+		D->levels[0].g = D->g0;
+		D->levels[0].x = NULL; /* z is passed per call */
+		D->levels[0].b = NULL;
+		D->levels[0].r = xbuf(D, D->tmsz * n, "multigrid");
+		for (int l = 1; l < D->nlev; l++) {
+			if (cs_init(&D->coarse[l], cs->N >> l))
+				ggb_die("fluid_init: coarse grid allocation failed");
+			const CubeSphere *fine = l == 1 ? cs : &D->coarse[l - 1];
+			D->levels[l].g = ggb_grid_dev(&D->coarse[l]);
+			D->levels[l - 1].xf = build_transfer(D, fine, &D->coarse[l]);
+			size_t nl = D->coarse[l].ncells;
+			D->levels[l].x = xbuf(D, D->tmsz * nl, "multigrid");
+			D->levels[l].b = xbuf(D, D->tmsz * nl, "multigrid");
+			D->levels[l].r = xbuf(D, D->tmsz * nl, "multigrid");
+		}
+	}
+	D->shadow_omega = falloc(n);
+	D->shadow_band = falloc(n);
+	D->shadow_psi = falloc(n);
+	D->shadow_vel = falloc(3 * n);
+	D->predict = 1;
+	if (ggbk_check("fluid_init"))
+		ggb_die("fluid_init: %s", ggbk_last_error());
+	live_add(F);
+	return 0;
+}
+
+void fluid_free(Fluid *F)
+{
+	struct ggb_fluid_dev *D = F->dev;
+	if (D) {
+		live_remove(F);
+		ggb_sync();
+		for (int k = 0; k < D->nxbufs; k++)
+			ggbk_free(D->xbufs[k]);
+		free(D->xbufs);
+		for (int l = 1; l < D->nlev; l++)
+			cs_free(&D->coarse[l]);
+		free(D->coarse);
+		free(D->levels);
+		ggbk_host_free(D->hs);
+		free(D->shadow_omega);
+		free(D->shadow_band);
+		free(D->shadow_psi);
+		free(D->shadow_vel);
+		free(D);
+	}
+	free(F->omega);
+	free(F->omega_band);
+	free(F->psi);
+	free(F->vel);
+	memset(F, 0, sizeof(*F));
+}
+
+/* ---------------- host mirrors ---------------- */
+
+/* float host array -> device array of type T (through a staging buffer) */
+static void put_f32(struct ggb_fluid_dev *D, void *dst, const float *src, size_t count)
+{
+	if (D->T == GGBK_F32) {
+		upload(dst, src, 4 * count);
+		return;
+	}
+	float *tmp = ggb_dmalloc(4 * count, "staging");
+	upload(tmp, src, 4 * count);
+	ggbk_convert(dst, GGBK_F64, tmp, GGBK_F32, count);
+	ggb_sync();
+	ggbk_free(tmp);
+}
+
+static void get_f32(struct ggb_fluid_dev *D, float *dst, const void *src, size_t count)
+{
+	if (D->T == GGBK_F32) {
+		download(dst, src, 4 * count);
+		return;
+	}
+	float *tmp = ggb_dmalloc(4 * count, "staging");
+	ggbk_convert(tmp, GGBK_F32, src, GGBK_F64, count);
+	download(dst, tmp, 4 * count);
+	ggbk_free(tmp);
+}
+
+/* push caller-modified mirrors to the device (only arrays that changed) */
+static void mirrors_pull(Fluid *F)
+{
+	struct ggb_fluid_dev *D = F->dev;
+	size_t n = D->n;
+	if (memcmp(F->omega, D->shadow_omega, 4 * n)) {
+		put_f32(D, D->omega, F->omega, n);
+		memcpy(D->shadow_omega, F->omega, 4 * n);
+	}
+	if (memcmp(F->omega_band, D->shadow_band, 4 * n)) {
+		put_f32(D, D->omega_band, F->omega_band, n);
+		memcpy(D->shadow_band, F->omega_band, 4 * n);
+	}
+	if (memcmp(F->psi, D->shadow_psi, 4 * n)) {
+		put_f32(D, D->psi, F->psi, n);
+		memcpy(D->shadow_psi, F->psi, 4 * n);
+	}
+	if (memcmp(F->vel, D->shadow_vel, 12 * n)) {
+		put_f32(D, D->vel, F->vel, 3 * n);
+		memcpy(D->shadow_vel, F->vel, 12 * n);
+	}
+}
+
+void fluid_sync_host(Fluid *F)
+{
+	struct ggb_fluid_dev *D = F->dev;
+	size_t n = D->n;
+	get_f32(D, F->omega, D->omega, n);
+	get_f32(D, F->omega_band, D->omega_band, n);
+	get_f32(D, F->psi, D->psi, n);
+	get_f32(D, F->vel, D->vel, 3 * n);
+	memcpy(D->shadow_omega, F->omega, 4 * n);
+	memcpy(D->shadow_band, F->omega_band, 4 * n);
+	memcpy(D->shadow_psi, F->psi, 4 * n);
+	memcpy(D->shadow_vel, F->vel, 12 * n);
+}
+
+void fluid_sync_device(Fluid *F)
+{
+	struct ggb_fluid_dev *D = F->dev;
+	size_t n = D->n;
+	put_f32(D, D->omega, F->omega, n);
+	put_f32(D, D->omega_band, F->omega_band, n);
+	put_f32(D, D->psi, F->psi, n);
+	put_f32(D, D->vel, F->vel, 3 * n);
+	memcpy(D->shadow_omega, F->omega, 4 * n);
+	memcpy(D->shadow_band, F->omega_band, 4 * n);
+	memcpy(D->shadow_psi, F->psi, 4 * n);
+	memcpy(D->shadow_vel, F->vel, 12 * n);
+}
+
+static void *field_of(struct ggb_fluid_dev *D, int field, size_t *count)
+{
+	size_t n = D->n;
+	switch (field) {
+	case GGB_FIELD_OMEGA: *count = n; return D->omega;
+	case GGB_FIELD_OMEGA_BAND: *count = n; return D->omega_band;
+	case GGB_FIELD_PSI: *count = n; return D->psi;
+	case GGB_FIELD_VEL: *count = 3 * n; return D->vel;
+	case GGB_FIELD_DEP_B: *count = 3 * n; return D->dep_b;
+	case GGB_FIELD_DEP_F: *count = 3 * n; return D->dep_f;
+	default: *count = 0; return NULL;
+	}
+}
+
+int fluid_get_f64(const Fluid *F, int field, double *out)
+{
+	struct ggb_fluid_dev *D = F->dev;
+	size_t cnt;
+	void *src = field_of(D, field, &cnt);
+	if (!src)
+		return -1;
+	if (D->T == GGBK_F64) {
+		download(out, src, 8 * cnt);
+		return 0;
+	}
+	double *tmp = ggb_dmalloc(8 * cnt, "staging");
+	ggbk_convert(tmp, GGBK_F64, src, GGBK_F32, cnt);
+	download(out, tmp, 8 * cnt);
+	ggbk_free(tmp);
+	return 0;
+}
+
+int fluid_set_f64(Fluid *F, int field, const double *in)
+{
+	struct ggb_fluid_dev *D = F->dev;
+	size_t cnt;
+	void *dst = field_of(D, field, &cnt);
+	if (!dst)
+		return -1;
+	if (D->T == GGBK_F64) {
+		upload(dst, in, 8 * cnt);
+		return 0;
+	}
+	double *tmp = ggb_dmalloc(8 * cnt, "staging");
+	upload(tmp, in, 8 * cnt);
+	ggbk_convert(dst, GGBK_F32, tmp, GGBK_F64, cnt);
+	ggb_sync();
+	ggbk_free(tmp);
+	return 0;
+}
+
+/* ---------------- solver ---------------- */
+
+static void precondition(struct ggb_fluid_dev *D)
+{
+	if (D->nlev > 0)
+		ggbk_vcycle(D->levels, D->nlev, D->r, D->T, D->z, D->TM, D->s);
+	else
+		ggbk_jacobi_precond(D->g0, D->r, D->z, D->T, D->s);
+}
+
+static void read_scalars(struct ggb_fluid_dev *D)
+{
+	if (ggbk_d2h(D->hs, D->s, sizeof(ggbk_scalars)))
+		ggb_die("device error: %s", ggbk_last_error());
+}
+
+/* PCG on the device state (reference fluid.c:288-358). */
+static int solve_device(Fluid *F, int maxit)
+{
+	struct ggb_fluid_dev *D = F->dev;
+	ggbk_grid g = D->g0;
+	ggbk_cg_start(g, D->omega, D->psi, D->r, D->T, F->prm.cg_tol, maxit, D->s, D->partials);
+	precondition(D);
+	ggbk_cg_first_dir(g, D->r, D->z, D->TZ, D->p, D->T, D->s, D->partials);
+	int batch = D->predict > 1 ? D->predict - 1 : 1;
+	int launched = 0;
+	for (;;) {
+		for (int b = 0; b < batch; b++) {
+			ggbk_cg_Ap(g, D->p, D->Ap, D->T, D->s, D->partials);
+			ggbk_cg_update(g, D->psi, D->r, D->p, D->Ap, D->T, D->s, D->partials);
+			precondition(D);
+			ggbk_cg_next_dir(g, D->r, D->z, D->TZ, D->p, D->T, D->s, D->partials);
+		}
+		launched += batch;
+		read_scalars(D);
+		if (!D->hs->active || launched >= maxit)
+			break;
+		batch = 1;
+	}
+	ggbk_cg_zero_if_bb0(g, D->psi, D->T, D->s);
+	if (D->hs->status != 2)
+		ggbk_remove_mean(g, D->psi, D->T, D->s, D->partials);
+	F->cg_iters = D->hs->it;
+	F->cg_relres = D->hs->bb == 0.0 ? 0.0 : sqrt(D->hs->rr / D->hs->bb);
+	D->predict = D->hs->it > 0 ? D->hs->it : 1;
+	return D->hs->it;
+}
+
+int fluid_solve_psi_device(Fluid *F)
+{
+	int it = solve_device(F, F->prm.cg_maxit);
+	if (ggbk_check("solve"))
+		ggb_die("solve: %s", ggbk_last_error());
+	return it;
+}
+
+int fluid_solve_psi(Fluid *F)
+{
+	mirrors_pull(F);
+	int it = solve_device(F, F->prm.cg_maxit);
+	get_f32(F->dev, F->psi, F->dev->psi, F->dev->n);
+	memcpy(F->dev->shadow_psi, F->psi, 4 * (size_t) F->dev->n);
+	return it;
+}
+
+static void velocity_device(Fluid *F)
+{
+	struct ggb_fluid_dev *D = F->dev;
+	ggbk_velocity(D->g0, D->psi, D->vel, D->T, D->s, D->partials);
+}
+
+void fluid_compute_velocity(Fluid *F)
+{
+	mirrors_pull(F);
+	velocity_device(F);
+	get_f32(F->dev, F->vel, F->dev->vel, 3 * (size_t) F->dev->n);
+	memcpy(F->dev->shadow_vel, F->vel, 12 * (size_t) F->dev->n);
+}
+
+void fluid_set_state(Fluid *F, const float *vel_initial, const float *vel_band)
+{
+	struct ggb_fluid_dev *D = F->dev;
+	ggbk_grid g = D->g0;
+	size_t n = D->n;
+	void *tmp = ggb_dmalloc(3 * D->tsz * n, "set_state");
+	put_f32(D, tmp, vel_initial, 3 * n);
+	ggbk_curl(g, tmp, D->omega, D->T);
+	ggbk_remove_mean(g, D->omega, D->T, D->s, D->partials);
+	if (vel_band) {
+		put_f32(D, tmp, vel_band, 3 * n);
+		ggbk_curl(g, tmp, D->omega_band, D->T);
+		ggbk_remove_mean(g, D->omega_band, D->T, D->s, D->partials);
+	} else {
+		ggbk_zero(D->omega_band, D->tsz * n);
+	}
+	ggb_sync();
+	ggbk_free(tmp);
+	ggbk_zero(D->psi, D->tsz * n);
+	int save = F->prm.cg_maxit;
+	int cold = save * 20 > 20000 ? save * 20 : 20000;
+	D->predict = 1;
+	solve_device(F, cold);
+	D->predict = 1;
+	velocity_device(F);
+	fluid_sync_host(F);
+	if (ggbk_check("fluid_set_state"))
+		ggb_die("fluid_set_state: %s", ggbk_last_error());
+}
+
+void fluid_step(Fluid *F, double dt)
+{
+	struct ggb_fluid_dev *D = F->dev;
+	ggbk_grid g = D->g0;
+	int T = D->T;
+	double twoOmega = 2.0 * F->prm.rotation;
+	int bfecc = F->prm.bfecc;
+	ggbk_departures(g, D->vel, D->dep_b, D->dep_f, T, dt, bfecc);
+	ggbk_absvort(g, D->omega, D->q0, T, twoOmega);
+	void *qn = D->q1;
+	if (!bfecc) {
+		ggbk_advect(g, D->q0, D->dep_b, D->q1, T);
+	} else {
+		ggbk_advect(g, D->q0, D->dep_b, D->q1, T);
+		ggbk_advect(g, D->q1, D->dep_f, D->q2, T);
+		ggbk_bfecc_source(D->n, D->q0, D->q2, T);
+		ggbk_limited_advect(g, D->q2, D->q0, D->dep_b, D->q1, T);
+	}
+	double relax = F->prm.forcing > 0 ? 1.0 - exp(-F->prm.forcing * dt) : 0.0;
+	ggbk_relax(g, qn, D->omega_band, D->omega, T, twoOmega, relax);
+	ggbk_remove_mean(g, D->omega, T, D->s, D->partials);
+	solve_device(F, F->prm.cg_maxit);
+	velocity_device(F);
+	if (ggbk_check("fluid_step"))
+		ggb_die("fluid_step: %s", ggbk_last_error());
+}
+
+double fluid_max_speed(const Fluid *F)
+{
+	struct ggb_fluid_dev *D = F->dev;
+	ggbk_max_speed2(D->g0, D->vel, D->T, D->s, D->partials);
+	read_scalars(D);
+	return sqrt(D->hs->vmax2);
+}
+
+/* ||b - K psi|| / ||b|| in double on the device state. */
+double fluid_true_relres(Fluid *F)
+{
+	struct ggb_fluid_dev *D = F->dev;
+	size_t n = D->n;
+	double *kpsi = ggb_dmalloc(8 * n, "relres");
+	ggbk_applyK(D->g0, D->psi, kpsi, D->T);
+	double *hk = malloc(8 * n), *hw = malloc(8 * n);
+	float *area = malloc(4 * n);
+	if (!hk || !hw || !area)
+		ggb_die("out of memory");
+	download(hk, kpsi, 8 * n);
+	ggbk_free(kpsi);
+	fluid_get_f64(F, GGB_FIELD_OMEGA, hw);
+	download(area, D->g0.area, 4 * n);
+	double rr = 0, bb = 0;
+	for (size_t i = 0; i < n; i++) {
+		double b = -(double) area[i] * hw[i];
+		double r = b - hk[i];
+		rr += r * r;
+		bb += b * b;
+	}
+	free(hk);
+	free(hw);
+	free(area);
+	return bb > 0 ? sqrt(rr / bb) : 0.0;
+}
+
+int fluid_vcycle_f64(Fluid *F, const double *b, double *x)
+{
+	struct ggb_fluid_dev *D = F->dev;
+	if (D->nlev < 1)
+		return -1;
+	size_t n = D->n;
+	void *db = ggb_dmalloc(D->tsz * n, "vcycle");
+	void *dz = ggb_dmalloc(D->tmsz * n, "vcycle");
+	double *tmp = ggb_dmalloc(8 * n, "vcycle");
+	upload(tmp, b, 8 * n);
+	ggbk_convert(db, D->T, tmp, GGBK_F64, n);
+	/* force the loop flag on for a stand-alone V-cycle */
+	ggbk_scalars hs;
+	memset(&hs, 0, sizeof(hs));
+	hs.active = 1;
+	ggbk_scalars *ds = ggb_dmalloc(sizeof(ggbk_scalars), "vcycle");
+	upload(ds, &hs, sizeof(hs));
+	ggbk_vcycle(D->levels, D->nlev, db, D->T, dz, D->TM, ds);
+	ggbk_convert(tmp, GGBK_F64, dz, D->TM, n);
+	download(x, tmp, 8 * n);
+	ggbk_free(db);
+	ggbk_free(dz);
+	ggbk_free(tmp);
+	ggbk_free(ds);
+	return ggbk_check("vcycle") ? -1 : 0;
+}
+
+/* ---------------- free operators (host arrays, reference semantics) ---------------- */
+
+typedef void (*op_fn)(ggbk_grid, const void *, void *, int);
+
+static void host_op(const CubeSphere *cs, op_fn fn, const void *in, size_t nin, void *out, size_t nout, size_t esz,
+		    int T)
+{
+	ggbk_grid g = ggb_grid_dev(cs);
+	void *din = ggb_dmalloc(esz * nin, "operator");
+	void *dout = ggb_dmalloc(esz * nout, "operator");
+	upload(din, in, esz * nin);
+	fn(g, din, dout, T);
+	download(out, dout, esz * nout);
+	ggbk_free(din);
+	ggbk_free(dout);
+}
+
+void fluid_curl(const CubeSphere *cs, const float *vel, float *omega)
+{
+	host_op(cs, ggbk_curl, vel, 3 * (size_t) cs->ncells, omega, cs->ncells, 4, GGBK_F32);
+}
+
+void fluid_divergence(const CubeSphere *cs, const float *vel, float *div)
+{
+	host_op(cs, ggbk_divergence, vel, 3 * (size_t) cs->ncells, div, cs->ncells, 4, GGBK_F32);
+}
+
+void fluid_laplacian(const CubeSphere *cs, const float *x, float *y)
+{
+	host_op(cs, ggbk_laplacian, x, cs->ncells, y, cs->ncells, 4, GGBK_F32);
+}
+
+int ggb_laplacian_f64(const CubeSphere *cs, const double *x, double *y)
+{
+	host_op(cs, ggbk_laplacian, x, cs->ncells, y, cs->ncells, 8, GGBK_F64);
+	return 0;
+}
+
+int ggb_curl_f64(const CubeSphere *cs, const double *vel, double *out)
+{
+	host_op(cs, ggbk_curl, vel, 3 * (size_t) cs->ncells, out, cs->ncells, 8, GGBK_F64);
+	return 0;
+}
+
+int ggb_divergence_f64(const CubeSphere *cs, const double *vel, double *out)
+{
+	host_op(cs, ggbk_divergence, vel, 3 * (size_t) cs->ncells, out, cs->ncells, 8, GGBK_F64);
+	return 0;
+}
+
+int ggb_applyK_f64(const CubeSphere *cs, const double *x, double *y)
+{
+	ggbk_grid g = ggb_grid_dev(cs);
+	size_t n = cs->ncells;
+	void *din = ggb_dmalloc(8 * n, "operator");
+	double *dout = ggb_dmalloc(8 * n, "operator");
+	upload(din, x, 8 * n);
+	ggbk_applyK(g, din, dout, GGBK_F64);
+	download(y, dout, 8 * n);
+	ggbk_free(din);
+	ggbk_free(dout);
+	return 0;
+}
+
+void fluid_remove_mean(const CubeSphere *cs, float *f)
+{
+	ggbk_grid g = ggb_grid_dev(cs);
+	size_t n = cs->ncells;
+	float *d = ggb_dmalloc(4 * n, "remove_mean");
+	ggbk_scalars *s = ggb_dmalloc(sizeof(ggbk_scalars), "remove_mean");
+	double *part = ggb_dmalloc(sizeof(double) * ((n + 255) / 256 + 1), "remove_mean");
+	ggbk_zero(s, sizeof(ggbk_scalars));
+	upload(d, f, 4 * n);
+	ggbk_remove_mean(g, d, GGBK_F32, s, part);
+	download(f, d, 4 * n);
+	ggbk_free(d);
+	ggbk_free(s);
+	ggbk_free(part);
+}

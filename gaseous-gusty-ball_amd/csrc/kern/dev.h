@@ -1,0 +1,312 @@
+// dev.h -- device-side math shared by the gfx950 kernels: double 3-vectors, the
+// great-circle step, cubed-sphere location and bilinear taps, and deterministic
+// block reductions (wave64 shuffles, then LDS, fixed order).
+//
+// The vector helpers keep the reference's operation order (src/vec3.h:9-42; e.g.
+// normalise = multiply by the reciprocal length) and every kernel is compiled with
+// -ffp-contract=off, so the only source of difference from the CPU path is the device
+// libm (atan, atan2, sin, cos), which is not bit-identical to glibc.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ggbk.h"
+
+#define GGB_PI 3.14159265358979323846
+
+struct dv3 {
+	double x, y, z;
+};
+
+__device__ __forceinline__ dv3 d3(double x, double y, double z)
+{
+	dv3 r;
+	r.x = x;
+	r.y = y;
+	r.z = z;
+	return r;
+}
+__device__ __forceinline__ dv3 vadd(dv3 a, dv3 b) { return d3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ dv3 vsub(dv3 a, dv3 b) { return d3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ dv3 vscl(dv3 a, double s) { return d3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ double vdot(dv3 a, dv3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ dv3 vcross(dv3 a, dv3 b)
+{
+	return d3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double vlen(dv3 a) { return sqrt(vdot(a, a)); }
+__device__ __forceinline__ dv3 vnorm(dv3 a)
+{
+	double l = vlen(a);
+	return l > 0.0 ? vscl(a, 1.0 / l) : a;
+}
+__device__ __forceinline__ dv3 vtan(dv3 v, dv3 n) { return vsub(v, vscl(n, vdot(v, n))); }
+__device__ __forceinline__ double vangle(dv3 a, dv3 b) { return atan2(vlen(vcross(a, b)), vdot(a, b)); }
+
+template <typename T>
+__device__ __forceinline__ dv3 ld3(const T *p)
+{
+	return d3((double) p[0], (double) p[1], (double) p[2]);
+}
+template <typename T>
+__device__ __forceinline__ void st3(T *p, dv3 a)
+{
+	p[0] = (T) a.x;
+	p[1] = (T) a.y;
+	p[2] = (T) a.z;
+}
+
+// Great-circle move (reference vec3.h:33-42).
+__device__ __forceinline__ dv3 gc_move(dv3 p, dv3 v, double s)
+{
+	dv3 t = vtan(v, p);
+	double sp = vlen(t);
+	if (sp < 1e-300)
+		return p;
+	double th = sp * s;
+	double sn, cs;
+	sincos(th, &sn, &cs);
+	return vnorm(vadd(vscl(p, cs), vscl(t, sn / sp)));
+}
+
+// Face of direction d: largest |component|, ties to x then y (reference cubesphere.c:12-20).
+__device__ __forceinline__ int face_of(dv3 d)
+{
+	double ax = fabs(d.x), ay = fabs(d.y), az = fabs(d.z);
+	if (ax >= ay && ax >= az)
+		return d.x > 0 ? 1 : 3;
+	if (ay >= az)
+		return d.y > 0 ? 4 : 5;
+	return d.z > 0 ? 0 : 2;
+}
+
+// Gnomonic coordinates on face f: dc = d.centre, u = d.right / dc, v = d.up / dc.  The
+// face basis vectors have one +-1 entry, so the dot products are single components
+// (bitwise equal to the full dot products for dc != 0).
+__device__ __forceinline__ void gnomonic(int f, dv3 d, double &dc, double &u, double &v)
+{
+	double cr, cu;
+	switch (f) {
+	case 0: dc = d.z; cr = d.x; cu = d.y; break;
+	case 1: dc = d.x; cr = -d.z; cu = d.y; break;
+	case 2: dc = -d.z; cr = -d.x; cu = d.y; break;
+	case 3: dc = -d.x; cr = d.z; cu = d.y; break;
+	case 4: dc = d.y; cr = d.x; cu = -d.z; break;
+	default: dc = -d.y; cr = d.x; cu = d.z; break;
+	}
+	u = cr / dc;
+	v = cu / dc;
+}
+
+// Dot product of d with the centre of face f (any f; used by the splat face test).
+__device__ __forceinline__ double face_dot(int f, dv3 d)
+{
+	switch (f) {
+	case 0: return d.z;
+	case 1: return d.x;
+	case 2: return -d.z;
+	case 3: return -d.x;
+	case 4: return d.y;
+	default: return -d.y;
+	}
+}
+
+// Continuous cell coordinates (reference cubesphere.c:29-38).
+__device__ __forceinline__ void locate(double h, dv3 d, int &f, double &x, double &y)
+{
+	f = face_of(d);
+	double dc, u, v;
+	gnomonic(f, d, dc, u, v);
+	x = (atan(u) + GGB_PI / 4) / h - 0.5;
+	y = (atan(v) + GGB_PI / 4) / h - 0.5;
+}
+
+// Four bilinear taps at p (reference cubesphere.c:68-92).  Cells off the face resolve
+// through the ghost table built on the host with the reference's rule.
+struct Taps {
+	int idx[4];
+	double fx, fy;
+};
+
+__device__ __forceinline__ Taps taps_at(const ggbk_grid &g, dv3 p)
+{
+	Taps t;
+	int f;
+	double x, y;
+	locate(g.h, p, f, x, y);
+	int i0 = (int) floor(x), j0 = (int) floor(y);
+	t.fx = x - i0;
+	t.fy = y - j0;
+	int N = g.N;
+	if (i0 >= 0 && j0 >= 0 && i0 + 1 < N && j0 + 1 < N) {
+		int b = (f * N + j0) * N + i0;
+		t.idx[0] = b;
+		t.idx[1] = b + 1;
+		t.idx[2] = b + N;
+		t.idx[3] = b + N + 1;
+	} else {
+		int P = N + 2;
+		const int *gt = g.ghost + (size_t) f * P * P;
+		int r0 = (j0 + 1) * P + (i0 + 1);
+		t.idx[0] = gt[r0];
+		t.idx[1] = gt[r0 + 1];
+		t.idx[2] = gt[r0 + P];
+		t.idx[3] = gt[r0 + P + 1];
+	}
+	return t;
+}
+
+__device__ __forceinline__ void tap_weights(const Taps &t, double wt[4])
+{
+	wt[0] = (1 - t.fx) * (1 - t.fy);
+	wt[1] = t.fx * (1 - t.fy);
+	wt[2] = (1 - t.fx) * t.fy;
+	wt[3] = t.fx * t.fy;
+}
+
+template <typename T>
+__device__ __forceinline__ double sample_scalar(const ggbk_grid &g, const T *fld, dv3 p)
+{
+	Taps t = taps_at(g, p);
+	double wt[4];
+	tap_weights(t, wt);
+	return wt[0] * (double) fld[t.idx[0]] + wt[1] * (double) fld[t.idx[1]] +
+	       wt[2] * (double) fld[t.idx[2]] + wt[3] * (double) fld[t.idx[3]];
+}
+
+template <typename T>
+__device__ __forceinline__ dv3 sample_vec(const ggbk_grid &g, const T *fld3, dv3 p)
+{
+	Taps t = taps_at(g, p);
+	double wt[4];
+	tap_weights(t, wt);
+	dv3 r = d3(0, 0, 0);
+	for (int k = 0; k < 4; k++)
+		r = vadd(r, vscl(ld3(fld3 + 3 * (size_t) t.idx[k]), wt[k]));
+	return vtan(r, p);
+}
+
+__device__ __forceinline__ dv3 cell_pos(const ggbk_grid &g, int i) { return ld3(g.pos + 3 * (size_t) i); }
+
+// ---------------- deterministic reductions ----------------
+
+// Wave64 tree sum via DPP-free shuffles; every lane returns the wave total.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1)
+		v += __shfl_xor(v, off, 64);
+	return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1)
+		v = fmax(v, __shfl_xor(v, off, 64));
+	return v;
+}
+
+// Block sum in a fixed order: wave totals, then wave 0 folds them.  Result valid in
+// thread 0.  BS = block size (multiple of 64, <= 1024).
+template <int BS>
+__device__ __forceinline__ double block_sum(double v)
+{
+	__shared__ double wsum[BS / 64];
+	v = wave_sum(v);
+	int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	if (lane == 0)
+		wsum[wid] = v;
+	__syncthreads();
+	double r = 0;
+	if (threadIdx.x == 0) {
+		for (int k = 0; k < BS / 64; k++)
+			r += wsum[k];
+	}
+	__syncthreads();
+	return r;
+}
+
+template <int BS>
+__device__ __forceinline__ double block_max(double v)
+{
+	__shared__ double wmax[BS / 64];
+	v = wave_max(v);
+	int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	if (lane == 0)
+		wmax[wid] = v;
+	__syncthreads();
+	double r = 0;
+	if (threadIdx.x == 0) {
+		for (int k = 0; k < BS / 64; k++)
+			r = fmax(r, wmax[k]);
+	}
+	__syncthreads();
+	return r;
+}
+
+// Last-block election: every block has stored its partial (thread 0) before the call.
+// Returns true in all threads of the block that arrives last; the ticket is reset for
+// the next use.  Device-scope fences make the other blocks' partials (written through
+// other XCDs' L2s) visible.
+__device__ __forceinline__ bool last_block(unsigned int *ticket)
+{
+	__shared__ int is_last;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		__threadfence();
+		unsigned int t = atomicAdd(ticket, 1u);
+		is_last = (t == gridDim.x - 1);
+	}
+	__syncthreads();
+	if (is_last) {
+		__threadfence();
+		if (threadIdx.x == 0)
+			__hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	}
+	return is_last != 0;
+}
+
+__device__ __forceinline__ double load_coherent(const double *p)
+{
+	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Fold nb partials in block order (thread 0 result; the fold is sequential in index
+// order within chunks of BS, then chunks in order -- fixed for a given nb).
+template <int BS>
+__device__ __forceinline__ double fold_partials(const double *part, int nb)
+{
+	double acc = 0;
+	for (int base = 0; base < nb; base += BS) {
+		int i = base + threadIdx.x;
+		double v = i < nb ? load_coherent(part + i) : 0.0;
+		acc += block_sum<BS>(v);
+	}
+	return acc;
+}
+
+template <int BS>
+__device__ __forceinline__ double fold_partials_max(const double *part, int nb)
+{
+	double acc = 0;
+	for (int base = 0; base < nb; base += BS) {
+		int i = base + threadIdx.x;
+		double v = i < nb ? load_coherent(part + i) : 0.0;
+		acc = fmax(acc, block_max<BS>(v));
+	}
+	return acc;
+}
+
+// Kernel launch error check helper for host launchers.
+#define GGBK_LAUNCH(kern, grid, block, shm, ...)                                           \
+	hipLaunchKernelGGL(kern, dim3(grid), dim3(block), shm, ggbk_stream(), __VA_ARGS__)
+
+hipStream_t ggbk_stream(void);
+
+static inline unsigned int ggbk_blocks(long long n, int bs)
+{
+	long long b = (n + bs - 1) / bs;
+	return (unsigned int) (b < 1 ? 1 : b);
+}

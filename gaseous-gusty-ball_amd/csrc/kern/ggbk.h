@@ -1,0 +1,171 @@
+/*
+ * ggbk.h -- the thin C ABI between the host C code and the gfx950 HIP kernels.
+ *
+ * Plain C structs of device pointers and sizes, passed by value; every launcher
+ * enqueues on the library's one stream and returns immediately unless its name says
+ * it synchronises.  Device arrays use "column-major ELL" for the Voronoi stencil:
+ * slot k of cell i lives at [k * n + i], so the threads of a wave read consecutive
+ * words for every slot.
+ *
+ * Precision codes: the state type T and the multigrid type TM are selected per call by
+ * an int (GGBK_F64 / GGBK_F32).
+ */
+#ifndef GGBK_H
+#define GGBK_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { GGBK_F64 = 0, GGBK_F32 = 1 };
+
+/* one grid level on the device */
+typedef struct {
+	int N, n, maxnb, pad_;
+	double h, total_area;
+	const float *pos;         /* [3n] AoS cell centres */
+	const float *area;        /* [n] */
+	const unsigned char *nnb; /* [n] */
+	const int *nbr;           /* [maxnb][n] */
+	const float *w;           /* [maxnb][n] */
+	const float *elen;        /* [maxnb][n] */
+	const float *gc;          /* [maxnb][3][n] */
+	const int *ghost;         /* [6][N+2][N+2] tap index of (f, i, j), i, j in [-1, N] */
+	const float *dinv;        /* [n] 1 / sum(w), rounded to float */
+} ggbk_grid;
+
+/* bilinear prolongation from level l+1 to l, and its transpose */
+typedef struct {
+	int nf, nc;
+	const int *ptap;   /* [4][nf] coarse taps of each fine cell */
+	const float *pw;   /* [4][nf] */
+	const int *rstart; /* [nc+1] CSR rows of R = P^T, fine index ascending */
+	const int *ridx;   /* [4 nf] */
+	const float *rw;   /* [4 nf] */
+} ggbk_xfer;
+
+/* Device-resident solver scalars.  Reductions write per-block partials; the last block
+ * to finish folds them in block order (deterministic) and updates these. */
+typedef struct {
+	double bb, rr, rz, pAp, alpha, beta, tol2;
+	double rznew;
+	double sum;        /* result of the last generic sum */
+	double vmax2;      /* max |vel|^2 of the last velocity pass */
+	double mean;       /* area-weighted mean of the last remove-mean */
+	int it, maxit;
+	int active;        /* CG loop running */
+	int status;        /* 0 running/converged, 1 pAp <= 0, 2 bb == 0, 3 maxit */
+	unsigned int ticket[8];
+} ggbk_scalars;
+
+enum { GGBK_TICKETS = 8 };
+
+/* ---- runtime ---- */
+int ggbk_init(void);                       /* select device 0, create the stream; 0 = ok */
+const char *ggbk_last_error(void);
+void *ggbk_malloc(size_t bytes);           /* NULL on failure */
+void ggbk_free(void *p);
+int ggbk_h2d(void *dst, const void *src, size_t bytes);      /* synchronous */
+int ggbk_d2h(void *dst, const void *src, size_t bytes);      /* synchronous */
+int ggbk_d2h_async(void *dst, const void *src, size_t bytes); /* ordered on the stream */
+int ggbk_d2d(void *dst, const void *src, size_t bytes);
+int ggbk_zero(void *p, size_t bytes);      /* fill kernel (graph-safe, no memset node) */
+int ggbk_sync(void);
+void *ggbk_host_alloc(size_t bytes);       /* pinned */
+void ggbk_host_free(void *p);
+int ggbk_check(const char *where);         /* reports a pending launch error */
+/* stream timing */
+void *ggbk_event_new(void);
+void ggbk_event_record(void *ev);
+float ggbk_event_ms(void *a, void *b);     /* synchronises on b */
+void ggbk_event_free(void *ev);
+void *ggbk_stream_handle(void);
+
+/* ---- conversions ---- */
+void ggbk_convert(void *dst, int tdst, const void *src, int tsrc, size_t count);
+
+/* ---- free-standing operators (reference fluid.c:211-275) ---- */
+void ggbk_curl(ggbk_grid g, const void *vel, void *out, int T);
+void ggbk_divergence(ggbk_grid g, const void *vel, void *out, int T);
+void ggbk_laplacian(ggbk_grid g, const void *x, void *y, int T);
+void ggbk_applyK(ggbk_grid g, const void *x, double *y, int T);
+/* f -= area-weighted mean (mean rounded to T, as the reference rounds to float) */
+void ggbk_remove_mean(ggbk_grid g, void *f, int T, ggbk_scalars *s, double *partials);
+
+/* ---- PCG (reference fluid.c:288-358) ---- */
+/* r = b - K psi with b = -area*omega; bb, rr; sets tol2, it = 0, active, status */
+void ggbk_cg_start(ggbk_grid g, const void *omega, const void *psi, void *r, int T,
+		   double tol, int maxit, ggbk_scalars *s, double *partials);
+/* p = z; rz = r.z  (guarded by active) */
+void ggbk_cg_first_dir(ggbk_grid g, const void *r, const void *z, int TZ, void *p, int T,
+		       ggbk_scalars *s, double *partials);
+/* Ap = K p; pAp; alpha = rz / pAp or stop on pAp <= 0 */
+void ggbk_cg_Ap(ggbk_grid g, const void *p, void *Ap, int T, ggbk_scalars *s, double *partials);
+/* psi += alpha p; r -= alpha Ap; rr; it++; stop on rr <= tol2 or it == maxit */
+void ggbk_cg_update(ggbk_grid g, void *psi, void *r, const void *p, const void *Ap, int T,
+		    ggbk_scalars *s, double *partials);
+/* rznew = r.z; beta = rznew / rz; rz = rznew; p = z + beta p */
+void ggbk_cg_next_dir(ggbk_grid g, const void *r, const void *z, int TZ, void *p, int T,
+		      ggbk_scalars *s, double *partials);
+/* Jacobi preconditioner z = dinv r (no multigrid) */
+void ggbk_jacobi_precond(ggbk_grid g, const void *r, void *z, int T, const ggbk_scalars *s);
+/* bb == 0 exit: psi = 0 when status says so */
+void ggbk_cg_zero_if_bb0(ggbk_grid g, void *psi, int T, const ggbk_scalars *s);
+
+/* ---- multigrid V-cycle (reference fluid.c:100-146) ---- */
+/* Level-0 input r (type T) -> output z (type TM), all intermediate arrays of type TM.
+ * levels[l].x/b/r are the per-level TM scratch; level 0's b is the input r. */
+typedef struct {
+	ggbk_grid g;
+	ggbk_xfer xf;     /* transfer to the next coarser level (unused on the coarsest) */
+	void *x, *b, *r;  /* TM scratch (level 0: x = z output, r = residual scratch) */
+} ggbk_level;
+void ggbk_vcycle(const ggbk_level *levels, int nlev, const void *rin, int T, void *zout, int TM,
+		 const ggbk_scalars *s);
+
+/* ---- velocity (reference fluid.c:360-376) + max speed (fluid.c:480-491) ---- */
+void ggbk_velocity(ggbk_grid g, const void *psi, void *vel, int T, ggbk_scalars *s, double *partials);
+/* max |vel|^2 into s->vmax2 */
+void ggbk_max_speed2(ggbk_grid g, const void *vel, int T, ggbk_scalars *s, double *partials);
+
+/* ---- semi-Lagrangian step (reference fluid.c:397-478) ---- */
+void ggbk_departures(ggbk_grid g, const void *vel, void *dep_b, void *dep_f, int T, double dt, int fwd);
+void ggbk_absvort(ggbk_grid g, const void *omega, void *q, int T, double twoOmega);
+void ggbk_advect(ggbk_grid g, const void *src, const void *dep, void *dst, int T);
+void ggbk_bfecc_source(int n, const void *q, void *q2, int T);
+void ggbk_limited_advect(ggbk_grid g, const void *q2, const void *q, const void *dep_b, void *q1, int T);
+void ggbk_relax(ggbk_grid g, const void *qn, const void *omega_band, void *omega, int T,
+		double twoOmega, double relax);
+
+/* ---- particles (reference particles.c) ---- */
+void ggbk_particles_init(int n, double *pos, double *prev, float *col, int round32);
+void ggbk_particles_color(int n, const double *pos, float *col, const float *img, int w, int h);
+void ggbk_particles_advect(ggbk_grid g, int np, double *pos, double *prev, const void *vel, int T,
+			   double dt, int round32);
+
+/* ---- renderer (reference render.c) ---- */
+typedef struct {
+	int R, tile, ntx, ntiles;   /* tile edge in texels, tiles per face edge, 6*ntx*ntx */
+	float radius, alpha;
+	float *buf;                 /* [6][R][R][4] */
+	int *tile_count;            /* [ntiles + 1] */
+	int *tile_start;            /* [ntiles + 1] */
+	int *tile_cursor;           /* [ntiles] */
+	int *list;                  /* [cap] particle indices per tile */
+	long long cap;
+	int *sort_scratch;          /* [scratch_cap] for tiles whose list exceeds the LDS sort */
+	long long scratch_cap;
+	long long *total;           /* [1] entries of the last deposit (device) */
+} ggbk_render;
+void ggbk_render_scale(float *buf, size_t count, float g);
+/* bin + ordered blend; returns the number of (particle, tile) entries, or -1 on error */
+long long ggbk_render_deposit(ggbk_render *r, int np, const double *prev, const double *pos,
+			      const float *col, float invg);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,72 @@
+/* Extensions to the reference API (no counterpart in the reference headers).
+ *
+ * The reference keeps all state in host float arrays; this library keeps it on the GPU
+ * (in double by default).  These calls move state between the device and the host
+ * mirrors, expose the full-precision state for verification, report device timings,
+ * and carry the discrete adjoint.
+ */
+#ifndef GGB_EXT_H
+#define GGB_EXT_H
+
+#include "fluid.h"
+#include "particles.h"
+#include "render.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Fields addressed by fluid_get_f64 / fluid_set_f64. */
+enum {
+	GGB_FIELD_OMEGA = 0,      /* [n] */
+	GGB_FIELD_OMEGA_BAND = 1, /* [n] */
+	GGB_FIELD_PSI = 2,        /* [n] */
+	GGB_FIELD_VEL = 3,        /* [3n] */
+	GGB_FIELD_DEP_B = 4,      /* [3n] backward departure points of the last step */
+	GGB_FIELD_DEP_F = 5,      /* [3n] forward departure points of the last step */
+};
+
+/* Selects the GPU and creates the library stream; 0 on success.  Called implicitly by
+ * every *_init; fails (-1) when no GPU is present. */
+int ggb_device_init(void);
+/* Library version string. */
+const char *ggb_version(void);
+/* Wait for all queued device work. */
+void ggb_sync(void);
+
+/* ---- fluid ---- */
+void fluid_sync_host(Fluid *F);   /* device state -> F.omega/omega_band/psi/vel mirrors */
+void fluid_sync_device(Fluid *F); /* mirrors -> device state (rounds FP64 state to float) */
+int fluid_get_f64(const Fluid *F, int field, double *out);
+int fluid_set_f64(Fluid *F, int field, const double *in);
+/* PCG on the device state (no mirror traffic); returns iterations. */
+int fluid_solve_psi_device(Fluid *F);
+/* True relative residual ||b - K psi|| / ||b|| (b = -A omega) of the device state, in FP64. */
+double fluid_true_relres(Fluid *F);
+/* V-cycle preconditioner applied to b (double[n]) -> x (double[n]); for verification. */
+int fluid_vcycle_f64(Fluid *F, const double *b, double *x);
+
+/* ---- free operators in double (device), for verification ---- */
+int ggb_laplacian_f64(const CubeSphere *cs, const double *x, double *y);
+int ggb_applyK_f64(const CubeSphere *cs, const double *x, double *y);
+int ggb_curl_f64(const CubeSphere *cs, const double *vel, double *out);
+int ggb_divergence_f64(const CubeSphere *cs, const double *vel, double *out);
+
+/* ---- particles ---- */
+void particles_sync_host(Particles *P); /* device -> pos/prev mirrors */
+int particles_get_f64(const Particles *P, double *pos, double *prev);
+int particles_set_f64(Particles *P, const double *pos, const double *prev);
+/* 1 = round stored positions to float (the reference's storage); 0 = double (default) */
+void particles_set_round32(Particles *P, int on);
+
+/* ---- render ---- */
+void render_sync_host(Render *r);             /* device buffer -> r->buf mirror */
+long long render_last_entries(const Render *r); /* (particle, tile) entries of the last deposit */
+/* 8-bit faces [6][R][R][4] (bg NULL: un-premultiplied RGBA) computed on the host mirror. */
+void render_faces8(Render *r, const float *bg, unsigned char *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
