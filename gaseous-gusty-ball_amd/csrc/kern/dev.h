@@ -135,10 +135,14 @@ __device__ __forceinline__ Taps taps_at(const ggbk_grid &g, dv3 p)
 	int f;
 	double x, y;
 	locate(g.h, p, f, x, y);
-	int i0 = (int) floor(x), j0 = (int) floor(y);
+	int N = g.N;
+	// x, y lie in [-0.5, N - 0.5] for any unit vector; the clamp only keeps a
+	// non-finite position from indexing outside the ghost table
+	double xf = fmin(fmax(floor(x), -1.0), (double) (N - 1));
+	double yf = fmin(fmax(floor(y), -1.0), (double) (N - 1));
+	int i0 = (int) xf, j0 = (int) yf;
 	t.fx = x - i0;
 	t.fy = y - j0;
-	int N = g.N;
 	if (i0 >= 0 && j0 >= 0 && i0 + 1 < N && j0 + 1 < N) {
 		int b = (f * N + j0) * N + i0;
 		t.idx[0] = b;

@@ -14,7 +14,7 @@
 #define BS 256
 #define MG_OMEGA 0.8
 #define MG_COARSE_SWEEPS 60
-#define COARSE_LDS_CELLS 4096
+#define COARSE_LDS_CELLS 3072
 #define COARSE_BS 1024
 
 template <typename TX>
