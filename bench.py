@@ -1,0 +1,234 @@
+"""Benchmark of the per-step hot path on one MI355X.
+
+Workload (default): N=512 cubed sphere (1,572,864 cells), 4,000,000 particles, 2048^2
+faces, forcing 0.5, rotation 10, CG tolerance 1e-8, mixed precision (FP32 V-cycle in
+FP64 CG), 8 bands + 4-octave curl noise from seed 1, cfl 1, fade 0.02, trail alpha 0.35,
+splat radius 1.  A step is the driver loop of the reference (main.c:421-435): max speed
+-> dt, particles_advect, fluid_step, render_fade + render_deposit.
+
+Reports (one JSON line): fluid steps/s (mixed; the headline `value`), Msplat-particles/s,
+the FP64 path's fluid steps/s on the same workload, the true FP64 relative residual of
+both, the roofline of the dominant kernel (HIP events on the library stream over the
+timed steps), and the CPU baseline timed on this host.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gaseous-gusty-ball_amd"))
+
+import ggb  # noqa: E402
+
+CONFIGS = {
+    # name: N, particles, R, forcing, rotation, cg_tol, precision, octaves
+    "c2": dict(N=512, P=4_000_000, R=2048, forcing=0.5, rotation=10.0, cg_tol=1e-8,
+               precision=ggb.PREC_MIXED, octaves=4),
+    "c1": dict(N=256, P=1_000_000, R=1024, forcing=0.0, rotation=0.0, cg_tol=1e-4,
+               precision=ggb.PREC_FP64, octaves=4),
+    "c0": dict(N=64, P=100_000, R=256, forcing=0.0, rotation=0.0, cg_tol=1e-4,
+               precision=ggb.PREC_FP64, octaves=4),
+}
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def setup(cfg, precision, with_particles=True, seed=1):
+    g = ggb.Grid(cfg["N"])
+    vel0, vband = g.initial_velocity(seed=seed, octaves=cfg["octaves"])
+    F = ggb.Fluid(g, forcing=cfg["forcing"], rotation=cfg["rotation"], cg_tol=cfg["cg_tol"],
+                  cg_maxit=500, precision=precision)
+    F.set_state(vel0, vband if cfg["forcing"] > 0 else None)
+    P = R = None
+    if with_particles:
+        P = ggb.Particles(cfg["P"])
+        P.color_from_image(ggb.strip_to_float(ggb.banded_strip(seed)), 64, 1024)
+        R = ggb.Render(cfg["R"], 1.0, 0.35)
+    return g, F, P, R
+
+
+def run_steps(g, F, P, R, nsteps, timers=None, acc=None):
+    """Driver loop; per-phase device time from HIP events on the library stream."""
+    for _ in range(nsteps):
+        vmax = F.max_speed()
+        dt = 1.0 * g.h / max(vmax, 1e-6)
+        if P is not None:
+            timers["p"].start()
+            P.advect(g, F, dt)
+            acc["p"] += timers["p"].stop_ms()
+        timers["f"].start()
+        F.step(dt)
+        acc["f"] += timers["f"].stop_ms()
+        acc["its"] += F.cg_iters
+        if R is not None:
+            timers["r"].start()
+            R.fade(0.02)
+            R.deposit(P)
+            acc["r"] += timers["r"].stop_ms()
+            acc["entries"] += R.last_entries()
+
+
+def cpu_baseline(cfg, steps):
+    """The FP64 restatement (fluid) and the reference's own optimised build (particles and
+    render), on this host's cores, over a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ggb_oracle as O
+
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = min(cores, 16)
+    O.set_omp_threads(cores)
+    N = cfg["N"]
+    nl = O.olib("f64").orc_mg_nlev(N)
+    levels = [ggb.Grid(N >> l) for l in range(nl)]
+    g = levels[0]
+    vel0, vband = g.initial_velocity(seed=1, octaves=cfg["octaves"])
+    OF = O.OracleFluid("f64", levels, forcing=cfg["forcing"], rotation=cfg["rotation"],
+                       cg_tol=cfg["cg_tol"], cg_maxit=500)
+    OF.set_state(vel0, vband if cfg["forcing"] > 0 else None)
+    res = {"cores": cores, "fluid_s": 0.0, "render_s": 0.0, "particles_s": 0.0}
+    have_ref = O.available("ref_fast")
+    if have_ref:
+        rg = O.RefGrid(N, kind="ref_fast")
+        RP = O.RefParticles(cfg["P"], kind="ref_fast")
+        RP.color_from_image(ggb.strip_to_float(ggb.banded_strip(1)), 64, 1024)
+        RR = O.RefRender(cfg["R"], 1.0, 0.35, kind="ref_fast")
+    for _ in range(steps):
+        dt = g.h / max(OF.max_speed(), 1e-6)
+        if have_ref:
+            vel32 = OF.get(3).astype(np.float32)
+            t0 = time.perf_counter()
+            RP.advect(rg, vel32, dt)
+            res["particles_s"] += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        OF.step(dt)
+        res["fluid_s"] += time.perf_counter() - t0
+        if have_ref:
+            t0 = time.perf_counter()
+            RR.fade(0.02)
+            RR.deposit(RP)
+            res["render_s"] += time.perf_counter() - t0
+    res["steps"] = steps
+    res["have_ref"] = have_ref
+    try:
+        model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+    except Exception:
+        model = "unknown"
+    res["cpu_model"] = model
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(the path would shard by cube face; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+    cfg = CONFIGS[args.config]
+    if ggb.device_init() != 0:
+        sys.exit("bench.py: no GPU")
+
+    t0 = time.perf_counter()
+    g, F, P, R = setup(cfg, cfg["precision"])
+    log("setup %.1fs: N=%d cells=%d particles=%d R=%d prec=%d cold CG %d its" %
+        (time.perf_counter() - t0, cfg["N"], g.n, cfg["P"], cfg["R"], cfg["precision"], F.cg_iters))
+    timers = {k: ggb.Timer() for k in "pfr"}
+    acc = dict(p=0.0, f=0.0, r=0.0, its=0, entries=0)
+    run_steps(g, F, P, R, args.warmup, timers, dict(acc))
+    ggb.lib().ggb_sync()
+    log("warmup done (%d steps)" % args.warmup)
+    t0 = time.perf_counter()
+    run_steps(g, F, P, R, args.steps, timers, acc)
+    ggb.lib().ggb_sync()
+    wall = time.perf_counter() - t0
+    K = args.steps
+    ms_fluid = acc["f"] / K
+    ms_render = acc["r"] / K
+    ms_part = acc["p"] / K
+    fluid_sps = 1000.0 / ms_fluid
+    msplat = cfg["P"] / (ms_render / 1000.0) / 1e6
+    relres = F.true_relres()
+    log("timed %d steps in %.2fs: fluid %.3f ms (%.1f its/step), particles %.3f ms, render %.3f ms"
+        % (K, wall, ms_fluid, acc["its"] / K, ms_part, ms_render))
+
+    out = {
+        "metric": "fluid steps/s @N=512 + Msplat-particles/s; mixed vs FP64 at 1e-8 residual",
+        "value": round(fluid_sps, 3),
+        "unit": "fluid steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1000.0 / K, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32" if cfg["precision"] == ggb.PREC_MIXED else "f64",
+        "data": "synthetic (seeded curl noise + zonal bands; Fibonacci particles; seeded banded strip)",
+        "config": {
+            "workload": "cubed-sphere N=%d (%d cells), %d particles, faces %d^2, forcing %g, rotation %g, "
+                        "CG tol %g, %s" % (cfg["N"], g.n, cfg["P"], cfg["R"], cfg["forcing"], cfg["rotation"],
+                                           cfg["cg_tol"], "FP32 V-cycle in FP64 CG" if cfg["precision"] == 1
+                                           else "FP64"),
+            "parallelism": "1 GPU",
+        },
+        "msplat_particles_per_s": round(msplat, 2),
+        "mparticles_advect_per_s": round(cfg["P"] / (ms_part / 1000.0) / 1e6, 2),
+        "full_loop_steps_per_s": round(K / wall, 3),
+        "cg_iters_per_step": round(acc["its"] / K, 2),
+        "true_relres": relres,
+        "splat_entries_per_step": acc["entries"] // K,
+    }
+    F.free()
+    P.free()
+    R.free()
+
+    if not args.no_fp64 and cfg["precision"] == ggb.PREC_MIXED:
+        g2, F2, _, _ = setup(cfg, ggb.PREC_FP64, with_particles=False)
+        acc2 = dict(p=0.0, f=0.0, r=0.0, its=0, entries=0)
+        run_steps(g2, F2, None, None, args.warmup, timers, dict(acc2))
+        run_steps(g2, F2, None, None, K, timers, acc2)
+        fp64_sps = 1000.0 / (acc2["f"] / K)
+        out["fp64_fluid_steps_per_s"] = round(fp64_sps, 3)
+        out["fp64_cg_iters_per_step"] = round(acc2["its"] / K, 2)
+        out["fp64_true_relres"] = F2.true_relres()
+        out["mixed_over_fp64"] = round(fluid_sps / fp64_sps, 3)
+        F2.free()
+
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        cb = cpu_baseline(cfg, args.cpu_steps)
+        cpu_fluid = cb["steps"] / cb["fluid_s"]
+        out["cpu_baseline"] = {
+            "value": round(cpu_fluid, 4),
+            "unit": "fluid steps/s",
+            "cores": cb["cores"],
+            "kind": "port",
+            "sample": "%d steps of the same workload after setup: fluid by the FP64 C restatement of the "
+                      "reference (the reference's float PCG cannot reach tol 1e-8); particles and render by "
+                      "the reference's own -O3 build; OpenMP on %d threads of %s" %
+                      (cb["steps"], cb["cores"], cb["cpu_model"]),
+        }
+        if cb["have_ref"]:
+            out["cpu_baseline"]["msplat_particles_per_s"] = round(cfg["P"] * cb["steps"] / cb["render_s"] / 1e6, 3)
+            out["cpu_baseline"]["mparticles_advect_per_s"] = round(cfg["P"] * cb["steps"] / cb["particles_s"] / 1e6, 3)
+        log("cpu baseline %.1fs" % (time.perf_counter() - t0))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -157,6 +157,17 @@ void ggb_grid_dev_release(CubeSphere *cs)
 
 /* ---------------- fluid device state ---------------- */
 
+/* host copy of one multigrid level, kept for the levels whose V-cycle is
+ * precomputed as a dense matrix */
+typedef struct {
+	const CubeSphere *cs;
+	float *dinv;
+	int *ptap;   /* [nf][4] taps into the next coarser level */
+	float *pw;
+	int *rstart, *ridx;
+	float *rw;
+} HostLevel;
+
 struct ggb_fluid_dev {
 	int prec;
 	int T, TM, TZ;          /* state, multigrid and preconditioned-residual types */
@@ -164,15 +175,19 @@ struct ggb_fluid_dev {
 	int n;
 	ggbk_grid g0;
 	int nlev;
+	int dense_level;        /* level whose sub-V-cycle is a dense matrix, or -1 */
 	CubeSphere *coarse;     /* [nlev]; [0] unused */
 	ggbk_level *levels;     /* [nlev] */
+	HostLevel *hl;          /* [nlev] */
 	void **xbufs;
 	int nxbufs;
-	void *omega, *omega_band, *psi, *vel, *q0, *q1, *q2, *dep_b, *dep_f, *r, *p, *Ap, *z;
+	void *omega, *omega_band, *psi, *psi_alt, *vel, *q0, *q1, *q2, *dep_b, *dep_f, *r, *pbuf[2], *Ap, *z;
 	ggbk_scalars *s;        /* device */
 	ggbk_scalars *hs;       /* pinned host copy */
-	double *partials;
+	ggbk_red red;
+	int nb0;                /* blocks of a level-0 pass */
 	int predict;            /* iterations the previous solve needed */
+	int vmax_valid;         /* s->vmax2 matches vel */
 	/* last-synced copies of the host mirrors (detects caller writes) */
 	float *shadow_omega, *shadow_band, *shadow_psi, *shadow_vel;
 };
@@ -242,9 +257,11 @@ static int mg_levels(int N)
 	return nlev;
 }
 
-/* Bilinear transfer from coarse level C to fine level Fg: taps of every fine centre on
- * the coarse grid, and R = P^T as CSR with fine indices ascending (fluid.c:64-96). */
-static ggbk_xfer build_transfer(struct ggb_fluid_dev *D, const CubeSphere *fine, const CubeSphere *coarse)
+/* Bilinear transfer from the coarse grid to the fine grid: taps of every fine centre on
+ * the coarse grid, and R = P^T as CSR with fine indices ascending (fluid.c:64-96).  The
+ * host tables are left in *hl. */
+static ggbk_xfer build_transfer(struct ggb_fluid_dev *D, const CubeSphere *fine, const CubeSphere *coarse,
+				HostLevel *hl)
 {
 	int nf = fine->ncells, nc = coarse->ncells;
 	int *ptap = malloc(sizeof(int) * 4 * (size_t) nf);
@@ -281,9 +298,27 @@ static ggbk_xfer build_transfer(struct ggb_fluid_dev *D, const CubeSphere *fine,
 			rw[cur[c]] = pw[4 * i + k];
 			cur[c]++;
 		}
+	/* R as padded rows (ELL, column-major), fine index ascending within a row */
+	int rmax = 0;
+	for (int c = 0; c < nc; c++)
+		if (rstart[c + 1] - rstart[c] > rmax)
+			rmax = rstart[c + 1] - rstart[c];
+	if (rmax > GGBK_RMAX)
+		ggb_die("fluid: restriction row of %d entries exceeds %d", rmax, GGBK_RMAX);
+	int *ri_ell = calloc((size_t) rmax * nc, sizeof(int));
+	float *rw_ell = calloc((size_t) rmax * nc, sizeof(float));
+	if (!ri_ell || !rw_ell)
+		ggb_die("fluid: out of memory");
+	for (int c = 0; c < nc; c++)
+		for (int k = rstart[c]; k < rstart[c + 1]; k++) {
+			ri_ell[(size_t) (k - rstart[c]) * nc + c] = ridx[k];
+			rw_ell[(size_t) (k - rstart[c]) * nc + c] = rw[k];
+		}
 	ggbk_xfer x;
+	memset(&x, 0, sizeof(x));
 	x.nf = nf;
 	x.nc = nc;
+	x.rmax = rmax;
 	void *d;
 	d = xbuf(D, sizeof(int) * 4 * (size_t) nf, "transfer");
 	upload(d, pt_cm, sizeof(int) * 4 * (size_t) nf);
@@ -291,17 +326,137 @@ static ggbk_xfer build_transfer(struct ggb_fluid_dev *D, const CubeSphere *fine,
 	d = xbuf(D, sizeof(float) * 4 * (size_t) nf, "transfer");
 	upload(d, pw_cm, sizeof(float) * 4 * (size_t) nf);
 	x.pw = d;
-	d = xbuf(D, sizeof(int) * ((size_t) nc + 1), "transfer");
-	upload(d, rstart, sizeof(int) * ((size_t) nc + 1));
-	x.rstart = d;
-	d = xbuf(D, sizeof(int) * 4 * (size_t) nf, "transfer");
-	upload(d, ridx, sizeof(int) * 4 * (size_t) nf);
+	d = xbuf(D, sizeof(int) * (size_t) rmax * nc, "transfer");
+	upload(d, ri_ell, sizeof(int) * (size_t) rmax * nc);
 	x.ridx = d;
-	d = xbuf(D, sizeof(float) * 4 * (size_t) nf, "transfer");
-	upload(d, rw, sizeof(float) * 4 * (size_t) nf);
+	d = xbuf(D, sizeof(float) * (size_t) rmax * nc, "transfer");
+	upload(d, rw_ell, sizeof(float) * (size_t) rmax * nc);
 	x.rw = d;
-	free(ptap); free(pw); free(rstart); free(ridx); free(rw); free(cur); free(pt_cm); free(pw_cm);
+	free(ri_ell);
+	free(rw_ell);
+	free(cur);
+	free(pt_cm);
+	free(pw_cm);
+	hl->ptap = ptap;
+	hl->pw = pw;
+	hl->rstart = rstart;
+	hl->ridx = ridx;
+	hl->rw = rw;
 	return x;
+}
+
+static void host_level_free(HostLevel *hl)
+{
+	free(hl->dinv);
+	free(hl->ptap);
+	free(hl->pw);
+	free(hl->rstart);
+	free(hl->ridx);
+	free(hl->rw);
+	memset(hl, 0, sizeof(*hl));
+}
+
+/* ---- host V-cycle in double (reference fluid.c:100-146), used only to tabulate the
+ * dense operator of the bottom levels ---- */
+
+static double hK(const CubeSphere *cs, const double *x, int i)
+{
+	double xi = x[i], s = 0;
+	for (int k = 0; k < cs->nnb[i]; k++)
+		s += cs->w[CS_MAXNB * i + k] * (xi - x[cs->nbr[CS_MAXNB * i + k]]);
+	return s;
+}
+
+static void h_smooth(const HostLevel *L, const double *b, double *x, double *r, int sweeps)
+{
+	int n = L->cs->ncells;
+	for (int s = 0; s < sweeps; s++) {
+		for (int i = 0; i < n; i++)
+			r[i] = b[i] - hK(L->cs, x, i);
+		for (int i = 0; i < n; i++)
+			x[i] += 0.8 * L->dinv[i] * r[i];
+	}
+}
+
+/* x = V_l(b); bufs[l] holds 3 scratch arrays of level l */
+static void h_vcycle(const HostLevel *hl, int nlev, int l, const double *b, double *x, double **bufs)
+{
+	const HostLevel *L = &hl[l];
+	int n = L->cs->ncells;
+	double *r = bufs[l];
+	memset(x, 0, sizeof(double) * n);
+	if (l == nlev - 1) {
+		h_smooth(L, b, x, r, 60);
+		return;
+	}
+	h_smooth(L, b, x, r, 1);
+	for (int i = 0; i < n; i++)
+		r[i] = b[i] - hK(L->cs, x, i);
+	const HostLevel *C = &hl[l + 1];
+	int nc = C->cs->ncells;
+	double *bc = bufs[l + 1] + nc, *xc = bufs[l + 1] + 2 * (size_t) nc;
+	for (int c = 0; c < nc; c++) {
+		double acc = 0;
+		for (int k = L->rstart[c]; k < L->rstart[c + 1]; k++)
+			acc += L->rw[k] * r[L->ridx[k]];
+		bc[c] = acc;
+	}
+	h_vcycle(hl, nlev, l + 1, bc, xc, bufs);
+	for (int i = 0; i < n; i++) {
+		const int *t = L->ptap + 4 * i;
+		const float *w = L->pw + 4 * i;
+		x[i] += 1.0 * (w[0] * xc[t[0]] + w[1] * xc[t[1]] + w[2] * xc[t[2]] + w[3] * xc[t[3]]);
+	}
+	h_smooth(L, b, x, r, 1);
+}
+
+#define DENSE_MAX 1536
+
+/* Tabulate the sub-V-cycle of level L as M[i][j] = (V_L e_j)_i and upload it. */
+static void build_dense(struct ggb_fluid_dev *D, int L)
+{
+	int n = D->hl[L].cs->ncells;
+	double *M = malloc(sizeof(double) * (size_t) n * n);
+	if (!M)
+		ggb_die("fluid: out of memory");
+	#pragma omp parallel
+	{
+		double **bufs = calloc((size_t) D->nlev, sizeof(double *));
+		for (int l = L; l < D->nlev; l++)
+			bufs[l] = calloc(3 * (size_t) D->hl[l].cs->ncells, sizeof(double));
+		double *e = calloc((size_t) n, sizeof(double)), *x = calloc((size_t) n, sizeof(double));
+		#pragma omp for schedule(dynamic, 8)
+		for (int j = 0; j < n; j++) {
+			memset(e, 0, sizeof(double) * n);
+			e[j] = 1.0;
+			h_vcycle(D->hl, D->nlev, L, e, x, bufs);
+			for (int i = 0; i < n; i++)
+				M[(size_t) i * n + j] = x[i];
+		}
+		for (int l = L; l < D->nlev; l++)
+			free(bufs[l]);
+		free(bufs);
+		free(e);
+		free(x);
+	}
+	size_t cnt = (size_t) n * n;
+	void *d;
+	if (D->TM == GGBK_F64) {
+		d = xbuf(D, 8 * cnt, "multigrid dense");
+		upload(d, M, 8 * cnt);
+	} else {
+		float *Mf = malloc(4 * cnt);
+		if (!Mf)
+			ggb_die("fluid: out of memory");
+		for (size_t k = 0; k < cnt; k++)
+			Mf[k] = (float) M[k];
+		d = xbuf(D, 4 * cnt, "multigrid dense");
+		upload(d, Mf, 4 * cnt);
+		free(Mf);
+	}
+	free(M);
+	D->levels[L].M = d;
+	D->dense_level = L;
 }
 
 int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
@@ -331,6 +486,7 @@ int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
 	D->n = (int) n;
 	D->g0 = ggb_grid_dev(cs);
 	D->nlev = prm->multigrid ? mg_levels(cs->N) : 0;
+	D->dense_level = -1;
 	D->TZ = prm->multigrid ? D->TM : D->T;
 	D->tzsz = D->TZ == GGBK_F64 ? 8 : 4;
 	F->nlev = D->nlev;
@@ -342,6 +498,7 @@ int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
 	D->omega = xbuf(D, ts * n, "fluid");
 	D->omega_band = xbuf(D, ts * n, "fluid");
 	D->psi = xbuf(D, ts * n, "fluid");
+	D->psi_alt = xbuf(D, ts * n, "fluid");
 	D->vel = xbuf(D, 3 * ts * n, "fluid");
 	D->q0 = xbuf(D, ts * n, "fluid");
 	D->q1 = xbuf(D, ts * n, "fluid");
@@ -349,41 +506,69 @@ int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
 	D->dep_b = xbuf(D, 3 * ts * n, "fluid");
 	D->dep_f = xbuf(D, 3 * ts * n, "fluid");
 	D->r = xbuf(D, ts * n, "fluid");
-	D->p = xbuf(D, ts * n, "fluid");
+	D->pbuf[0] = xbuf(D, ts * n, "fluid");
+	D->pbuf[1] = xbuf(D, ts * n, "fluid");
 	D->Ap = xbuf(D, ts * n, "fluid");
 	D->z = xbuf(D, D->tzsz * n, "fluid");
 	D->s = xbuf(D, sizeof(ggbk_scalars), "fluid");
 	D->hs = ggbk_host_alloc(sizeof(ggbk_scalars));
 	if (!D->hs)
 		ggb_die("fluid: pinned allocation failed");
-	size_t nblk = (n + 255) / 256 + 1;
-	D->partials = xbuf(D, sizeof(double) * 2 * nblk, "fluid");
+	D->nb0 = ggbk_nblocks((int) n);
+	D->red.stride = D->nb0;
+	D->red.part = xbuf(D, sizeof(double) * GGBK_NSLOTS * (size_t) D->nb0, "fluid");
 	ggbk_zero(D->s, sizeof(ggbk_scalars));
 	ggbk_zero(D->omega, ts * n);
 	ggbk_zero(D->omega_band, ts * n);
 	ggbk_zero(D->psi, ts * n);
 	ggbk_zero(D->vel, 3 * ts * n);
+	ggbk_zero(D->pbuf[0], ts * n);
+	ggbk_zero(D->pbuf[1], ts * n);
 
 	if (D->nlev > 0) {
 		D->coarse = calloc((size_t) D->nlev, sizeof(CubeSphere));
 		D->levels = calloc((size_t) D->nlev, sizeof(ggbk_level));
-		if (!D->coarse || !D->levels)
+		D->hl = calloc((size_t) D->nlev, sizeof(HostLevel));
+		if (!D->coarse || !D->levels || !D->hl)
 			ggb_die("fluid: out of memory");
 		D->levels[0].g = D->g0;
-		D->levels[0].x = NULL; /* z is passed per call */
-		D->levels[0].b = NULL;
 		D->levels[0].r = xbuf(D, D->tmsz * n, "multigrid");
+		D->hl[0].cs = cs;
 		for (int l = 1; l < D->nlev; l++) {
 			if (cs_init(&D->coarse[l], cs->N >> l))
 				ggb_die("fluid_init: coarse grid allocation failed");
 			const CubeSphere *fine = l == 1 ? cs : &D->coarse[l - 1];
+			D->hl[l].cs = &D->coarse[l];
 			D->levels[l].g = ggb_grid_dev(&D->coarse[l]);
-			D->levels[l - 1].xf = build_transfer(D, fine, &D->coarse[l]);
+			D->levels[l - 1].xf = build_transfer(D, fine, &D->coarse[l], &D->hl[l - 1]);
 			size_t nl = D->coarse[l].ncells;
 			D->levels[l].x = xbuf(D, D->tmsz * nl, "multigrid");
 			D->levels[l].b = xbuf(D, D->tmsz * nl, "multigrid");
 			D->levels[l].r = xbuf(D, D->tmsz * nl, "multigrid");
 		}
+		/* dinv per level on the host (fluid.c:58-63) */
+		for (int l = 0; l < D->nlev; l++) {
+			const CubeSphere *c = D->hl[l].cs;
+			D->hl[l].dinv = malloc(sizeof(float) * (size_t) c->ncells);
+			if (!D->hl[l].dinv)
+				ggb_die("fluid: out of memory");
+			for (int i = 0; i < c->ncells; i++) {
+				double sw = 0;
+				for (int k = 0; k < c->nnb[i]; k++)
+					sw += c->w[CS_MAXNB * i + k];
+				D->hl[l].dinv[i] = (float) (1.0 / sw);
+			}
+		}
+		/* the bottom of the V-cycle as one dense operator (not in reference-precision
+		 * mode, which keeps the reference's float sweeps bit for bit) */
+		int L = D->prec == GGB_PREC_REF32 ? D->nlev : -1;
+		for (int l = 1; l < D->nlev && L < 0; l++)
+			if (D->coarse[l].ncells <= DENSE_MAX)
+				L = l;
+		if (L > 0 && L < D->nlev)
+			build_dense(D, L);
+		for (int l = 0; l < D->nlev; l++)
+			host_level_free(&D->hl[l]);
 	}
 	D->shadow_omega = falloc(n);
 	D->shadow_band = falloc(n);
@@ -409,6 +594,7 @@ void fluid_free(Fluid *F)
 			cs_free(&D->coarse[l]);
 		free(D->coarse);
 		free(D->levels);
+		free(D->hl);
 		ggbk_host_free(D->hs);
 		free(D->shadow_omega);
 		free(D->shadow_band);
@@ -471,6 +657,7 @@ static void mirrors_pull(Fluid *F)
 	if (memcmp(F->vel, D->shadow_vel, 12 * n)) {
 		put_f32(D, D->vel, F->vel, 3 * n);
 		memcpy(D->shadow_vel, F->vel, 12 * n);
+		D->vmax_valid = 0;
 	}
 }
 
@@ -496,6 +683,7 @@ void fluid_sync_device(Fluid *F)
 	put_f32(D, D->omega_band, F->omega_band, n);
 	put_f32(D, D->psi, F->psi, n);
 	put_f32(D, D->vel, F->vel, 3 * n);
+	D->vmax_valid = 0;
 	memcpy(D->shadow_omega, F->omega, 4 * n);
 	memcpy(D->shadow_band, F->omega_band, 4 * n);
 	memcpy(D->shadow_psi, F->psi, 4 * n);
@@ -541,6 +729,8 @@ int fluid_set_f64(Fluid *F, int field, const double *in)
 	void *dst = field_of(D, field, &cnt);
 	if (!dst)
 		return -1;
+	if (field == GGB_FIELD_VEL)
+		D->vmax_valid = 0;
 	if (D->T == GGBK_F64) {
 		upload(dst, in, 8 * cnt);
 		return 0;
@@ -555,12 +745,16 @@ int fluid_set_f64(Fluid *F, int field, const double *in)
 
 /* ---------------- solver ---------------- */
 
-static void precondition(struct ggb_fluid_dev *D)
+static int mean_op(const struct ggb_fluid_dev *D) { return D->T == GGBK_F64 ? GGBK_FIN_MEAN_F64 : GGBK_FIN_MEAN_F32; }
+
+/* z = M^-1 r and rz (fluid.c:148-160, 315-318, 344-348) */
+static void precondition(struct ggb_fluid_dev *D, int first)
 {
 	if (D->nlev > 0)
-		ggbk_vcycle(D->levels, D->nlev, D->r, D->T, D->z, D->TM, D->s);
+		ggbk_vcycle(D->levels, D->nlev, D->r, D->T, D->z, D->TM, D->s, D->red);
 	else
-		ggbk_jacobi_precond(D->g0, D->r, D->z, D->T, D->s);
+		ggbk_jacobi_precond(D->g0, D->r, D->z, D->T, D->s, D->red);
+	ggbk_fin(first ? GGBK_FIN_RZ_FIRST : GGBK_FIN_RZ, D->s, D->red, D->nb0, 0, 0, 0);
 }
 
 static void read_scalars(struct ggb_fluid_dev *D)
@@ -569,41 +763,65 @@ static void read_scalars(struct ggb_fluid_dev *D)
 		ggb_die("device error: %s", ggbk_last_error());
 }
 
-/* PCG on the device state (reference fluid.c:288-358). */
-static int solve_device(Fluid *F, int maxit)
+/* PCG on the device state (reference fluid.c:288-358).  With sub_mean the pending mean
+ * of omega (in s->mean) is subtracted by the first pass.  psi's mean is NOT removed. */
+static int solve_device(Fluid *F, int maxit, int sub_mean)
 {
 	struct ggb_fluid_dev *D = F->dev;
 	ggbk_grid g = D->g0;
-	ggbk_cg_start(g, D->omega, D->psi, D->r, D->T, F->prm.cg_tol, maxit, D->s, D->partials);
-	precondition(D);
-	ggbk_cg_first_dir(g, D->r, D->z, D->TZ, D->p, D->T, D->s, D->partials);
+	ggbk_cg_start(g, D->omega, D->psi, D->r, D->T, sub_mean, D->s, D->red);
+	ggbk_fin(GGBK_FIN_START, D->s, D->red, D->nb0, F->prm.cg_tol, maxit, 0);
+	precondition(D, 1);
 	int batch = D->predict > 1 ? D->predict - 1 : 1;
-	int launched = 0;
+	int k = 0;
 	for (;;) {
-		for (int b = 0; b < batch; b++) {
-			ggbk_cg_Ap(g, D->p, D->Ap, D->T, D->s, D->partials);
-			ggbk_cg_update(g, D->psi, D->r, D->p, D->Ap, D->T, D->s, D->partials);
-			precondition(D);
-			ggbk_cg_next_dir(g, D->r, D->z, D->TZ, D->p, D->T, D->s, D->partials);
+		for (int b = 0; b < batch; b++, k++) {
+			void *pin = D->pbuf[k & 1], *pout = D->pbuf[(k + 1) & 1];
+			ggbk_cg_Ap(g, D->z, D->TZ, pin, pout, D->Ap, D->T, k == 0, D->s, D->red);
+			ggbk_fin(GGBK_FIN_PAP, D->s, D->red, D->nb0, 0, 0, 0);
+			ggbk_cg_update(g, D->psi, D->r, pout, D->Ap, D->T, D->s, D->red);
+			ggbk_fin(GGBK_FIN_RR, D->s, D->red, D->nb0, 0, 0, 0);
+			precondition(D, 0);
 		}
-		launched += batch;
 		read_scalars(D);
-		if (!D->hs->active || launched >= maxit)
+		if (!D->hs->active || k >= maxit)
 			break;
 		batch = 1;
 	}
 	ggbk_cg_zero_if_bb0(g, D->psi, D->T, D->s);
-	if (D->hs->status != 2)
-		ggbk_remove_mean(g, D->psi, D->T, D->s, D->partials);
 	F->cg_iters = D->hs->it;
 	F->cg_relres = D->hs->bb == 0.0 ? 0.0 : sqrt(D->hs->rr / D->hs->bb);
 	D->predict = D->hs->it > 0 ? D->hs->it : 1;
 	return D->hs->it;
 }
 
+/* psi -= mean(psi) fused into the velocity pass (fluid.c:354 then :361-376) */
+static void psi_mean_and_velocity(Fluid *F)
+{
+	struct ggb_fluid_dev *D = F->dev;
+	ggbk_mean_partial(D->g0, D->psi, D->T, D->red);
+	ggbk_fin(mean_op(D), D->s, D->red, D->nb0, 0, 0, D->g0.total_area);
+	ggbk_velocity(D->g0, D->psi, D->psi_alt, 1, D->vel, D->T, D->s, D->red);
+	ggbk_fin(GGBK_FIN_VMAX, D->s, D->red, D->nb0, 0, 0, 0);
+	void *t = D->psi;
+	D->psi = D->psi_alt;
+	D->psi_alt = t;
+	D->vmax_valid = 1;
+}
+
+static void velocity_device(Fluid *F)
+{
+	struct ggb_fluid_dev *D = F->dev;
+	ggbk_velocity(D->g0, D->psi, NULL, 0, D->vel, D->T, D->s, D->red);
+	ggbk_fin(GGBK_FIN_VMAX, D->s, D->red, D->nb0, 0, 0, 0);
+	D->vmax_valid = 1;
+}
+
 int fluid_solve_psi_device(Fluid *F)
 {
-	int it = solve_device(F, F->prm.cg_maxit);
+	struct ggb_fluid_dev *D = F->dev;
+	int it = solve_device(F, F->prm.cg_maxit, 0);
+	ggbk_remove_mean(D->g0, D->psi, D->T, D->s, D->red);
 	if (ggbk_check("solve"))
 		ggb_die("solve: %s", ggbk_last_error());
 	return it;
@@ -612,16 +830,10 @@ int fluid_solve_psi_device(Fluid *F)
 int fluid_solve_psi(Fluid *F)
 {
 	mirrors_pull(F);
-	int it = solve_device(F, F->prm.cg_maxit);
+	int it = fluid_solve_psi_device(F);
 	get_f32(F->dev, F->psi, F->dev->psi, F->dev->n);
 	memcpy(F->dev->shadow_psi, F->psi, 4 * (size_t) F->dev->n);
 	return it;
-}
-
-static void velocity_device(Fluid *F)
-{
-	struct ggb_fluid_dev *D = F->dev;
-	ggbk_velocity(D->g0, D->psi, D->vel, D->T, D->s, D->partials);
 }
 
 void fluid_compute_velocity(Fluid *F)
@@ -640,11 +852,11 @@ void fluid_set_state(Fluid *F, const float *vel_initial, const float *vel_band)
 	void *tmp = ggb_dmalloc(3 * D->tsz * n, "set_state");
 	put_f32(D, tmp, vel_initial, 3 * n);
 	ggbk_curl(g, tmp, D->omega, D->T);
-	ggbk_remove_mean(g, D->omega, D->T, D->s, D->partials);
+	ggbk_remove_mean(g, D->omega, D->T, D->s, D->red);
 	if (vel_band) {
 		put_f32(D, tmp, vel_band, 3 * n);
 		ggbk_curl(g, tmp, D->omega_band, D->T);
-		ggbk_remove_mean(g, D->omega_band, D->T, D->s, D->partials);
+		ggbk_remove_mean(g, D->omega_band, D->T, D->s, D->red);
 	} else {
 		ggbk_zero(D->omega_band, D->tsz * n);
 	}
@@ -654,14 +866,15 @@ void fluid_set_state(Fluid *F, const float *vel_initial, const float *vel_band)
 	int save = F->prm.cg_maxit;
 	int cold = save * 20 > 20000 ? save * 20 : 20000;
 	D->predict = 1;
-	solve_device(F, cold);
+	solve_device(F, cold, 0);
 	D->predict = 1;
-	velocity_device(F);
+	psi_mean_and_velocity(F);
 	fluid_sync_host(F);
 	if (ggbk_check("fluid_set_state"))
 		ggb_die("fluid_set_state: %s", ggbk_last_error());
 }
 
+/* reference fluid.c:424-478 */
 void fluid_step(Fluid *F, double dt)
 {
 	struct ggb_fluid_dev *D = F->dev;
@@ -669,22 +882,16 @@ void fluid_step(Fluid *F, double dt)
 	int T = D->T;
 	double twoOmega = 2.0 * F->prm.rotation;
 	int bfecc = F->prm.bfecc;
-	ggbk_departures(g, D->vel, D->dep_b, D->dep_f, T, dt, bfecc);
-	ggbk_absvort(g, D->omega, D->q0, T, twoOmega);
-	void *qn = D->q1;
-	if (!bfecc) {
-		ggbk_advect(g, D->q0, D->dep_b, D->q1, T);
-	} else {
-		ggbk_advect(g, D->q0, D->dep_b, D->q1, T);
-		ggbk_advect(g, D->q1, D->dep_f, D->q2, T);
-		ggbk_bfecc_source(D->n, D->q0, D->q2, T);
-		ggbk_limited_advect(g, D->q2, D->q0, D->dep_b, D->q1, T);
-	}
 	double relax = F->prm.forcing > 0 ? 1.0 - exp(-F->prm.forcing * dt) : 0.0;
-	ggbk_relax(g, qn, D->omega_band, D->omega, T, twoOmega, relax);
-	ggbk_remove_mean(g, D->omega, T, D->s, D->partials);
-	solve_device(F, F->prm.cg_maxit);
-	velocity_device(F);
+	ggbk_departures(g, D->vel, D->omega, D->q0, D->dep_b, D->dep_f, T, dt, twoOmega, bfecc);
+	ggbk_advect(g, D->q0, D->dep_b, D->q1, T);
+	if (bfecc)
+		ggbk_advect_bfecc(g, D->q1, D->q0, D->dep_f, D->q2, T);
+	ggbk_limited_relax(g, D->q2, D->q0, D->dep_b, D->q1, bfecc, D->omega_band, D->omega, T, twoOmega, relax,
+			   D->red);
+	ggbk_fin(mean_op(D), D->s, D->red, D->nb0, 0, 0, g.total_area);
+	solve_device(F, F->prm.cg_maxit, 1);
+	psi_mean_and_velocity(F);
 	if (ggbk_check("fluid_step"))
 		ggb_die("fluid_step: %s", ggbk_last_error());
 }
@@ -692,12 +899,15 @@ void fluid_step(Fluid *F, double dt)
 double fluid_max_speed(const Fluid *F)
 {
 	struct ggb_fluid_dev *D = F->dev;
-	ggbk_max_speed2(D->g0, D->vel, D->T, D->s, D->partials);
+	if (!D->vmax_valid) {
+		ggbk_max_speed2(D->g0, D->vel, D->T, D->red);
+		ggbk_fin(GGBK_FIN_VMAX, D->s, D->red, D->nb0, 0, 0, 0);
+		D->vmax_valid = 1;
+	}
 	read_scalars(D);
 	return sqrt(D->hs->vmax2);
 }
 
-/* ||b - K psi|| / ||b|| in double on the device state. */
 double fluid_true_relres(Fluid *F)
 {
 	struct ggb_fluid_dev *D = F->dev;
@@ -742,7 +952,8 @@ int fluid_vcycle_f64(Fluid *F, const double *b, double *x)
 	hs.active = 1;
 	ggbk_scalars *ds = ggb_dmalloc(sizeof(ggbk_scalars), "vcycle");
 	upload(ds, &hs, sizeof(hs));
-	ggbk_vcycle(D->levels, D->nlev, db, D->T, dz, D->TM, ds);
+	ggbk_red none = { NULL, 0 };
+	ggbk_vcycle(D->levels, D->nlev, db, D->T, dz, D->TM, ds, none);
 	ggbk_convert(tmp, GGBK_F64, dz, D->TM, n);
 	download(x, tmp, 8 * n);
 	ggbk_free(db);
@@ -820,14 +1031,17 @@ void fluid_remove_mean(const CubeSphere *cs, float *f)
 {
 	ggbk_grid g = ggb_grid_dev(cs);
 	size_t n = cs->ncells;
+	int nb = ggbk_nblocks((int) n);
 	float *d = ggb_dmalloc(4 * n, "remove_mean");
 	ggbk_scalars *s = ggb_dmalloc(sizeof(ggbk_scalars), "remove_mean");
-	double *part = ggb_dmalloc(sizeof(double) * ((n + 255) / 256 + 1), "remove_mean");
+	ggbk_red red;
+	red.stride = nb;
+	red.part = ggb_dmalloc(sizeof(double) * GGBK_NSLOTS * (size_t) nb, "remove_mean");
 	ggbk_zero(s, sizeof(ggbk_scalars));
 	upload(d, f, 4 * n);
-	ggbk_remove_mean(g, d, GGBK_F32, s, part);
+	ggbk_remove_mean(g, d, GGBK_F32, s, red);
 	download(f, d, 4 * n);
 	ggbk_free(d);
 	ggbk_free(s);
-	ggbk_free(part);
+	ggbk_free(red.part);
 }

@@ -2,10 +2,12 @@
 // the PCG vector passes with device-resident scalars, the velocity reconstruction and
 // the BFECC semi-Lagrangian advection.
 //
-// Every pass is HBM-bound (sparse stencil / gather); reductions are deterministic
-// (wave64 tree, then blocks folded in order by the last block to finish), and the
-// CG loop's control lives on the device (ggbk_scalars.active), so the host can launch
-// iterations ahead without reading scalars back.
+// Every pass is HBM-bound (sparse stencil / gather).  Reductions are deterministic:
+// each block stores one partial (wave64 tree, then waves in order), and a one-workgroup
+// fold kernel combines the partials in block order and updates the solver scalars, so
+// the CG loop's control stays on the device and the host can enqueue iterations ahead.
+// Elementwise epilogues are fused into the producing pass (p = z + beta p into Ap,
+// mean subtraction into the next consumer, limiter + relaxation + mean partial).
 //
 // T is the state storage type (double, or float in reference-precision mode); per-cell
 // arithmetic is double as in the reference (src/fluid.c).
@@ -15,6 +17,7 @@
 #include "ggbk.h"
 
 #define BS 256
+#define FIN_BS 256
 
 template <typename T>
 __device__ __forceinline__ double applyK_at(const ggbk_grid &g, const T *x, int i)
@@ -27,6 +30,132 @@ __device__ __forceinline__ double applyK_at(const ggbk_grid &g, const T *x, int 
 		s += (double) g.w[(size_t) k * g.n + i] * (xi - (double) x[j]);
 	}
 	return s;
+}
+
+__device__ __forceinline__ void store_partial(ggbk_red red, int slot, double v)
+{
+	double b = block_sum<BS>(v);
+	if (threadIdx.x == 0)
+		red.part[(size_t) slot * red.stride + blockIdx.x] = b;
+}
+
+__device__ __forceinline__ void store_partial_max(ggbk_red red, int slot, double v)
+{
+	double b = block_max<BS>(v);
+	if (threadIdx.x == 0)
+		red.part[(size_t) slot * red.stride + blockIdx.x] = b;
+}
+
+// ---------------- fold kernel ----------------
+
+// Fold nb partials: thread t sums p[t], p[t + FIN_BS], ... in order, then the block
+// tree; fixed order for a given nb.
+__device__ __forceinline__ double fold_sum(const double *p, int nb)
+{
+	double acc = 0;
+	for (int i = threadIdx.x; i < nb; i += FIN_BS)
+		acc += p[i];
+	return block_sum<FIN_BS>(acc);
+}
+
+__device__ __forceinline__ double fold_max(const double *p, int nb)
+{
+	double acc = 0;
+	for (int i = threadIdx.x; i < nb; i += FIN_BS)
+		acc = fmax(acc, p[i]);
+	return block_max<FIN_BS>(acc);
+}
+
+__global__ __launch_bounds__(FIN_BS) void k_fin(int op, ggbk_scalars *s, ggbk_red red, int nb, double tol, int maxit,
+						 double total_area)
+{
+	bool guarded = op == GGBK_FIN_PAP || op == GGBK_FIN_RR || op == GGBK_FIN_RZ_FIRST || op == GGBK_FIN_RZ;
+	if (guarded && !s->active)
+		return;
+	const double *P = red.part;
+	size_t st = red.stride;
+	switch (op) {
+	case GGBK_FIN_START: {
+		double bb = fold_sum(P + GGBK_SL_BB * st, nb);
+		double rr = fold_sum(P + GGBK_SL_RR0 * st, nb);
+		if (threadIdx.x == 0) {
+			s->bb = bb;
+			s->rr = rr;
+			s->it = 0;
+			s->maxit = maxit;
+			s->rz = 0;
+			if (bb == 0.0) {
+				s->status = 2;
+				s->tol2 = 0;
+				s->active = 0;
+			} else {
+				s->status = 0;
+				s->tol2 = tol * tol * bb;
+				s->active = (rr > s->tol2 && maxit > 0) ? 1 : 0;
+			}
+		}
+		break;
+	}
+	case GGBK_FIN_PAP: {
+		double pAp = fold_sum(P + GGBK_SL_PAP * st, nb);
+		if (threadIdx.x == 0) {
+			s->pAp = pAp;
+			if (pAp <= 0) {
+				s->status = 1;
+				s->active = 0;
+			} else {
+				s->alpha = s->rz / pAp;
+			}
+		}
+		break;
+	}
+	case GGBK_FIN_RR: {
+		double rr = fold_sum(P + GGBK_SL_RR * st, nb);
+		if (threadIdx.x == 0) {
+			s->rr = rr;
+			int it = s->it + 1;
+			s->it = it;
+			if (rr <= s->tol2) {
+				s->active = 0;
+				s->status = 0;
+			} else if (it >= s->maxit) {
+				s->active = 0;
+				s->status = 3;
+			}
+		}
+		break;
+	}
+	case GGBK_FIN_RZ_FIRST: {
+		double rz = fold_sum(P + GGBK_SL_RZ * st, nb);
+		if (threadIdx.x == 0)
+			s->rz = rz;
+		break;
+	}
+	case GGBK_FIN_RZ: {
+		double rznew = fold_sum(P + GGBK_SL_RZ * st, nb);
+		if (threadIdx.x == 0) {
+			s->beta = rznew / s->rz;
+			s->rz = rznew;
+		}
+		break;
+	}
+	case GGBK_FIN_MEAN_F64:
+	case GGBK_FIN_MEAN_F32: {
+		double sum = fold_sum(P + GGBK_SL_MEAN * st, nb);
+		if (threadIdx.x == 0) {
+			s->sum = sum;
+			double m = sum / total_area;
+			s->mean = op == GGBK_FIN_MEAN_F32 ? (double) (float) m : m;
+		}
+		break;
+	}
+	case GGBK_FIN_VMAX: {
+		double v = fold_max(P + GGBK_SL_VMAX * st, nb);
+		if (threadIdx.x == 0)
+			s->vmax2 = v;
+		break;
+	}
+	}
 }
 
 // ---------------- free operators ----------------
@@ -91,22 +220,13 @@ __global__ __launch_bounds__(BS) void k_applyK(ggbk_grid g, const T *x, double *
 		y[i] = applyK_at(g, x, i);
 }
 
-// area-weighted sum; the last block stores the mean rounded to T (fluid.c:264-275)
+// area-weighted sum partials (fluid.c:264-275)
 template <typename T>
-__global__ __launch_bounds__(BS) void k_mean(ggbk_grid g, const T *f, ggbk_scalars *s, double *part)
+__global__ __launch_bounds__(BS) void k_mean_partial(ggbk_grid g, const T *f, ggbk_red red)
 {
 	int i = blockIdx.x * BS + threadIdx.x;
 	double v = i < g.n ? (double) g.area[i] * (double) f[i] : 0.0;
-	double bsum = block_sum<BS>(v);
-	if (threadIdx.x == 0)
-		part[blockIdx.x] = bsum;
-	if (last_block(&s->ticket[1])) {
-		double tot = fold_partials<BS>(part, gridDim.x);
-		if (threadIdx.x == 0) {
-			s->sum = tot;
-			s->mean = (double) (T) (tot / g.total_area);
-		}
-	}
+	store_partial(red, GGBK_SL_MEAN, v);
 }
 
 template <typename T>
@@ -120,100 +240,57 @@ __global__ __launch_bounds__(BS) void k_sub_mean(int n, T *f, const ggbk_scalars
 // ---------------- PCG (fluid.c:288-358) ----------------
 
 template <typename T>
-__global__ __launch_bounds__(BS) void k_cg_start(ggbk_grid g, const T *omega, const T *psi, T *r,
-						  double tol, int maxit, ggbk_scalars *s, double *part)
+__global__ __launch_bounds__(BS) void k_cg_start(ggbk_grid g, T *omega, const T *psi, T *r, int sub_mean,
+						  const ggbk_scalars *s, ggbk_red red)
 {
 	int i = blockIdx.x * BS + threadIdx.x;
 	double bbl = 0, rrl = 0;
 	if (i < g.n) {
-		double b = -(double) g.area[i] * (double) omega[i];
+		T om = omega[i];
+		if (sub_mean) {
+			om -= (T) s->mean;
+			omega[i] = om;
+		}
+		double b = -(double) g.area[i] * (double) om;
 		double ri = b - applyK_at(g, psi, i);
 		r[i] = (T) ri;
 		bbl = b * b;
 		rrl = ri * ri;
 	}
-	double bsum = block_sum<BS>(bbl);
-	double rsum = block_sum<BS>(rrl);
-	if (threadIdx.x == 0) {
-		part[blockIdx.x] = bsum;
-		part[gridDim.x + blockIdx.x] = rsum;
-	}
-	if (last_block(&s->ticket[0])) {
-		double bb = fold_partials<BS>(part, gridDim.x);
-		double rr = fold_partials<BS>(part + gridDim.x, gridDim.x);
-		if (threadIdx.x == 0) {
-			s->bb = bb;
-			s->rr = rr;
-			s->it = 0;
-			s->maxit = maxit;
-			s->rz = 0;
-			if (bb == 0.0) {
-				s->status = 2;
-				s->tol2 = 0;
-				s->active = 0;
-			} else {
-				s->status = 0;
-				s->tol2 = tol * tol * bb;
-				s->active = (rr > s->tol2 && maxit > 0) ? 1 : 0;
-			}
-		}
-	}
+	store_partial(red, GGBK_SL_BB, bbl);
+	store_partial(red, GGBK_SL_RR0, rrl);
 }
 
+// p_out = z (first) or z + beta p_in, then Ap = K p_out with p_out recomputed at the
+// neighbours (the update of p is fused into the stencil pass; p is double-buffered).
 template <typename T, typename TZ>
-__global__ __launch_bounds__(BS) void k_cg_first_dir(int n, const T *r, const TZ *z, T *p, ggbk_scalars *s,
-						      double *part)
+__global__ __launch_bounds__(BS) void k_cg_Ap(ggbk_grid g, const TZ *z, const T *p_in, T *p_out, T *Ap, int first,
+					       const ggbk_scalars *s, ggbk_red red)
 {
 	if (!s->active)
 		return;
-	int i = blockIdx.x * BS + threadIdx.x;
-	double v = 0;
-	if (i < n) {
-		p[i] = (T) z[i];
-		v = (double) r[i] * (double) z[i];
-	}
-	double bsum = block_sum<BS>(v);
-	if (threadIdx.x == 0)
-		part[blockIdx.x] = bsum;
-	if (last_block(&s->ticket[2])) {
-		double rz = fold_partials<BS>(part, gridDim.x);
-		if (threadIdx.x == 0)
-			s->rz = rz;
-	}
-}
-
-template <typename T>
-__global__ __launch_bounds__(BS) void k_cg_Ap(ggbk_grid g, const T *p, T *Ap, ggbk_scalars *s, double *part)
-{
-	if (!s->active)
-		return;
+	double beta = first ? 0.0 : s->beta;
 	int i = blockIdx.x * BS + threadIdx.x;
 	double v = 0;
 	if (i < g.n) {
-		double a = applyK_at(g, p, i);
-		Ap[i] = (T) a;
-		v = a * (double) p[i];
-	}
-	double bsum = block_sum<BS>(v);
-	if (threadIdx.x == 0)
-		part[blockIdx.x] = bsum;
-	if (last_block(&s->ticket[3])) {
-		double pAp = fold_partials<BS>(part, gridDim.x);
-		if (threadIdx.x == 0) {
-			s->pAp = pAp;
-			if (pAp <= 0) {
-				s->status = 1;
-				s->active = 0;
-			} else {
-				s->alpha = s->rz / pAp;
-			}
+		T pi = first ? (T) z[i] : (T) ((double) z[i] + beta * (double) p_in[i]);
+		double sum = 0, xi = (double) pi;
+		int nn = g.nnb[i];
+		for (int k = 0; k < nn; k++) {
+			int j = g.nbr[(size_t) k * g.n + i];
+			T pj = first ? (T) z[j] : (T) ((double) z[j] + beta * (double) p_in[j]);
+			sum += (double) g.w[(size_t) k * g.n + i] * (xi - (double) pj);
 		}
+		p_out[i] = pi;
+		Ap[i] = (T) sum;
+		v = sum * xi;
 	}
+	store_partial(red, GGBK_SL_PAP, v);
 }
 
 template <typename T>
-__global__ __launch_bounds__(BS) void k_cg_update(int n, T *psi, T *r, const T *p, const T *Ap, ggbk_scalars *s,
-						   double *part)
+__global__ __launch_bounds__(BS) void k_cg_update(int n, T *psi, T *r, const T *p, const T *Ap, const ggbk_scalars *s,
+						   ggbk_red red)
 {
 	if (!s->active)
 		return;
@@ -226,64 +303,22 @@ __global__ __launch_bounds__(BS) void k_cg_update(int n, T *psi, T *r, const T *
 		r[i] = (T) ri;
 		v = ri * ri;
 	}
-	double bsum = block_sum<BS>(v);
-	if (threadIdx.x == 0)
-		part[blockIdx.x] = bsum;
-	if (last_block(&s->ticket[4])) {
-		double rr = fold_partials<BS>(part, gridDim.x);
-		if (threadIdx.x == 0) {
-			s->rr = rr;
-			int it = s->it + 1;
-			s->it = it;
-			if (rr <= s->tol2) {
-				s->active = 0;
-				s->status = 0;
-			} else if (it >= s->maxit) {
-				s->active = 0;
-				s->status = 3;
-			}
-		}
-	}
-}
-
-template <typename T, typename TZ>
-__global__ __launch_bounds__(BS) void k_cg_rz(int n, const T *r, const TZ *z, ggbk_scalars *s, double *part)
-{
-	if (!s->active)
-		return;
-	int i = blockIdx.x * BS + threadIdx.x;
-	double v = i < n ? (double) r[i] * (double) z[i] : 0.0;
-	double bsum = block_sum<BS>(v);
-	if (threadIdx.x == 0)
-		part[blockIdx.x] = bsum;
-	if (last_block(&s->ticket[5])) {
-		double rznew = fold_partials<BS>(part, gridDim.x);
-		if (threadIdx.x == 0) {
-			s->beta = rznew / s->rz;
-			s->rz = rznew;
-		}
-	}
-}
-
-template <typename T, typename TZ>
-__global__ __launch_bounds__(BS) void k_cg_pupdate(int n, const TZ *z, T *p, const ggbk_scalars *s)
-{
-	if (!s->active)
-		return;
-	double beta = s->beta;
-	int i = blockIdx.x * BS + threadIdx.x;
-	if (i < n)
-		p[i] = (T) ((double) z[i] + beta * (double) p[i]);
+	store_partial(red, GGBK_SL_RR, v);
 }
 
 template <typename T>
-__global__ __launch_bounds__(BS) void k_jacobi_precond(ggbk_grid g, const T *r, T *z, const ggbk_scalars *s)
+__global__ __launch_bounds__(BS) void k_jacobi_precond(ggbk_grid g, const T *r, T *z, const ggbk_scalars *s, ggbk_red red)
 {
 	if (!s->active)
 		return;
 	int i = blockIdx.x * BS + threadIdx.x;
-	if (i < g.n)
-		z[i] = g.dinv[i] * r[i];
+	double v = 0;
+	if (i < g.n) {
+		T zi = g.dinv[i] * r[i];
+		z[i] = zi;
+		v = (double) r[i] * (double) zi;
+	}
+	store_partial(red, GGBK_SL_RZ, v);
 }
 
 template <typename T>
@@ -298,24 +333,34 @@ __global__ __launch_bounds__(BS) void k_zero_if_bb0(int n, T *psi, const ggbk_sc
 
 // ---------------- velocity and max speed ----------------
 
-// u = p x sum_k gc_k (psi_j - psi_i) (fluid.c:360-376); max |u|^2 of the stored values
-// in storage arithmetic (fluid.c:480-491) folded into s->vmax2.
+// [psi' = psi - mean] u = p x sum_k gc_k (psi'_j - psi'_i) (fluid.c:360-376); max |u|^2 of
+// the stored values in storage arithmetic (fluid.c:480-491).
 template <typename T>
-__global__ __launch_bounds__(BS) void k_velocity(ggbk_grid g, const T *psi, T *vel, ggbk_scalars *s, double *part)
+__global__ __launch_bounds__(BS) void k_velocity(ggbk_grid g, const T *psi_in, T *psi_out, int sub_mean, T *vel,
+						  const ggbk_scalars *s, ggbk_red red)
 {
 	int i = blockIdx.x * BS + threadIdx.x;
 	double m = 0;
 	if (i < g.n) {
+		T mean = sub_mean ? (T) s->mean : (T) 0;
+		T psii = psi_in[i];
+		if (sub_mean)
+			psii -= mean;
 		dv3 p = cell_pos(g, i);
 		dv3 gr = d3(0, 0, 0);
-		double pi = (double) psi[i];
+		double pi = (double) psii;
 		int nn = g.nnb[i];
 		for (int k = 0; k < nn; k++) {
 			int j = g.nbr[(size_t) k * g.n + i];
+			T pj = psi_in[j];
+			if (sub_mean)
+				pj -= mean;
 			dv3 c = d3(g.gc[((size_t) k * 3 + 0) * g.n + i], g.gc[((size_t) k * 3 + 1) * g.n + i],
 				   g.gc[((size_t) k * 3 + 2) * g.n + i]);
-			gr = vadd(gr, vscl(c, (double) psi[j] - pi));
+			gr = vadd(gr, vscl(c, (double) pj - pi));
 		}
+		if (psi_out)
+			psi_out[i] = psii;
 		dv3 u = vcross(p, gr);
 		T u0 = (T) u.x, u1 = (T) u.y, u2 = (T) u.z;
 		vel[3 * (size_t) i + 0] = u0;
@@ -324,18 +369,11 @@ __global__ __launch_bounds__(BS) void k_velocity(ggbk_grid g, const T *psi, T *v
 		T sq = u0 * u0 + u1 * u1 + u2 * u2;
 		m = (double) sq;
 	}
-	double bm = block_max<BS>(m);
-	if (threadIdx.x == 0)
-		part[blockIdx.x] = bm;
-	if (last_block(&s->ticket[6])) {
-		double v = fold_partials_max<BS>(part, gridDim.x);
-		if (threadIdx.x == 0)
-			s->vmax2 = v;
-	}
+	store_partial_max(red, GGBK_SL_VMAX, m);
 }
 
 template <typename T>
-__global__ __launch_bounds__(BS) void k_max_speed2(int n, const T *vel, ggbk_scalars *s, double *part)
+__global__ __launch_bounds__(BS) void k_max_speed2(int n, const T *vel, ggbk_red red)
 {
 	int i = blockIdx.x * BS + threadIdx.x;
 	double m = 0;
@@ -344,20 +382,14 @@ __global__ __launch_bounds__(BS) void k_max_speed2(int n, const T *vel, ggbk_sca
 		T sq = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
 		m = (double) sq;
 	}
-	double bm = block_max<BS>(m);
-	if (threadIdx.x == 0)
-		part[blockIdx.x] = bm;
-	if (last_block(&s->ticket[6])) {
-		double v = fold_partials_max<BS>(part, gridDim.x);
-		if (threadIdx.x == 0)
-			s->vmax2 = v;
-	}
+	store_partial_max(red, GGBK_SL_VMAX, m);
 }
 
 // ---------------- semi-Lagrangian advection (fluid.c:397-478) ----------------
 
 template <typename T>
-__global__ __launch_bounds__(BS) void k_departures(ggbk_grid g, const T *vel, T *dep_b, T *dep_f, double dt, int fwd)
+__global__ __launch_bounds__(BS) void k_departures(ggbk_grid g, const T *vel, const T *omega, T *q0, T *dep_b, T *dep_f,
+						    double dt, double twoOmega, int fwd)
 {
 	int i = blockIdx.x * BS + threadIdx.x;
 	if (i >= g.n)
@@ -372,14 +404,8 @@ __global__ __launch_bounds__(BS) void k_departures(ggbk_grid g, const T *vel, T 
 		um = vtan(sample_vec(g, vel, pm), p);
 		st3(dep_f + 3 * (size_t) i, gc_move(p, um, dt));
 	}
-}
-
-template <typename T>
-__global__ __launch_bounds__(BS) void k_absvort(ggbk_grid g, const T *omega, T *q, double twoOmega)
-{
-	int i = blockIdx.x * BS + threadIdx.x;
-	if (i < g.n)
-		q[i] = (T) ((double) omega[i] + twoOmega * (double) g.pos[3 * (size_t) i + 1]);
+	// absolute vorticity q = omega + 2 Omega y (fluid.c:435-437)
+	q0[i] = (T) ((double) omega[i] + twoOmega * (double) g.pos[3 * (size_t) i + 1]);
 }
 
 template <typename T>
@@ -390,54 +416,77 @@ __global__ __launch_bounds__(BS) void k_advect(ggbk_grid g, const T *src, const 
 		dst[i] = (T) sample_scalar(g, src, ld3(dep + 3 * (size_t) i));
 }
 
+// q2 = A_f q1, then the corrected source q2 = q + 0.5 (q - q2) in storage arithmetic
+// (fluid.c:445-448)
 template <typename T>
-__global__ __launch_bounds__(BS) void k_bfecc_source(int n, const T *q, T *q2)
-{
-	int i = blockIdx.x * BS + threadIdx.x;
-	if (i < n)
-		q2[i] = q[i] + (T) 0.5f * (q[i] - q2[i]);
-}
-
-template <typename T>
-__global__ __launch_bounds__(BS) void k_limited_advect(ggbk_grid g, const T *q2, const T *q, const T *dep_b, T *q1)
+__global__ __launch_bounds__(BS) void k_advect_bfecc(ggbk_grid g, const T *q1, const T *q, const T *dep_f, T *q2)
 {
 	int i = blockIdx.x * BS + threadIdx.x;
 	if (i >= g.n)
 		return;
-	Taps t = taps_at(g, ld3(dep_b + 3 * (size_t) i));
-	double wt[4];
-	tap_weights(t, wt);
-	double v = 0, lo = 1e30, hi = -1e30;
-	for (int k = 0; k < 4; k++) {
-		v += wt[k] * (double) q2[t.idx[k]];
-		double o = (double) q[t.idx[k]];
-		if (o < lo)
-			lo = o;
-		if (o > hi)
-			hi = o;
+	T a = (T) sample_scalar(g, q1, ld3(dep_f + 3 * (size_t) i));
+	T qi = q[i];
+	q2[i] = qi + (T) 0.5f * (qi - a);
+}
+
+// limited final advection (fluid.c:449-464), un-Coriolis and relaxation (fluid.c:468-474),
+// and the area-weighted partial of the new omega for the mean removal
+template <typename T>
+__global__ __launch_bounds__(BS) void k_limited_relax(ggbk_grid g, const T *q2, const T *q, const T *dep_b, const T *q1,
+						       int bfecc, const T *band, T *omega, double twoOmega, double relax,
+						       ggbk_red red)
+{
+	int i = blockIdx.x * BS + threadIdx.x;
+	double part = 0;
+	if (i < g.n) {
+		T qn;
+		if (bfecc) {
+			Taps t = taps_at(g, ld3(dep_b + 3 * (size_t) i));
+			double wt[4];
+			tap_weights(t, wt);
+			double v = 0, lo = 1e30, hi = -1e30;
+			for (int k = 0; k < 4; k++) {
+				v += wt[k] * (double) q2[t.idx[k]];
+				double o = (double) q[t.idx[k]];
+				if (o < lo)
+					lo = o;
+				if (o > hi)
+					hi = o;
+			}
+			if (v < lo)
+				v = lo;
+			if (v > hi)
+				v = hi;
+			qn = (T) v;
+		} else {
+			qn = q1[i];
+		}
+		double w = (double) qn - twoOmega * (double) g.pos[3 * (size_t) i + 1];
+		w += relax * ((double) band[i] - w);
+		T om = (T) w;
+		omega[i] = om;
+		part = (double) g.area[i] * (double) om;
 	}
-	if (v < lo)
-		v = lo;
-	if (v > hi)
-		v = hi;
-	q1[i] = (T) v;
-}
-
-template <typename T>
-__global__ __launch_bounds__(BS) void k_relax(ggbk_grid g, const T *qn, const T *band, T *omega, double twoOmega,
-					       double relax)
-{
-	int i = blockIdx.x * BS + threadIdx.x;
-	if (i >= g.n)
-		return;
-	double w = (double) qn[i] - twoOmega * (double) g.pos[3 * (size_t) i + 1];
-	w += relax * ((double) band[i] - w);
-	omega[i] = (T) w;
+	store_partial(red, GGBK_SL_MEAN, part);
 }
 
 // ---------------- launchers ----------------
 
 #define NB(n) ggbk_blocks((n), BS)
+#define DISPATCH_T(T, fn, ...)                                     \
+	do {                                                       \
+		if ((T) == GGBK_F64)                               \
+			GGBK_LAUNCH(fn<double>, __VA_ARGS__);      \
+		else                                               \
+			GGBK_LAUNCH(fn<float>, __VA_ARGS__);       \
+	} while (0)
+
+extern "C" int ggbk_nblocks(int n) { return (int) NB(n); }
+
+extern "C" void ggbk_fin(int op, ggbk_scalars *s, ggbk_red red, int nb, double tol, int maxit, double total_area)
+{
+	GGBK_LAUNCH(k_fin, 1, FIN_BS, 0, op, s, red, nb, tol, maxit, total_area);
+}
 
 extern "C" void ggbk_curl(ggbk_grid g, const void *vel, void *out, int T)
 {
@@ -471,88 +520,71 @@ extern "C" void ggbk_applyK(ggbk_grid g, const void *x, double *y, int T)
 		GGBK_LAUNCH(k_applyK<float>, NB(g.n), BS, 0, g, (const float *) x, y);
 }
 
-extern "C" void ggbk_remove_mean(ggbk_grid g, void *f, int T, ggbk_scalars *s, double *part)
-{
-	if (T == GGBK_F64) {
-		GGBK_LAUNCH(k_mean<double>, NB(g.n), BS, 0, g, (const double *) f, s, part);
-		GGBK_LAUNCH(k_sub_mean<double>, NB(g.n), BS, 0, g.n, (double *) f, (const ggbk_scalars *) s);
-	} else {
-		GGBK_LAUNCH(k_mean<float>, NB(g.n), BS, 0, g, (const float *) f, s, part);
-		GGBK_LAUNCH(k_sub_mean<float>, NB(g.n), BS, 0, g.n, (float *) f, (const ggbk_scalars *) s);
-	}
-}
-
-extern "C" void ggbk_cg_start(ggbk_grid g, const void *omega, const void *psi, void *r, int T, double tol,
-			      int maxit, ggbk_scalars *s, double *part)
+extern "C" void ggbk_mean_partial(ggbk_grid g, const void *f, int T, ggbk_red red)
 {
 	if (T == GGBK_F64)
-		GGBK_LAUNCH(k_cg_start<double>, NB(g.n), BS, 0, g, (const double *) omega, (const double *) psi,
-			    (double *) r, tol, maxit, s, part);
+		GGBK_LAUNCH(k_mean_partial<double>, NB(g.n), BS, 0, g, (const double *) f, red);
 	else
-		GGBK_LAUNCH(k_cg_start<float>, NB(g.n), BS, 0, g, (const float *) omega, (const float *) psi,
-			    (float *) r, tol, maxit, s, part);
+		GGBK_LAUNCH(k_mean_partial<float>, NB(g.n), BS, 0, g, (const float *) f, red);
 }
 
-extern "C" void ggbk_cg_first_dir(ggbk_grid g, const void *r, const void *z, int TZ, void *p, int T,
-				  ggbk_scalars *s, double *part)
+extern "C" void ggbk_sub_mean(int n, void *f, int T, const ggbk_scalars *s)
+{
+	if (T == GGBK_F64)
+		GGBK_LAUNCH(k_sub_mean<double>, NB(n), BS, 0, n, (double *) f, s);
+	else
+		GGBK_LAUNCH(k_sub_mean<float>, NB(n), BS, 0, n, (float *) f, s);
+}
+
+extern "C" void ggbk_remove_mean(ggbk_grid g, void *f, int T, ggbk_scalars *s, ggbk_red red)
+{
+	ggbk_mean_partial(g, f, T, red);
+	ggbk_fin(T == GGBK_F64 ? GGBK_FIN_MEAN_F64 : GGBK_FIN_MEAN_F32, s, red, (int) NB(g.n), 0, 0, g.total_area);
+	ggbk_sub_mean(g.n, f, T, s);
+}
+
+extern "C" void ggbk_cg_start(ggbk_grid g, void *omega, const void *psi, void *r, int T, int sub_mean,
+			      const ggbk_scalars *s, ggbk_red red)
+{
+	if (T == GGBK_F64)
+		GGBK_LAUNCH(k_cg_start<double>, NB(g.n), BS, 0, g, (double *) omega, (const double *) psi, (double *) r,
+			    sub_mean, s, red);
+	else
+		GGBK_LAUNCH(k_cg_start<float>, NB(g.n), BS, 0, g, (float *) omega, (const float *) psi, (float *) r,
+			    sub_mean, s, red);
+}
+
+extern "C" void ggbk_cg_Ap(ggbk_grid g, const void *z, int TZ, const void *p_in, void *p_out, void *Ap, int T,
+			   int first, const ggbk_scalars *s, ggbk_red red)
 {
 	if (T == GGBK_F64 && TZ == GGBK_F64)
-		GGBK_LAUNCH((k_cg_first_dir<double, double>), NB(g.n), BS, 0, g.n, (const double *) r,
-			    (const double *) z, (double *) p, s, part);
+		GGBK_LAUNCH((k_cg_Ap<double, double>), NB(g.n), BS, 0, g, (const double *) z, (const double *) p_in,
+			    (double *) p_out, (double *) Ap, first, s, red);
 	else if (T == GGBK_F64)
-		GGBK_LAUNCH((k_cg_first_dir<double, float>), NB(g.n), BS, 0, g.n, (const double *) r,
-			    (const float *) z, (double *) p, s, part);
+		GGBK_LAUNCH((k_cg_Ap<double, float>), NB(g.n), BS, 0, g, (const float *) z, (const double *) p_in,
+			    (double *) p_out, (double *) Ap, first, s, red);
 	else
-		GGBK_LAUNCH((k_cg_first_dir<float, float>), NB(g.n), BS, 0, g.n, (const float *) r,
-			    (const float *) z, (float *) p, s, part);
-}
-
-extern "C" void ggbk_cg_Ap(ggbk_grid g, const void *p, void *Ap, int T, ggbk_scalars *s, double *part)
-{
-	if (T == GGBK_F64)
-		GGBK_LAUNCH(k_cg_Ap<double>, NB(g.n), BS, 0, g, (const double *) p, (double *) Ap, s, part);
-	else
-		GGBK_LAUNCH(k_cg_Ap<float>, NB(g.n), BS, 0, g, (const float *) p, (float *) Ap, s, part);
+		GGBK_LAUNCH((k_cg_Ap<float, float>), NB(g.n), BS, 0, g, (const float *) z, (const float *) p_in,
+			    (float *) p_out, (float *) Ap, first, s, red);
 }
 
 extern "C" void ggbk_cg_update(ggbk_grid g, void *psi, void *r, const void *p, const void *Ap, int T,
-			       ggbk_scalars *s, double *part)
+			       const ggbk_scalars *s, ggbk_red red)
 {
 	if (T == GGBK_F64)
-		GGBK_LAUNCH(k_cg_update<double>, NB(g.n), BS, 0, g.n, (double *) psi, (double *) r,
-			    (const double *) p, (const double *) Ap, s, part);
+		GGBK_LAUNCH(k_cg_update<double>, NB(g.n), BS, 0, g.n, (double *) psi, (double *) r, (const double *) p,
+			    (const double *) Ap, s, red);
 	else
 		GGBK_LAUNCH(k_cg_update<float>, NB(g.n), BS, 0, g.n, (float *) psi, (float *) r, (const float *) p,
-			    (const float *) Ap, s, part);
+			    (const float *) Ap, s, red);
 }
 
-extern "C" void ggbk_cg_next_dir(ggbk_grid g, const void *r, const void *z, int TZ, void *p, int T,
-				 ggbk_scalars *s, double *part)
-{
-	if (T == GGBK_F64 && TZ == GGBK_F64) {
-		GGBK_LAUNCH((k_cg_rz<double, double>), NB(g.n), BS, 0, g.n, (const double *) r, (const double *) z,
-			    s, part);
-		GGBK_LAUNCH((k_cg_pupdate<double, double>), NB(g.n), BS, 0, g.n, (const double *) z, (double *) p,
-			    (const ggbk_scalars *) s);
-	} else if (T == GGBK_F64) {
-		GGBK_LAUNCH((k_cg_rz<double, float>), NB(g.n), BS, 0, g.n, (const double *) r, (const float *) z, s,
-			    part);
-		GGBK_LAUNCH((k_cg_pupdate<double, float>), NB(g.n), BS, 0, g.n, (const float *) z, (double *) p,
-			    (const ggbk_scalars *) s);
-	} else {
-		GGBK_LAUNCH((k_cg_rz<float, float>), NB(g.n), BS, 0, g.n, (const float *) r, (const float *) z, s,
-			    part);
-		GGBK_LAUNCH((k_cg_pupdate<float, float>), NB(g.n), BS, 0, g.n, (const float *) z, (float *) p,
-			    (const ggbk_scalars *) s);
-	}
-}
-
-extern "C" void ggbk_jacobi_precond(ggbk_grid g, const void *r, void *z, int T, const ggbk_scalars *s)
+extern "C" void ggbk_jacobi_precond(ggbk_grid g, const void *r, void *z, int T, const ggbk_scalars *s, ggbk_red red)
 {
 	if (T == GGBK_F64)
-		GGBK_LAUNCH(k_jacobi_precond<double>, NB(g.n), BS, 0, g, (const double *) r, (double *) z, s);
+		GGBK_LAUNCH(k_jacobi_precond<double>, NB(g.n), BS, 0, g, (const double *) r, (double *) z, s, red);
 	else
-		GGBK_LAUNCH(k_jacobi_precond<float>, NB(g.n), BS, 0, g, (const float *) r, (float *) z, s);
+		GGBK_LAUNCH(k_jacobi_precond<float>, NB(g.n), BS, 0, g, (const float *) r, (float *) z, s, red);
 }
 
 extern "C" void ggbk_cg_zero_if_bb0(ggbk_grid g, void *psi, int T, const ggbk_scalars *s)
@@ -563,38 +595,34 @@ extern "C" void ggbk_cg_zero_if_bb0(ggbk_grid g, void *psi, int T, const ggbk_sc
 		GGBK_LAUNCH(k_zero_if_bb0<float>, NB(g.n), BS, 0, g.n, (float *) psi, s);
 }
 
-extern "C" void ggbk_velocity(ggbk_grid g, const void *psi, void *vel, int T, ggbk_scalars *s, double *part)
+extern "C" void ggbk_velocity(ggbk_grid g, const void *psi_in, void *psi_out, int sub_mean, void *vel, int T,
+			      const ggbk_scalars *s, ggbk_red red)
 {
 	if (T == GGBK_F64)
-		GGBK_LAUNCH(k_velocity<double>, NB(g.n), BS, 0, g, (const double *) psi, (double *) vel, s, part);
+		GGBK_LAUNCH(k_velocity<double>, NB(g.n), BS, 0, g, (const double *) psi_in, (double *) psi_out, sub_mean,
+			    (double *) vel, s, red);
 	else
-		GGBK_LAUNCH(k_velocity<float>, NB(g.n), BS, 0, g, (const float *) psi, (float *) vel, s, part);
+		GGBK_LAUNCH(k_velocity<float>, NB(g.n), BS, 0, g, (const float *) psi_in, (float *) psi_out, sub_mean,
+			    (float *) vel, s, red);
 }
 
-extern "C" void ggbk_max_speed2(ggbk_grid g, const void *vel, int T, ggbk_scalars *s, double *part)
+extern "C" void ggbk_max_speed2(ggbk_grid g, const void *vel, int T, ggbk_red red)
 {
 	if (T == GGBK_F64)
-		GGBK_LAUNCH(k_max_speed2<double>, NB(g.n), BS, 0, g.n, (const double *) vel, s, part);
+		GGBK_LAUNCH(k_max_speed2<double>, NB(g.n), BS, 0, g.n, (const double *) vel, red);
 	else
-		GGBK_LAUNCH(k_max_speed2<float>, NB(g.n), BS, 0, g.n, (const float *) vel, s, part);
+		GGBK_LAUNCH(k_max_speed2<float>, NB(g.n), BS, 0, g.n, (const float *) vel, red);
 }
 
-extern "C" void ggbk_departures(ggbk_grid g, const void *vel, void *dep_b, void *dep_f, int T, double dt, int fwd)
+extern "C" void ggbk_departures(ggbk_grid g, const void *vel, const void *omega, void *q0, void *dep_b, void *dep_f,
+				int T, double dt, double twoOmega, int fwd)
 {
 	if (T == GGBK_F64)
-		GGBK_LAUNCH(k_departures<double>, NB(g.n), BS, 0, g, (const double *) vel, (double *) dep_b,
-			    (double *) dep_f, dt, fwd);
+		GGBK_LAUNCH(k_departures<double>, NB(g.n), BS, 0, g, (const double *) vel, (const double *) omega,
+			    (double *) q0, (double *) dep_b, (double *) dep_f, dt, twoOmega, fwd);
 	else
-		GGBK_LAUNCH(k_departures<float>, NB(g.n), BS, 0, g, (const float *) vel, (float *) dep_b,
-			    (float *) dep_f, dt, fwd);
-}
-
-extern "C" void ggbk_absvort(ggbk_grid g, const void *omega, void *q, int T, double twoOmega)
-{
-	if (T == GGBK_F64)
-		GGBK_LAUNCH(k_absvort<double>, NB(g.n), BS, 0, g, (const double *) omega, (double *) q, twoOmega);
-	else
-		GGBK_LAUNCH(k_absvort<float>, NB(g.n), BS, 0, g, (const float *) omega, (float *) q, twoOmega);
+		GGBK_LAUNCH(k_departures<float>, NB(g.n), BS, 0, g, (const float *) vel, (const float *) omega,
+			    (float *) q0, (float *) dep_b, (float *) dep_f, dt, twoOmega, fwd);
 }
 
 extern "C" void ggbk_advect(ggbk_grid g, const void *src, const void *dep, void *dst, int T)
@@ -607,31 +635,26 @@ extern "C" void ggbk_advect(ggbk_grid g, const void *src, const void *dep, void 
 			    (float *) dst);
 }
 
-extern "C" void ggbk_bfecc_source(int n, const void *q, void *q2, int T)
+extern "C" void ggbk_advect_bfecc(ggbk_grid g, const void *q1, const void *q, const void *dep_f, void *q2, int T)
 {
 	if (T == GGBK_F64)
-		GGBK_LAUNCH(k_bfecc_source<double>, NB(n), BS, 0, n, (const double *) q, (double *) q2);
+		GGBK_LAUNCH(k_advect_bfecc<double>, NB(g.n), BS, 0, g, (const double *) q1, (const double *) q,
+			    (const double *) dep_f, (double *) q2);
 	else
-		GGBK_LAUNCH(k_bfecc_source<float>, NB(n), BS, 0, n, (const float *) q, (float *) q2);
+		GGBK_LAUNCH(k_advect_bfecc<float>, NB(g.n), BS, 0, g, (const float *) q1, (const float *) q,
+			    (const float *) dep_f, (float *) q2);
 }
 
-extern "C" void ggbk_limited_advect(ggbk_grid g, const void *q2, const void *q, const void *dep_b, void *q1, int T)
+extern "C" void ggbk_limited_relax(ggbk_grid g, const void *q2, const void *q, const void *dep_b, const void *q1,
+				   int bfecc, const void *band, void *omega, int T, double twoOmega, double relax,
+				   ggbk_red red)
 {
 	if (T == GGBK_F64)
-		GGBK_LAUNCH(k_limited_advect<double>, NB(g.n), BS, 0, g, (const double *) q2, (const double *) q,
-			    (const double *) dep_b, (double *) q1);
+		GGBK_LAUNCH(k_limited_relax<double>, NB(g.n), BS, 0, g, (const double *) q2, (const double *) q,
+			    (const double *) dep_b, (const double *) q1, bfecc, (const double *) band, (double *) omega,
+			    twoOmega, relax, red);
 	else
-		GGBK_LAUNCH(k_limited_advect<float>, NB(g.n), BS, 0, g, (const float *) q2, (const float *) q,
-			    (const float *) dep_b, (float *) q1);
-}
-
-extern "C" void ggbk_relax(ggbk_grid g, const void *qn, const void *band, void *omega, int T, double twoOmega,
-			   double relax)
-{
-	if (T == GGBK_F64)
-		GGBK_LAUNCH(k_relax<double>, NB(g.n), BS, 0, g, (const double *) qn, (const double *) band,
-			    (double *) omega, twoOmega, relax);
-	else
-		GGBK_LAUNCH(k_relax<float>, NB(g.n), BS, 0, g, (const float *) qn, (const float *) band,
-			    (float *) omega, twoOmega, relax);
+		GGBK_LAUNCH(k_limited_relax<float>, NB(g.n), BS, 0, g, (const float *) q2, (const float *) q,
+			    (const float *) dep_b, (const float *) q1, bfecc, (const float *) band, (float *) omega,
+			    twoOmega, relax, red);
 }

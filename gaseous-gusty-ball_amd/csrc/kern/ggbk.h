@@ -38,13 +38,13 @@ typedef struct {
 
 /* bilinear prolongation from level l+1 to l, and its transpose */
 typedef struct {
-	int nf, nc;
+	int nf, nc, rmax, pad_;
 	const int *ptap;   /* [4][nf] coarse taps of each fine cell */
 	const float *pw;   /* [4][nf] */
-	const int *rstart; /* [nc+1] CSR rows of R = P^T, fine index ascending */
-	const int *ridx;   /* [4 nf] */
-	const float *rw;   /* [4 nf] */
+	const int *ridx;   /* [rmax][nc] rows of R = P^T, fine index ascending, zero-weight padded */
+	const float *rw;   /* [rmax][nc] */
 } ggbk_xfer;
+#define GGBK_RMAX 24
 
 /* Device-resident solver scalars.  Reductions write per-block partials; the last block
  * to finish folds them in block order (deterministic) and updates these. */
@@ -91,53 +91,84 @@ void ggbk_curl(ggbk_grid g, const void *vel, void *out, int T);
 void ggbk_divergence(ggbk_grid g, const void *vel, void *out, int T);
 void ggbk_laplacian(ggbk_grid g, const void *x, void *y, int T);
 void ggbk_applyK(ggbk_grid g, const void *x, double *y, int T);
-/* f -= area-weighted mean (mean rounded to T, as the reference rounds to float) */
-void ggbk_remove_mean(ggbk_grid g, void *f, int T, ggbk_scalars *s, double *partials);
+
+/* ---- reductions ----
+ * Reduction kernels store one partial per block in slot `slot` of a partials array
+ * (slot stride ggbk_red.stride doubles); a one-workgroup fold kernel then combines the
+ * partials in block order and updates ggbk_scalars.  Deterministic, no cross-block
+ * synchronisation inside a kernel. */
+typedef struct {
+	double *part;
+	int stride;
+} ggbk_red;
+enum { GGBK_SL_BB = 0, GGBK_SL_RR0, GGBK_SL_PAP, GGBK_SL_RR, GGBK_SL_RZ, GGBK_SL_MEAN, GGBK_SL_VMAX, GGBK_NSLOTS };
+enum {
+	GGBK_FIN_START = 0, /* bb, rr -> tol2, active, status */
+	GGBK_FIN_PAP,       /* pAp -> alpha, or stop */
+	GGBK_FIN_RR,        /* rr -> it++, stop on convergence or maxit */
+	GGBK_FIN_RZ_FIRST,  /* rz */
+	GGBK_FIN_RZ,        /* rznew -> beta */
+	GGBK_FIN_MEAN_F64,  /* mean = sum / total_area */
+	GGBK_FIN_MEAN_F32,  /* mean rounded to float */
+	GGBK_FIN_VMAX,      /* vmax2 = max */
+};
+/* nb = number of blocks that wrote partials */
+void ggbk_fin(int op, ggbk_scalars *s, ggbk_red red, int nb, double tol, int maxit, double total_area);
+int ggbk_nblocks(int n);
+
+/* f's area-weighted sum partials (slot MEAN); fold with FIN_MEAN_* */
+void ggbk_mean_partial(ggbk_grid g, const void *f, int T, ggbk_red red);
+void ggbk_sub_mean(int n, void *f, int T, const ggbk_scalars *s);
+/* f -= area-weighted mean, rounded to T as the reference rounds it (fluid.c:264-275) */
+void ggbk_remove_mean(ggbk_grid g, void *f, int T, ggbk_scalars *s, ggbk_red red);
 
 /* ---- PCG (reference fluid.c:288-358) ---- */
-/* r = b - K psi with b = -area*omega; bb, rr; sets tol2, it = 0, active, status */
-void ggbk_cg_start(ggbk_grid g, const void *omega, const void *psi, void *r, int T,
-		   double tol, int maxit, ggbk_scalars *s, double *partials);
-/* p = z; rz = r.z  (guarded by active) */
-void ggbk_cg_first_dir(ggbk_grid g, const void *r, const void *z, int TZ, void *p, int T,
-		       ggbk_scalars *s, double *partials);
-/* Ap = K p; pAp; alpha = rz / pAp or stop on pAp <= 0 */
-void ggbk_cg_Ap(ggbk_grid g, const void *p, void *Ap, int T, ggbk_scalars *s, double *partials);
-/* psi += alpha p; r -= alpha Ap; rr; it++; stop on rr <= tol2 or it == maxit */
+/* [omega -= mean if sub_mean] r = b - K psi with b = -area*omega; bb, rr partials */
+void ggbk_cg_start(ggbk_grid g, void *omega, const void *psi, void *r, int T, int sub_mean,
+		   const ggbk_scalars *s, ggbk_red red);
+/* p_out = z (first) or z + beta p_in; Ap = K p_out; pAp partial (guarded by active) */
+void ggbk_cg_Ap(ggbk_grid g, const void *z, int TZ, const void *p_in, void *p_out, void *Ap, int T,
+		int first, const ggbk_scalars *s, ggbk_red red);
+/* psi += alpha p; r -= alpha Ap; rr partial */
 void ggbk_cg_update(ggbk_grid g, void *psi, void *r, const void *p, const void *Ap, int T,
-		    ggbk_scalars *s, double *partials);
-/* rznew = r.z; beta = rznew / rz; rz = rznew; p = z + beta p */
-void ggbk_cg_next_dir(ggbk_grid g, const void *r, const void *z, int TZ, void *p, int T,
-		      ggbk_scalars *s, double *partials);
-/* Jacobi preconditioner z = dinv r (no multigrid) */
-void ggbk_jacobi_precond(ggbk_grid g, const void *r, void *z, int T, const ggbk_scalars *s);
+		    const ggbk_scalars *s, ggbk_red red);
+/* Jacobi preconditioner z = dinv r and the r.z partial (no multigrid) */
+void ggbk_jacobi_precond(ggbk_grid g, const void *r, void *z, int T, const ggbk_scalars *s, ggbk_red red);
 /* bb == 0 exit: psi = 0 when status says so */
 void ggbk_cg_zero_if_bb0(ggbk_grid g, void *psi, int T, const ggbk_scalars *s);
 
 /* ---- multigrid V-cycle (reference fluid.c:100-146) ---- */
-/* Level-0 input r (type T) -> output z (type TM), all intermediate arrays of type TM.
- * levels[l].x/b/r are the per-level TM scratch; level 0's b is the input r. */
+/* Level-0 input r (type T) -> output z (type TM), intermediate arrays of type TM.
+ * levels[l].x/b/r are the per-level TM scratch; level 0's x and b are unused.  A level
+ * with M != NULL applies the precomputed matrix of the V-cycle below it (x = M b;
+ * M is [n][n] row-major, double when TM is double else float). */
 typedef struct {
 	ggbk_grid g;
 	ggbk_xfer xf;     /* transfer to the next coarser level (unused on the coarsest) */
-	void *x, *b, *r;  /* TM scratch (level 0: x = z output, r = residual scratch) */
+	void *x, *b, *r;  /* TM scratch */
+	const void *M;    /* dense sub-V-cycle operator, or NULL */
 } ggbk_level;
+/* z = V(r); with red.part != NULL also the r.z partial (slot RZ) of ggbk_nblocks(n0) blocks */
 void ggbk_vcycle(const ggbk_level *levels, int nlev, const void *rin, int T, void *zout, int TM,
-		 const ggbk_scalars *s);
+		 const ggbk_scalars *s, ggbk_red red);
 
 /* ---- velocity (reference fluid.c:360-376) + max speed (fluid.c:480-491) ---- */
-void ggbk_velocity(ggbk_grid g, const void *psi, void *vel, int T, ggbk_scalars *s, double *partials);
-/* max |vel|^2 into s->vmax2 */
-void ggbk_max_speed2(ggbk_grid g, const void *vel, int T, ggbk_scalars *s, double *partials);
+/* [psi_out = psi_in - mean] vel = p x grad(psi_out); vmax partial (slot VMAX) */
+void ggbk_velocity(ggbk_grid g, const void *psi_in, void *psi_out, int sub_mean, void *vel, int T,
+		   const ggbk_scalars *s, ggbk_red red);
+void ggbk_max_speed2(ggbk_grid g, const void *vel, int T, ggbk_red red);
 
 /* ---- semi-Lagrangian step (reference fluid.c:397-478) ---- */
-void ggbk_departures(ggbk_grid g, const void *vel, void *dep_b, void *dep_f, int T, double dt, int fwd);
-void ggbk_absvort(ggbk_grid g, const void *omega, void *q, int T, double twoOmega);
+/* departure points and q0 = omega + 2 Omega y */
+void ggbk_departures(ggbk_grid g, const void *vel, const void *omega, void *q0, void *dep_b, void *dep_f,
+		     int T, double dt, double twoOmega, int fwd);
 void ggbk_advect(ggbk_grid g, const void *src, const void *dep, void *dst, int T);
-void ggbk_bfecc_source(int n, const void *q, void *q2, int T);
-void ggbk_limited_advect(ggbk_grid g, const void *q2, const void *q, const void *dep_b, void *q1, int T);
-void ggbk_relax(ggbk_grid g, const void *qn, const void *omega_band, void *omega, int T,
-		double twoOmega, double relax);
+/* q2 = q + 0.5 (q - A_f q1) */
+void ggbk_advect_bfecc(ggbk_grid g, const void *q1, const void *q, const void *dep_f, void *q2, int T);
+/* qn = limited A_b q2 (bfecc) or qn = q1 (plain); omega = qn - 2 Omega y relaxed to band;
+ * area-weighted omega partial (slot MEAN) */
+void ggbk_limited_relax(ggbk_grid g, const void *q2, const void *q, const void *dep_b, const void *q1, int bfecc,
+			const void *band, void *omega, int T, double twoOmega, double relax, ggbk_red red);
 
 /* ---- particles (reference particles.c) ---- */
 void ggbk_particles_init(int n, double *pos, double *prev, float *col, int round32);
@@ -157,9 +188,11 @@ typedef struct {
 	long long cap;
 	int *sort_scratch;          /* [scratch_cap] for tiles whose list exceeds the LDS sort */
 	long long scratch_cap;
-	long long *total;           /* [1] entries of the last deposit (device) */
+	long long *total;           /* [8] entries, longest list, draw diagnostics (device) */
 } ggbk_render;
 void ggbk_render_scale(float *buf, size_t count, float g);
+/* copies the 8 diagnostics of the last deposit (synchronises) */
+int ggbk_render_stats(const ggbk_render *r, long long *out8);
 /* bin + ordered blend; returns the number of (particle, tile) entries, or -1 on error */
 long long ggbk_render_deposit(ggbk_render *r, int np, const double *prev, const double *pos,
 			      const float *col, float invg);

@@ -1,7 +1,15 @@
 // mg_k.hip -- geometric multigrid V(1,1) preconditioner on the cubed-sphere level
-// hierarchy (reference src/fluid.c:100-146): damped Jacobi (omega = 0.8), residual,
-// restriction R = P^T as a CSR gather (fine index order, deterministic), bilinear
-// prolongation, and the coarsest level's 60 sweeps in one LDS-resident workgroup.
+// hierarchy (reference src/fluid.c:100-146): damped Jacobi (omega = 0.8), one pre- and
+// one post-sweep, restriction R = P^T, bilinear prolongation, 60 coarsest sweeps.
+//
+// Three passes per level instead of the reference's seven loops:
+//   down     r1 = b - K x1 with x1 = 0.8 dinv b (the pre-sweep from x = 0) recomputed at
+//            the neighbours, so x1 is never stored;
+//   restrict b_coarse = R r1 as a padded-row gather in fine-index order (deterministic);
+//   up       x = x1 + P x_coarse, then x_out = x + 0.8 dinv (b - K x), with x recomputed at
+//            the neighbours (out of place), plus the r.z partial on level 0 for PCG.
+// Below the first level with at most DENSE_MAX cells the V-cycle is a fixed linear map;
+// it is precomputed on the host and applied as one dense matrix-vector product.
 //
 // TM is the level storage type (double in FP64 mode, float in the mixed and reference
 // modes); TB is the type of the level-0 right-hand side (the CG residual).  Per-cell
@@ -16,30 +24,128 @@
 #define MG_COARSE_SWEEPS 60
 #define COARSE_LDS_CELLS 3072
 #define COARSE_BS 1024
+#define DENSE_BS 256
 
-template <typename TX>
-__device__ __forceinline__ double kx_at(const ggbk_grid &g, const TX *x, int i)
+// pre-sweep value at cell j: r = (TM) b, x1 = 0 + (TM)(0.8 dinv r)
+template <typename TM, typename TB>
+__device__ __forceinline__ TM x1_at(const ggbk_grid &g, const TB *b, int j)
 {
-	double xi = (double) x[i], s = 0;
-	int nn = g.nnb[i];
-	for (int k = 0; k < nn; k++) {
-		int j = g.nbr[(size_t) k * g.n + i];
-		s += (double) g.w[(size_t) k * g.n + i] * (xi - (double) x[j]);
-	}
-	return s;
+	TM rj = (TM) (double) b[j];
+	return (TM) 0 + (TM) (MG_OMEGA * (double) g.dinv[j] * (double) rj);
 }
 
-// First sweep from x = 0: r = b, x = 0.8 dinv r (the stencil of a zero vector vanishes).
 template <typename TM, typename TB>
-__global__ __launch_bounds__(BS) void k_mg_first(ggbk_grid g, const TB *b, TM *x, const ggbk_scalars *s)
+__global__ __launch_bounds__(BS) void k_mg_down(ggbk_grid g, const TB *b, TM *r1, const ggbk_scalars *s)
 {
 	if (!s->active)
 		return;
 	int i = blockIdx.x * BS + threadIdx.x;
 	if (i >= g.n)
 		return;
-	TM ri = (TM) ((double) b[i] - 0.0);
-	x[i] = (TM) 0 + (TM) (MG_OMEGA * (double) g.dinv[i] * (double) ri);
+	double xi = (double) x1_at<TM, TB>(g, b, i), sum = 0;
+	int nn = g.nnb[i];
+	for (int k = 0; k < nn; k++) {
+		int j = g.nbr[(size_t) k * g.n + i];
+		sum += (double) g.w[(size_t) k * g.n + i] * (xi - (double) x1_at<TM, TB>(g, b, j));
+	}
+	r1[i] = (TM) ((double) b[i] - sum);
+}
+
+// coarse b = sum over the row of R of rw * r_fine (product in storage precision,
+// accumulated in double, fine-index order: fluid.c:130-136).  Rows are stored padded to
+// rmax with zero weights, so all loads of a row are independent.
+template <typename TM>
+__global__ __launch_bounds__(BS) void k_mg_restrict(ggbk_xfer xf, const TM *rf, TM *bc, const ggbk_scalars *s)
+{
+	if (!s->active)
+		return;
+	int c = blockIdx.x * BS + threadIdx.x;
+	if (c >= xf.nc)
+		return;
+	size_t nc = (size_t) xf.nc;
+	int idx[GGBK_RMAX];
+	float w[GGBK_RMAX];
+#pragma unroll
+	for (int k = 0; k < GGBK_RMAX; k++)
+		if (k < xf.rmax) {
+			idx[k] = xf.ridx[k * nc + c];
+			w[k] = xf.rw[k * nc + c];
+		}
+	double acc = 0;
+#pragma unroll
+	for (int k = 0; k < GGBK_RMAX; k++)
+		if (k < xf.rmax)
+			acc += (double) (w[k] * rf[idx[k]]);
+	bc[c] = (TM) acc;
+}
+
+// x after prolongation at cell j: x1 + (TM)(1.0 * sum_k pw_k xc[tap_k]) with the tap
+// products and sum in storage precision (fluid.c:138-144)
+template <typename TM, typename TB>
+__device__ __forceinline__ TM xcorr_at(const ggbk_grid &g, const ggbk_xfer &xf, const TB *b, const TM *xc, int j)
+{
+	size_t nf = (size_t) xf.nf;
+	TM v = xf.pw[j] * xc[xf.ptap[j]] + xf.pw[nf + j] * xc[xf.ptap[nf + j]] +
+	       xf.pw[2 * nf + j] * xc[xf.ptap[2 * nf + j]] + xf.pw[3 * nf + j] * xc[xf.ptap[3 * nf + j]];
+	return x1_at<TM, TB>(g, b, j) + (TM) (1.0 * (double) v);
+}
+
+template <typename TM, typename TB>
+__global__ __launch_bounds__(BS) void k_mg_up(ggbk_grid g, ggbk_xfer xf, const TB *b, const TM *xc, TM *xout,
+					       const ggbk_scalars *s, ggbk_red red)
+{
+	if (!s->active)
+		return;
+	int i = blockIdx.x * BS + threadIdx.x;
+	double dot = 0;
+	if (i < g.n) {
+		TM xi = xcorr_at<TM, TB>(g, xf, b, xc, i);
+		double sum = 0;
+		int nn = g.nnb[i];
+		for (int k = 0; k < nn; k++) {
+			int j = g.nbr[(size_t) k * g.n + i];
+			sum += (double) g.w[(size_t) k * g.n + i] * ((double) xi - (double) xcorr_at<TM, TB>(g, xf, b, xc, j));
+		}
+		TM ri = (TM) ((double) b[i] - sum);
+		TM xo = xi + (TM) (MG_OMEGA * (double) g.dinv[i] * (double) ri);
+		xout[i] = xo;
+		dot = (double) b[i] * (double) xo;
+	}
+	if (red.part) {
+		double bsum = block_sum<BS>(dot);
+		if (threadIdx.x == 0)
+			red.part[(size_t) GGBK_SL_RZ * red.stride + blockIdx.x] = bsum;
+	}
+}
+
+// x = M b, one wave per row, rows split over blocks; the row sum is a fixed tree.
+template <typename TM, typename TD, typename TB>
+__global__ __launch_bounds__(DENSE_BS) void k_mg_dense(const TD *M, int n, const TB *b, TM *x, const ggbk_scalars *s)
+{
+	if (!s->active)
+		return;
+	int row = blockIdx.x * (DENSE_BS / 64) + (threadIdx.x >> 6);
+	int lane = threadIdx.x & 63;
+	if (row >= n)
+		return;
+	const TD *m = M + (size_t) row * n;
+	double acc = 0;
+	for (int c = lane; c < n; c += 64)
+		acc += (double) m[c] * (double) b[c];
+	acc = wave_sum(acc);
+	if (lane == 0)
+		x[row] = (TM) acc;
+}
+
+// Fallback for a large coarsest level: sweeps as separate passes.
+template <typename TM, typename TB>
+__global__ __launch_bounds__(BS) void k_mg_first(ggbk_grid g, const TB *b, TM *x, const ggbk_scalars *s)
+{
+	if (!s->active)
+		return;
+	int i = blockIdx.x * BS + threadIdx.x;
+	if (i < g.n)
+		x[i] = x1_at<TM, TB>(g, b, i);
 }
 
 template <typename TM, typename TB>
@@ -49,8 +155,15 @@ __global__ __launch_bounds__(BS) void k_mg_residual(ggbk_grid g, const TB *b, co
 	if (!s->active)
 		return;
 	int i = blockIdx.x * BS + threadIdx.x;
-	if (i < g.n)
-		r[i] = (TM) ((double) b[i] - kx_at(g, x, i));
+	if (i >= g.n)
+		return;
+	double xi = (double) x[i], sum = 0;
+	int nn = g.nnb[i];
+	for (int k = 0; k < nn; k++) {
+		int j = g.nbr[(size_t) k * g.n + i];
+		sum += (double) g.w[(size_t) k * g.n + i] * (xi - (double) x[j]);
+	}
+	r[i] = (TM) ((double) b[i] - sum);
 }
 
 template <typename TM>
@@ -63,39 +176,7 @@ __global__ __launch_bounds__(BS) void k_mg_jacobi(ggbk_grid g, TM *x, const TM *
 		x[i] += (TM) (MG_OMEGA * (double) g.dinv[i] * (double) r[i]);
 }
 
-// coarse b = sum over the CSR row of rw * r_fine (product in storage precision,
-// accumulated in double: fluid.c:130-136)
-template <typename TM>
-__global__ __launch_bounds__(BS) void k_mg_restrict(ggbk_xfer xf, const TM *rf, TM *bc, const ggbk_scalars *s)
-{
-	if (!s->active)
-		return;
-	int c = blockIdx.x * BS + threadIdx.x;
-	if (c >= xf.nc)
-		return;
-	double acc = 0;
-	int e = xf.rstart[c + 1];
-	for (int k = xf.rstart[c]; k < e; k++)
-		acc += (double) (xf.rw[k] * rf[xf.ridx[k]]);
-	bc[c] = (TM) acc;
-}
-
-// x += P xc with the 4 stored taps (fluid.c:138-144; sum in storage precision)
-template <typename TM>
-__global__ __launch_bounds__(BS) void k_mg_prolong(ggbk_xfer xf, const TM *xc, TM *x, const ggbk_scalars *s)
-{
-	if (!s->active)
-		return;
-	int i = blockIdx.x * BS + threadIdx.x;
-	if (i >= xf.nf)
-		return;
-	size_t nf = (size_t) xf.nf;
-	TM v = xf.pw[i] * xc[xf.ptap[i]] + xf.pw[nf + i] * xc[xf.ptap[nf + i]] +
-	       xf.pw[2 * nf + i] * xc[xf.ptap[2 * nf + i]] + xf.pw[3 * nf + i] * xc[xf.ptap[3 * nf + i]];
-	x[i] += (TM) (1.0 * (double) v);
-}
-
-// Coarsest level: all sweeps in one workgroup with x and r in LDS.
+// Coarsest level with all sweeps in one workgroup, x and r in LDS.
 template <typename TM, typename TB>
 __global__ __launch_bounds__(COARSE_BS) void k_mg_coarse_lds(ggbk_grid g, const TB *b, TM *xout, const ggbk_scalars *s,
 							     int sweeps)
@@ -105,15 +186,19 @@ __global__ __launch_bounds__(COARSE_BS) void k_mg_coarse_lds(ggbk_grid g, const 
 	__shared__ TM xs[COARSE_LDS_CELLS];
 	__shared__ TM rs[COARSE_LDS_CELLS];
 	int n = g.n;
-	for (int i = threadIdx.x; i < n; i += COARSE_BS) {
-		TM ri = (TM) ((double) b[i] - 0.0);
-		rs[i] = ri;
-		xs[i] = (TM) 0 + (TM) (MG_OMEGA * (double) g.dinv[i] * (double) ri);
-	}
+	for (int i = threadIdx.x; i < n; i += COARSE_BS)
+		xs[i] = x1_at<TM, TB>(g, b, i);
 	__syncthreads();
 	for (int sw = 1; sw < sweeps; sw++) {
-		for (int i = threadIdx.x; i < n; i += COARSE_BS)
-			rs[i] = (TM) ((double) b[i] - kx_at(g, xs, i));
+		for (int i = threadIdx.x; i < n; i += COARSE_BS) {
+			double xi = (double) xs[i], sum = 0;
+			int nn = g.nnb[i];
+			for (int k = 0; k < nn; k++) {
+				int j = g.nbr[(size_t) k * g.n + i];
+				sum += (double) g.w[(size_t) k * g.n + i] * (xi - (double) xs[j]);
+			}
+			rs[i] = (TM) ((double) b[i] - sum);
+		}
 		__syncthreads();
 		for (int i = threadIdx.x; i < n; i += COARSE_BS)
 			xs[i] += (TM) (MG_OMEGA * (double) g.dinv[i] * (double) rs[i]);
@@ -123,52 +208,70 @@ __global__ __launch_bounds__(COARSE_BS) void k_mg_coarse_lds(ggbk_grid g, const 
 		xout[i] = xs[i];
 }
 
+template <typename TM, typename TB>
+__global__ __launch_bounds__(BS) void k_dot_partial(int n, const TB *r, const TM *z, const ggbk_scalars *s, ggbk_red red)
+{
+	if (!s->active)
+		return;
+	int i = blockIdx.x * BS + threadIdx.x;
+	double v = i < n ? (double) r[i] * (double) z[i] : 0.0;
+	double bsum = block_sum<BS>(v);
+	if (threadIdx.x == 0)
+		red.part[(size_t) GGBK_SL_RZ * red.stride + blockIdx.x] = bsum;
+}
+
 #define NB(n) ggbk_blocks((n), BS)
 
 template <typename TM, typename TB>
-static void smooth_from_zero(const ggbk_level &L, const TB *b, TM *x, TM *r, int sweeps, const ggbk_scalars *s)
+static void coarsest(const ggbk_level &L, const TB *b, TM *x, TM *r, const ggbk_scalars *s)
 {
 	const ggbk_grid &g = L.g;
-	if (sweeps > 1 && g.n <= COARSE_LDS_CELLS) {
-		GGBK_LAUNCH((k_mg_coarse_lds<TM, TB>), 1, COARSE_BS, 0, g, b, x, s, sweeps);
+	if (g.n <= COARSE_LDS_CELLS) {
+		GGBK_LAUNCH((k_mg_coarse_lds<TM, TB>), 1, COARSE_BS, 0, g, b, x, s, MG_COARSE_SWEEPS);
 		return;
 	}
 	GGBK_LAUNCH((k_mg_first<TM, TB>), NB(g.n), BS, 0, g, b, x, s);
-	for (int sw = 1; sw < sweeps; sw++) {
+	for (int sw = 1; sw < MG_COARSE_SWEEPS; sw++) {
 		GGBK_LAUNCH((k_mg_residual<TM, TB>), NB(g.n), BS, 0, g, b, (const TM *) x, r, s);
 		GGBK_LAUNCH((k_mg_jacobi<TM>), NB(g.n), BS, 0, g, x, (const TM *) r, s);
 	}
 }
 
 template <typename TM, typename TB>
-static void vcycle_level(const ggbk_level *lv, int nlev, int l, const TB *b, TM *x, TM *r, const ggbk_scalars *s)
+static void vcycle_level(const ggbk_level *lv, int nlev, int l, const TB *b, TM *x, const ggbk_scalars *s, ggbk_red red)
 {
 	const ggbk_level &L = lv[l];
 	const ggbk_grid &g = L.g;
-	if (l == nlev - 1) {
-		smooth_from_zero<TM, TB>(L, b, x, r, MG_COARSE_SWEEPS, s);
+	TM *r = (TM *) L.r;
+	if (l > 0 && L.M) {
+		unsigned nb = (unsigned) ((g.n + DENSE_BS / 64 - 1) / (DENSE_BS / 64));
+		if (sizeof(TM) == 8)
+			GGBK_LAUNCH((k_mg_dense<TM, double, TB>), nb, DENSE_BS, 0, (const double *) L.M, g.n, b, x, s);
+		else
+			GGBK_LAUNCH((k_mg_dense<TM, float, TB>), nb, DENSE_BS, 0, (const float *) L.M, g.n, b, x, s);
 		return;
 	}
-	// pre-smooth (one sweep from zero), residual, restrict
-	smooth_from_zero<TM, TB>(L, b, x, r, 1, s);
-	GGBK_LAUNCH((k_mg_residual<TM, TB>), NB(g.n), BS, 0, g, b, (const TM *) x, r, s);
+	if (l == nlev - 1) {
+		coarsest<TM, TB>(L, b, x, r, s);
+		if (l == 0 && red.part)
+			GGBK_LAUNCH((k_dot_partial<TM, TB>), NB(g.n), BS, 0, g.n, b, (const TM *) x, s, red);
+		return;
+	}
 	const ggbk_level &C = lv[l + 1];
+	GGBK_LAUNCH((k_mg_down<TM, TB>), NB(g.n), BS, 0, g, b, r, s);
 	GGBK_LAUNCH((k_mg_restrict<TM>), NB(L.xf.nc), BS, 0, L.xf, (const TM *) r, (TM *) C.b, s);
-	vcycle_level<TM, TM>(lv, nlev, l + 1, (const TM *) C.b, (TM *) C.x, (TM *) C.r, s);
-	// correct and post-smooth
-	GGBK_LAUNCH((k_mg_prolong<TM>), NB(L.xf.nf), BS, 0, L.xf, (const TM *) C.x, x, s);
-	GGBK_LAUNCH((k_mg_residual<TM, TB>), NB(g.n), BS, 0, g, b, (const TM *) x, r, s);
-	GGBK_LAUNCH((k_mg_jacobi<TM>), NB(g.n), BS, 0, g, x, (const TM *) r, s);
+	ggbk_red none = { nullptr, 0 };
+	vcycle_level<TM, TM>(lv, nlev, l + 1, (const TM *) C.b, (TM *) C.x, s, none);
+	GGBK_LAUNCH((k_mg_up<TM, TB>), NB(g.n), BS, 0, g, L.xf, b, (const TM *) C.x, x, s, l == 0 ? red : none);
 }
 
 extern "C" void ggbk_vcycle(const ggbk_level *levels, int nlev, const void *rin, int T, void *zout, int TM,
-			    const ggbk_scalars *s)
+			    const ggbk_scalars *s, ggbk_red red)
 {
-	void *r0 = levels[0].r;
 	if (TM == GGBK_F64)
-		vcycle_level<double, double>(levels, nlev, 0, (const double *) rin, (double *) zout, (double *) r0, s);
+		vcycle_level<double, double>(levels, nlev, 0, (const double *) rin, (double *) zout, s, red);
 	else if (T == GGBK_F64)
-		vcycle_level<float, double>(levels, nlev, 0, (const double *) rin, (float *) zout, (float *) r0, s);
+		vcycle_level<float, double>(levels, nlev, 0, (const double *) rin, (float *) zout, s, red);
 	else
-		vcycle_level<float, float>(levels, nlev, 0, (const float *) rin, (float *) zout, (float *) r0, s);
+		vcycle_level<float, float>(levels, nlev, 0, (const float *) rin, (float *) zout, s, red);
 }

@@ -4,16 +4,21 @@
 // bin-major / thread-minor and blends each bin in order, so every texel sees its
 // splats in (particle index, sample) order.  Here the unit of work is a face TILE of
 // TILE x TILE texels held in LDS:
-//   1. count  -- each particle walks its arc samples and adds itself once to every
-//                tile its footprints' bounding box (per face) overlaps;
+//   1. count  -- each particle adds itself once to every tile that the bounding box of
+//                its arc's footprints (per face) overlaps; for short arcs the box comes
+//                from the two endpoints (the arc is a straight segment on the face);
 //   2. scan   -- exclusive scan of the tile counts (one workgroup);
-//   3. fill   -- the same walk writes the particle index into each tile's list;
+//   3. fill   -- the same boxes write the particle index into each tile's list;
 //   4. draw   -- one workgroup per tile: sort the list (bitonic, LDS), load the tile,
-//                then blend particles in ascending index.  Particles whose footprint
-//                boxes overlap are serialised by an LDS ownership pass (atomicMin of
-//                the window rank per texel): a particle blends only when it is the
-//                lowest unprocessed claimant of every texel in its box, so each texel
-//                sees the reference order.  Each particle blends its samples in k order.
+//                then per window of WIN particles (ascending index): each particle
+//                computes its samples on the tile once (cached in LDS) and appends a
+//                2-byte fragment (window rank, sample) to every texel it touches; each
+//                texel then sorts its few fragments and blends them in (particle,
+//                sample) order -- the reference's per-texel order, with no
+//                cross-particle synchronisation.  A window in which a texel collects
+//                more than CAPT fragments is retried with a quarter of the particles;
+//                a single particle with more than SMAX samples on the tile is blended
+//                in order by one thread.
 // The texel arithmetic is the reference's (render.c:113-154) in float with no FMA
 // contraction, so texels match the CPU path bit for bit for identical positions.
 #include <hip/hip_runtime.h>
@@ -25,7 +30,7 @@
 #define PBS 256          // particle kernels
 #define DBS 256          // draw workgroup
 #define TILE_MAX 32
-#define SORT_CAP 2048    // LDS sort capacity per tile
+#define SORT_CAP 1024    // LDS sort capacity per tile
 #define SCAN_BS 1024
 
 struct SplatGeo {
@@ -78,11 +83,40 @@ __device__ __forceinline__ bool footprint(float px, float py, float rad, int R, 
 	return xlo <= xhi && ylo <= yhi;
 }
 
-// Per-face union of the footprint boxes of a particle's samples.
+// Per-face union of the footprint boxes of a particle's samples (conservative).
 struct FaceBoxes {
 	int xlo[6], xhi[6], ylo[6], yhi[6];
 };
 
+__device__ __forceinline__ void box_add(FaceBoxes &fb, int f, int xlo, int xhi, int ylo, int yhi)
+{
+	fb.xlo[f] = min(fb.xlo[f], xlo);
+	fb.xhi[f] = max(fb.xhi[f], xhi);
+	fb.ylo[f] = min(fb.ylo[f], ylo);
+	fb.yhi[f] = max(fb.yhi[f], yhi);
+}
+
+// exact per-sample boxes of face f
+__device__ __forceinline__ void face_box_samples(dv3 a, dv3 b, int ns, int f, const SplatGeo &sg, FaceBoxes &fb)
+{
+	for (int k = 1; k <= ns; k++) {
+		dv3 d = arc_point(a, b, k, ns);
+		if (!(face_dot(f, d) >= 0.5))
+			continue;
+		float px, py;
+		face_pixel(f, d, sg.R, px, py);
+		int xlo, xhi, ylo, yhi;
+		if (footprint(px, py, sg.rad, sg.R, xlo, xhi, ylo, yhi))
+			box_add(fb, f, xlo, xhi, ylo, yhi);
+	}
+}
+
+// Short arcs: every sample d(t) = normalize((1-t) a + t b) has face_dot between
+// min(da, db) and max(da, db) / cos(angle / 2), so with both endpoints at dc >= 0.45 the
+// arc projects onto face f as the straight segment between the endpoint pixels (great
+// circles are gnomonic lines), and with both below 0.45 no sample reaches dc = 0.5.
+// Otherwise the samples are walked.  The box is widened by 0.01 texel for rounding.
+#define SHORT_ARC 0.05
 __device__ __forceinline__ void arc_boxes(dv3 a, dv3 b, const SplatGeo &sg, FaceBoxes &fb)
 {
 	for (int f = 0; f < 6; f++) {
@@ -91,22 +125,32 @@ __device__ __forceinline__ void arc_boxes(dv3 a, dv3 b, const SplatGeo &sg, Face
 		fb.xhi[f] = INT_MIN;
 		fb.yhi[f] = INT_MIN;
 	}
-	int ns = arc_samples(a, b, sg.spacing);
-	for (int k = 1; k <= ns; k++) {
-		dv3 d = arc_point(a, b, k, ns);
-		for (int f = 0; f < 6; f++) {
-			if (!(face_dot(f, d) >= 0.5))
-				continue;
-			float px, py;
-			face_pixel(f, d, sg.R, px, py);
-			int xlo, xhi, ylo, yhi;
-			if (!footprint(px, py, sg.rad, sg.R, xlo, xhi, ylo, yhi))
-				continue;
-			fb.xlo[f] = min(fb.xlo[f], xlo);
-			fb.xhi[f] = max(fb.xhi[f], xhi);
-			fb.ylo[f] = min(fb.ylo[f], ylo);
-			fb.yhi[f] = max(fb.yhi[f], yhi);
+	double ang = vangle(a, b);
+	int ns = 0;
+	bool shortarc = ang < SHORT_ARC;
+	for (int f = 0; f < 6; f++) {
+		double da = face_dot(f, a), db = face_dot(f, b);
+		if (shortarc && da < 0.45 && db < 0.45)
+			continue;
+		if (shortarc && da >= 0.45 && db >= 0.45) {
+			double dc, u, v;
+			gnomonic(f, a, dc, u, v);
+			double pxa = (u + 1.0) * 0.5 * sg.R - 0.5, pya = (1.0 - v) * 0.5 * sg.R - 0.5;
+			gnomonic(f, b, dc, u, v);
+			double pxb = (u + 1.0) * 0.5 * sg.R - 0.5, pyb = (1.0 - v) * 0.5 * sg.R - 0.5;
+			double x0 = fmin(pxa, pxb) - 0.01 - sg.rad, x1 = fmax(pxa, pxb) + 0.01 + sg.rad;
+			double y0 = fmin(pya, pyb) - 0.01 - sg.rad, y1 = fmax(pya, pyb) + 0.01 + sg.rad;
+			int xlo = max((int) ceil(x0), 0), xhi = min((int) floor(x1), sg.R - 1);
+			int ylo = max((int) ceil(y0), 0), yhi = min((int) floor(y1), sg.R - 1);
+			if (xlo <= xhi && ylo <= yhi)
+				box_add(fb, f, xlo, xhi, ylo, yhi);
+			continue;
 		}
+		if (ns == 0) {
+			ns = (int) ceil(ang / sg.spacing);
+			ns = ns < 1 ? 1 : (ns > 4096 ? 4096 : ns);
+		}
+		face_box_samples(a, b, ns, f, sg, fb);
 	}
 }
 
@@ -186,6 +230,8 @@ __global__ __launch_bounds__(SCAN_BS) void k_render_scan(int ntiles, const int *
 		starts[ntiles] = (int) part[SCAN_BS - 1];
 		stats[0] = part[SCAN_BS - 1];
 		stats[1] = pmax[SCAN_BS - 1];
+		for (int k = 2; k < 8; k++)
+			stats[k] = 0;
 	}
 }
 
@@ -212,85 +258,74 @@ struct TileRect {
 	int f, x0, y0, w, h;
 };
 
-// Blend all samples of one particle that fall on this tile (render.c:113-154 applied
-// to the tile's texels).
-__device__ void blend_particle(float4 *tex, const TileRect &tr, dv3 a, dv3 b, const float *colp, const SplatGeo &sg,
-			       float alpha, float invg)
+// Footprint of one splat clipped to the tile (false if it misses the tile).
+__device__ __forceinline__ bool tile_clip(const TileRect &tr, float px, float py, float rad, int R, int &xlo, int &xhi,
+					  int &ylo, int &yhi)
 {
-	float rad = sg.rad, invrad = 1.0f / rad;
-	float c0 = colp[0], c1 = colp[1], c2 = colp[2], c3 = colp[3];
-	float w0 = alpha * c3;
-	int ns = arc_samples(a, b, sg.spacing);
-	for (int k = 1; k <= ns; k++) {
-		dv3 d = arc_point(a, b, k, ns);
-		float px, py;
-		if (!face_pixel(tr.f, d, sg.R, px, py))
+	if (!footprint(px, py, rad, R, xlo, xhi, ylo, yhi))
+		return false;
+	xlo = max(xlo, tr.x0);
+	xhi = min(xhi, tr.x0 + tr.w - 1);
+	ylo = max(ylo, tr.y0);
+	yhi = min(yhi, tr.y0 + tr.h - 1);
+	return xlo <= xhi && ylo <= yhi;
+}
+
+#define WIN 128        // particles per window (one thread each)
+#define SMAX 12        // samples per particle cached in LDS (k fits 4 bits)
+#define CAPT 12        // fragments one texel can hold per window
+#define NTEX (TILE_MAX * TILE_MAX)
+
+// Texels of one splat on the tile with positive tent weight (render.c:129-144 tests).
+// fn(local texel index) is called for each, in y-then-x order.
+template <typename Fn>
+__device__ __forceinline__ void splat_texels(const TileRect &tr, float px, float py, float rad, float invrad, int R,
+					     Fn fn)
+{
+	int xlo, xhi, ylo, yhi;
+	if (!tile_clip(tr, px, py, rad, R, xlo, xhi, ylo, yhi))
+		return;
+	for (int y = ylo; y <= yhi; y++) {
+		float wy = 1.0f - fabsf((float) y - py) * invrad;
+		if (wy <= 0)
 			continue;
-		int xlo, xhi, ylo, yhi;
-		if (!footprint(px, py, rad, sg.R, xlo, xhi, ylo, yhi))
-			continue;
-		xlo = max(xlo, tr.x0);
-		xhi = min(xhi, tr.x0 + tr.w - 1);
-		ylo = max(ylo, tr.y0);
-		yhi = min(yhi, tr.y0 + tr.h - 1);
-		for (int y = ylo; y <= yhi; y++) {
-			float wy = 1.0f - fabsf((float) y - py) * invrad;
-			if (wy <= 0)
+		for (int x = xlo; x <= xhi; x++) {
+			float wx = 1.0f - fabsf((float) x - px) * invrad;
+			if (wx <= 0)
 				continue;
-			for (int x = xlo; x <= xhi; x++) {
-				float wx = 1.0f - fabsf((float) x - px) * invrad;
-				if (wx <= 0)
-					continue;
-				float w = w0 * wx * wy;
-				float keep = 1.0f - w, add = w * invg;
-				float4 &dd = tex[(y - tr.y0) * TILE_MAX + (x - tr.x0)];
-				dd.x = dd.x * keep + add * c0;
-				dd.y = dd.y * keep + add * c1;
-				dd.z = dd.z * keep + add * c2;
-				dd.w = dd.w * keep + add;
-			}
+			fn((y - tr.y0) * TILE_MAX + (x - tr.x0));
 		}
 	}
 }
 
-// Box of the particle's footprints on this tile (false if none).
-__device__ bool tile_box(const TileRect &tr, dv3 a, dv3 b, const SplatGeo &sg, int &bx0, int &bx1, int &by0, int &by1)
+// One texel's blend of one splat (render.c:145-150).
+__device__ __forceinline__ void blend_texel(float4 &dd, int t, const TileRect &tr, float px, float py, float invrad,
+					    float w0, float4 c, float invg)
 {
-	bx0 = INT_MAX;
-	by0 = INT_MAX;
-	bx1 = INT_MIN;
-	by1 = INT_MIN;
-	int ns = arc_samples(a, b, sg.spacing);
-	for (int k = 1; k <= ns; k++) {
-		dv3 d = arc_point(a, b, k, ns);
-		float px, py;
-		if (!face_pixel(tr.f, d, sg.R, px, py))
-			continue;
-		int xlo, xhi, ylo, yhi;
-		if (!footprint(px, py, sg.rad, sg.R, xlo, xhi, ylo, yhi))
-			continue;
-		xlo = max(xlo, tr.x0);
-		xhi = min(xhi, tr.x0 + tr.w - 1);
-		ylo = max(ylo, tr.y0);
-		yhi = min(yhi, tr.y0 + tr.h - 1);
-		if (xlo > xhi || ylo > yhi)
-			continue;
-		bx0 = min(bx0, xlo);
-		bx1 = max(bx1, xhi);
-		by0 = min(by0, ylo);
-		by1 = max(by1, yhi);
-	}
-	return bx0 <= bx1;
+	int x = tr.x0 + t % TILE_MAX, y = tr.y0 + t / TILE_MAX;
+	float wy = 1.0f - fabsf((float) y - py) * invrad;
+	float wx = 1.0f - fabsf((float) x - px) * invrad;
+	float w = w0 * wx * wy;
+	float keep = 1.0f - w, add = w * invg;
+	dd.x = dd.x * keep + add * c.x;
+	dd.y = dd.y * keep + add * c.y;
+	dd.z = dd.z * keep + add * c.z;
+	dd.w = dd.w * keep + add;
 }
 
 __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, const double *prev, const double *pos,
 						     const float *col, SplatGeo sg, float invg, const long long *stats)
 {
-	__shared__ float4 tex[TILE_MAX * TILE_MAX];
-	__shared__ int owner[TILE_MAX * TILE_MAX];
+	__shared__ float4 tex[NTEX];
 	__shared__ int sbuf[SORT_CAP];
+	__shared__ float2 samp[SMAX][WIN];
+	__shared__ float4 pcol[WIN];
+	__shared__ int tcnt[NTEX];
+	__shared__ unsigned short frag[CAPT][NTEX];
+	__shared__ int serial;
 	if (stats[0] > r.cap)
-		return; // list overflowed: the host re-runs with a larger list
+		return; // list overflowed: the host grows it before this launch
+	long long c0 = clock64();
 	int t = blockIdx.x;
 	int L = r.tile_start[t + 1] - r.tile_start[t];
 	if (L == 0)
@@ -304,74 +339,155 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, const double
 	tr.w = min(T, r.R - tr.x0);
 	tr.h = min(T, r.R - tr.y0);
 	const int *lst = r.list + r.tile_start[t];
+	const float rad = sg.rad, invrad = 1.0f / rad;
 
 	// sorted particle list: in LDS when it fits, else in this tile's scratch region
 	int P = 1;
 	while (P < L)
 		P <<= 1;
-	int *sorted;
-	if (P <= SORT_CAP) {
-		sorted = sbuf;
-	} else {
-		sorted = r.sort_scratch + 2 * (size_t) r.tile_start[t];
-	}
+	int *sorted = P <= SORT_CAP ? sbuf : r.sort_scratch + 2 * (size_t) r.tile_start[t];
 	for (int i = threadIdx.x; i < P; i += DBS)
 		sorted[i] = i < L ? lst[i] : INT_MAX;
 	__syncthreads();
 	bitonic_sort(sorted, P);
 
-	// load the tile
+	long long c1 = clock64();
 	float4 *gbuf = (float4 *) r.buf;
-	for (int i = threadIdx.x; i < TILE_MAX * TILE_MAX; i += DBS) {
+	for (int i = threadIdx.x; i < NTEX; i += DBS) {
 		int ly = i / TILE_MAX, lx = i % TILE_MAX;
-		owner[i] = INT_MAX;
 		if (ly < tr.h && lx < tr.w)
 			tex[i] = gbuf[((size_t) tr.f * r.R + tr.y0 + ly) * r.R + tr.x0 + lx];
 	}
-	__syncthreads();
-
-	for (int base = 0; base < L; base += DBS) {
-		int slot = base + (int) threadIdx.x;
-		int pid = slot < L ? sorted[slot] : -1;
-		dv3 a = d3(0, 0, 0), b = d3(0, 0, 0);
-		int bx0 = 0, bx1 = -1, by0 = 0, by1 = -1;
-		bool pending = false;
-		if (pid >= 0) {
-			a = ld3(prev + 3 * (size_t) pid);
-			b = ld3(pos + 3 * (size_t) pid);
-			pending = tile_box(tr, a, b, sg, bx0, bx1, by0, by1);
-		}
-		int rank = threadIdx.x;
-		while (__syncthreads_or(pending)) {
-			bool claimed = pending;
-			if (claimed)
-				for (int y = by0; y <= by1; y++)
-					for (int x = bx0; x <= bx1; x++)
-						atomicMin(&owner[(y - tr.y0) * TILE_MAX + (x - tr.x0)], rank);
-			__syncthreads();
-			bool go = claimed;
-			if (claimed)
-				for (int y = by0; y <= by1 && go; y++)
-					for (int x = bx0; x <= bx1 && go; x++)
-						go = owner[(y - tr.y0) * TILE_MAX + (x - tr.x0)] == rank;
-			__syncthreads();
-			if (go) {
-				blend_particle(tex, tr, a, b, col + 4 * (size_t) pid, sg, r.alpha, invg);
-				pending = false;
+	long long csamp = 0, cblend = 0;
+	int windows = 0, serial_windows = 0;
+	// windows of W particles; a window whose fragments overflow is retried with W / 4
+	// particles (the tile is only written by a window that fits), down to one particle,
+	// which the serial path always handles
+	int W = WIN;
+	for (int base = 0; base < L;) {
+		long long ca = clock64();
+		windows++;
+		for (int i = threadIdx.x; i < NTEX; i += DBS)
+			tcnt[i] = 0;
+		if (threadIdx.x == 0)
+			serial = 0;
+		__syncthreads();
+		// samples of this window's particles on the tile (one thread per particle); every
+		// (particle rank, sample, texel) fragment goes to its texel's slot list
+		const int rank = threadIdx.x;
+		if (rank < W && base + rank < L) {
+			int pid = sorted[base + rank];
+			dv3 a = ld3(prev + 3 * (size_t) pid), b = ld3(pos + 3 * (size_t) pid);
+			int ns = arc_samples(a, b, sg.spacing);
+			int nsamp = 0;
+			bool over = false;
+			for (int k = 1; k <= ns; k++) {
+				float px, py;
+				if (!face_pixel(tr.f, arc_point(a, b, k, ns), sg.R, px, py))
+					continue;
+				int xlo, xhi, ylo, yhi;
+				if (!tile_clip(tr, px, py, rad, sg.R, xlo, xhi, ylo, yhi))
+					continue;
+				if (nsamp >= SMAX) {
+					over = true;
+					break;
+				}
+				samp[nsamp][rank] = make_float2(px, py);
+				unsigned short key = (unsigned short) ((rank << 4) | nsamp);
+				splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
+					int slot = atomicAdd(&tcnt[tx], 1);
+					if (slot < CAPT)
+						frag[slot][tx] = key;
+					else
+						over = true;
+				});
+				nsamp++;
 			}
-			__syncthreads();
-			if (claimed)
-				for (int y = by0; y <= by1; y++)
-					for (int x = bx0; x <= bx1; x++)
-						owner[(y - tr.y0) * TILE_MAX + (x - tr.x0)] = INT_MAX;
-			__syncthreads();
+			const float *cp = col + 4 * (size_t) pid;
+			pcol[rank] = make_float4(cp[0], cp[1], cp[2], cp[3]);
+			if (over)
+				serial = 1;
 		}
+		__syncthreads();
+		long long cb = clock64();
+		csamp += cb - ca;
+		if (!serial) {
+			// each texel applies its fragments in (particle rank, sample) order
+			for (int tx = threadIdx.x; tx < NTEX; tx += DBS) {
+				int n = tcnt[tx];
+				if (n == 0)
+					continue;
+				unsigned short keys[CAPT];
+#pragma unroll
+				for (int a = 0; a < CAPT; a++)
+					keys[a] = a < n ? frag[a][tx] : (unsigned short) 0xffff;
+#pragma unroll
+				for (int a = 1; a < CAPT; a++) {
+					unsigned short key = keys[a];
+#pragma unroll
+					for (int bpos = a; bpos > 0; bpos--) {
+						if (keys[bpos - 1] > key) {
+							keys[bpos] = keys[bpos - 1];
+							keys[bpos - 1] = key;
+						}
+					}
+				}
+				float4 dd = tex[tx];
+				for (int a = 0; a < n; a++) {
+					int pr = keys[a] >> 4, k = keys[a] & 15;
+					float2 q = samp[k][pr];
+					float4 c = pcol[pr];
+					blend_texel(dd, tx, tr, q.x, q.y, invrad, r.alpha * c.w, c, invg);
+				}
+				tex[tx] = dd;
+			}
+			base += W;
+			W = WIN;
+		} else if (W > 1) {
+			W = W >= 4 ? W / 4 : 1;
+		} else {
+			// one particle with more than SMAX samples on the tile, or more than CAPT
+			// samples on one texel: one thread blends it in order
+			serial_windows++;
+			if (threadIdx.x == 0) {
+				for (int rr = 0; rr < 1 && base + rr < L; rr++) {
+					int pid = sorted[base + rr];
+					dv3 a = ld3(prev + 3 * (size_t) pid), b = ld3(pos + 3 * (size_t) pid);
+					int ns = arc_samples(a, b, sg.spacing);
+					const float *cp = col + 4 * (size_t) pid;
+					float4 c = make_float4(cp[0], cp[1], cp[2], cp[3]);
+					for (int k = 1; k <= ns; k++) {
+						float px, py;
+						if (!face_pixel(tr.f, arc_point(a, b, k, ns), sg.R, px, py))
+							continue;
+						splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
+							blend_texel(tex[tx], tx, tr, px, py, invrad, r.alpha * c.w, c, invg);
+						});
+					}
+				}
+			}
+			base += 1;
+			W = WIN;
+		}
+		__syncthreads();
+		cblend += clock64() - cb;
 	}
 
-	for (int i = threadIdx.x; i < TILE_MAX * TILE_MAX; i += DBS) {
+	for (int i = threadIdx.x; i < NTEX; i += DBS) {
 		int ly = i / TILE_MAX, lx = i % TILE_MAX;
 		if (ly < tr.h && lx < tr.w)
 			gbuf[((size_t) tr.f * r.R + tr.y0 + ly) * r.R + tr.x0 + lx] = tex[i];
+	}
+	// per-launch diagnostics: [2] windows, [3] serial windows, [4] sort cycles,
+	// [5] sample cycles, [6] blend cycles, [7] total cycles (sums over tiles)
+	if (threadIdx.x == 0) {
+		long long *st = (long long *) stats;
+		atomicAdd((unsigned long long *) &st[2], (unsigned long long) windows);
+		atomicAdd((unsigned long long *) &st[3], (unsigned long long) serial_windows);
+		atomicAdd((unsigned long long *) &st[4], (unsigned long long) (c1 - c0));
+		atomicAdd((unsigned long long *) &st[5], (unsigned long long) csamp);
+		atomicAdd((unsigned long long *) &st[6], (unsigned long long) cblend);
+		atomicAdd((unsigned long long *) &st[7], (unsigned long long) (clock64() - c0));
 	}
 }
 
@@ -394,6 +510,11 @@ extern "C" void ggbk_render_scale(float *buf, size_t count, float g)
 // Host side of one deposit.  Returns the number of (particle, tile) entries, or -1.
 // The only synchronisation is the read-back of the entry total after the scan
 // (the list is grown when the total exceeds its capacity).
+extern "C" int ggbk_render_stats(const ggbk_render *r, long long *out8)
+{
+	return ggbk_d2h(out8, r->total, 8 * sizeof(long long));
+}
+
 extern "C" long long ggbk_render_deposit(ggbk_render *r, int np, const double *prev, const double *pos,
 					 const float *col, float invg)
 {

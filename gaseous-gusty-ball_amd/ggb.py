@@ -86,7 +86,7 @@ PUBLIC_SYMBOLS = [
     "fluid_get_f64", "fluid_set_f64", "fluid_solve_psi_device", "fluid_true_relres",
     "fluid_vcycle_f64", "ggb_laplacian_f64", "ggb_applyK_f64", "ggb_curl_f64",
     "ggb_divergence_f64", "particles_sync_host", "particles_get_f64", "particles_set_f64",
-    "particles_set_round32", "render_sync_host", "render_last_entries", "render_faces8",
+    "particles_set_round32", "render_sync_host", "render_last_entries", "render_last_stats", "render_faces8",
     # ggb_inputs.h
     "ggb_initial_velocity", "ggb_banded_strip", "ggb_rgba8_to_float",
 ]
@@ -147,6 +147,7 @@ def lib():
             "render_sync_host": (None, [P(CRender)]),
             "render_last_entries": (C.c_longlong, [P(CRender)]),
             "render_faces8": (None, [P(CRender), fp, P(C.c_ubyte)]),
+            "render_last_stats": (C.c_int, [P(CRender), P(C.c_longlong)]),
             "ggb_device_init": (C.c_int, []),
             "ggb_version": (C.c_char_p, []),
             "ggb_sync": (None, []),
@@ -433,6 +434,11 @@ class Render:
 
     def last_entries(self):
         return lib().render_last_entries(C.byref(self.c))
+
+    def last_stats(self):
+        out = (C.c_longlong * 8)()
+        lib().render_last_stats(C.byref(self.c), out)
+        return list(out)
 
     def faces8(self, bg=None):
         ch = 4 if bg is None else 3
