@@ -31,6 +31,10 @@ class CCubeSphere(C.Structure):
     ]
 
 
+class CVec3(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
+
+
 class CFluidParams(C.Structure):
     _fields_ = [
         ("forcing", C.c_double), ("rotation", C.c_double), ("bfecc", C.c_int),
@@ -107,7 +111,8 @@ def lib():
         sig = {
             "cs_init": (C.c_int, [P(CCubeSphere), C.c_int]),
             "cs_free": (None, [P(CCubeSphere)]),
-            "cs_cell_of": (C.c_int, [P(CCubeSphere), C.c_double * 3]),
+            "cs_cell_of": (C.c_int, [P(CCubeSphere), CVec3]),
+            "cs_taps": (None, [P(CCubeSphere), CVec3, P(C.c_int), P(C.c_double)]),
             "fluid_init": (C.c_int, [P(CFluid), P(CCubeSphere), P(CFluidParams)]),
             "fluid_free": (None, [P(CFluid)]),
             "fluid_curl": (None, [P(CCubeSphere), fp, fp]),
@@ -217,7 +222,13 @@ class Grid:
             pass
 
     def cell_of(self, p):
-        return lib().cs_cell_of(C.byref(self.c), (C.c_double * 3)(*map(float, p)))
+        return lib().cs_cell_of(C.byref(self.c), CVec3(*map(float, p)))
+
+    def taps(self, p):
+        idx = (C.c_int * 4)()
+        wt = (C.c_double * 4)()
+        lib().cs_taps(C.byref(self.c), CVec3(*map(float, p)), idx, wt)
+        return np.array(idx[:], np.int32), np.array(wt[:])
 
     def initial_velocity(self, seed=1, bands=8, band_amp=1.0, noise_amp=0.5, noise_scale=3.0,
                          lacunarity=2.0, gain=0.5, octaves=4):
