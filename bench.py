@@ -12,6 +12,7 @@ both, the roofline of the dominant kernel (HIP events on the library stream over
 timed steps), and the CPU baseline timed on this host.
 """
 import argparse
+import ctypes as C
 import json
 import os
 import subprocess
@@ -35,6 +36,36 @@ CONFIGS = {
                precision=ggb.PREC_FP64, octaves=4),
 }
 HBM_PEAK = 8.0e12
+
+# Probed kernels (ids of the library's launch probes) and their algorithmic bytes per
+# launch under the per-pass model: every pass reads its inputs and writes its outputs
+# once, gathered neighbours are cache hits, stencil and transfer coefficients are the
+# stored float tables.  kbar = mean Voronoi degree, n = cells, P = particles,
+# E = (particle, tile) entries, T = texels touched, Ts/Tm = state / multigrid word.
+PROBES = {
+    "k_render_draw": 1, "render_binning": 2, "k_mg_up(level 0)": 3, "k_mg_down(level 0)": 4,
+    "k_cg_Ap": 5, "k_cg_update": 6, "k_particles_advect": 7, "k_departures": 8,
+}
+
+
+def kernel_bytes(name, n, kbar, P, E, T, Ts, Tm, Tz):
+    if name == "k_render_draw":      # texels read+written, list entry, FP64 endpoints, colour
+        return 32 * T + E * (4 + 48 + 16)
+    if name == "render_binning":     # two passes over the FP64 endpoints, list entries, counters
+        return P * 2 * 48 + E * 4
+    if name == "k_mg_up(level 0)":   # b, dinv, nnb, stencil, taps+weights, coarse x, z out
+        return n * (Ts + 4 + 1 + 8 * kbar + 32 + Tm / 4 + Tm)
+    if name == "k_mg_down(level 0)":  # b, dinv, nnb, stencil, r1 out
+        return n * (Ts + 4 + 1 + 8 * kbar + Tm)
+    if name == "k_cg_Ap":            # z, p in, nnb, stencil, p out, Ap out
+        return n * (Tz + Ts + 1 + 8 * kbar + 2 * Ts)
+    if name == "k_cg_update":        # psi, r, p, Ap in; psi, r out
+        return n * 6 * Ts
+    if name == "k_particles_advect":  # pos in, pos+prev out (FP64), velocity field once
+        return P * 72 + n * 3 * Ts
+    if name == "k_departures":       # vel, omega, centre in; q0, dep_b, dep_f out
+        return n * (3 * Ts + Ts + 12 + Ts + 6 * Ts)
+    raise KeyError(name)
 
 
 def log(*a):
@@ -132,8 +163,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="first run a plain copy kernel of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -142,6 +175,15 @@ def main():
     cfg = CONFIGS[args.config]
     if ggb.device_init() != 0:
         sys.exit("bench.py: no GPU")
+    if args.calib_copy:
+        # a plain copy of known size, so counter runs can scale FETCH_SIZE / WRITE_SIZE
+        nb = args.calib_copy << 20
+        a, b = ggb.lib().ggbk_malloc(nb), ggb.lib().ggbk_malloc(nb)
+        ggb.lib().ggbk_copy_kernel(b, a, nb)
+        ggb.lib().ggb_sync()
+        ggb.lib().ggbk_free(a)
+        ggb.lib().ggbk_free(b)
+        log("calibration copy: %d MiB read, %d MiB written" % (args.calib_copy, args.calib_copy))
 
     t0 = time.perf_counter()
     g, F, P, R = setup(cfg, cfg["precision"])
@@ -149,13 +191,29 @@ def main():
         (time.perf_counter() - t0, cfg["N"], g.n, cfg["P"], cfg["R"], cfg["precision"], F.cg_iters))
     timers = {k: ggb.Timer() for k in "pfr"}
     acc = dict(p=0.0, f=0.0, r=0.0, its=0, entries=0)
+    L = ggb.lib()
+    # warmup with every probe on: per-kernel device time per step, to pick the dominant
+    for pid in PROBES.values():
+        L.ggbk_probe_enable(pid, 1)
     run_steps(g, F, P, R, args.warmup, timers, dict(acc))
-    ggb.lib().ggb_sync()
-    log("warmup done (%d steps)" % args.warmup)
+    warm = {}
+    for name, pid in PROBES.items():
+        ms = C.c_double()
+        cnt = L.ggbk_probe_collect(pid, C.byref(ms))
+        warm[name] = (cnt, ms.value)
+        L.ggbk_probe_enable(pid, 0)
+    dominant = max(warm, key=lambda k: warm[k][1])
+    log("warmup done (%d steps); dominant kernel %s" % (args.warmup, dominant))
+    # timed region: only the dominant kernel is bracketed by events
+    L.ggbk_probe_enable(PROBES[dominant], 1)
     t0 = time.perf_counter()
     run_steps(g, F, P, R, args.steps, timers, acc)
     ggb.lib().ggb_sync()
     wall = time.perf_counter() - t0
+    dms = C.c_double()
+    dcount = L.ggbk_probe_collect(PROBES[dominant], C.byref(dms))
+    L.ggbk_probe_enable(PROBES[dominant], 0)
+    rstats = R.last_stats()
     K = args.steps
     ms_fluid = acc["f"] / K
     ms_render = acc["r"] / K
@@ -166,6 +224,32 @@ def main():
     log("timed %d steps in %.2fs: fluid %.3f ms (%.1f its/step), particles %.3f ms, render %.3f ms"
         % (K, wall, ms_fluid, acc["its"] / K, ms_part, ms_render))
 
+    # roofline of the dominant kernel
+    nn = g.nnb.astype(np.int64)
+    kbar = float(nn.mean())
+    Ts = 4 if cfg["precision"] == ggb.PREC_REF32 else 8
+    Tm = 8 if cfg["precision"] == ggb.PREC_FP64 else 4
+    E = rstats[0]
+    T = rstats[8]
+    nbytes = kernel_bytes(dominant, g.n, kbar, cfg["P"], E, T, Ts, Tm, Tm)
+    avg_s = dms.value / 1000.0 / max(dcount, 1)
+    achieved = nbytes / avg_s / 1e9
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated),
+    # recorded in profiles/traffic.json by the counter runs of this workload
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        tr = json.load(open(tpath)).get(args.config, {}).get(dominant)
+        if tr:
+            traffic = tr.get("bytes_per_launch")
+    kernels = {}
+    for name, (cnt, ms) in warm.items():
+        if cnt:
+            b = kernel_bytes(name, g.n, kbar, cfg["P"], E, T, Ts, Tm, Tm)
+            kernels[name] = {"launches_per_step": round(cnt / max(args.warmup, 1), 2),
+                             "avg_us": round(ms * 1000.0 / cnt, 2),
+                             "algo_bytes": int(b),
+                             "achieved_GBps": round(b / (ms / 1000.0 / cnt) / 1e9, 1)}
     out = {
         "metric": "fluid steps/s @N=512 + Msplat-particles/s; mixed vs FP64 at 1e-8 residual",
         "value": round(fluid_sps, 3),
@@ -192,6 +276,12 @@ def main():
         "cg_iters_per_step": round(acc["its"] / K, 2),
         "true_relres": relres,
         "splat_entries_per_step": acc["entries"] // K,
+        "texels_touched_last_step": int(T),
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4),
+                     "traffic": traffic, "launches": dcount, "avg_us": round(avg_s * 1e6, 2),
+                     "algo_bytes_per_launch": int(nbytes)},
+        "kernels_warmup": kernels,
     }
     F.free()
     P.free()

@@ -69,7 +69,7 @@ int render_init(Render *r, int R, float radius, float alpha)
 	k->tile_count = ggbk_malloc(sizeof(int) * ((size_t) k->ntiles + 1));
 	k->tile_start = ggbk_malloc(sizeof(int) * ((size_t) k->ntiles + 1));
 	k->tile_cursor = ggbk_malloc(sizeof(int) * ((size_t) k->ntiles + 1));
-	k->total = ggbk_malloc(sizeof(long long) * 8);
+	k->total = ggbk_malloc(sizeof(long long) * 16);
 	k->cap = 0;
 	k->list = NULL;
 	k->sort_scratch = NULL;
@@ -132,7 +132,7 @@ void render_deposit(Render *r, const Particles *P)
 
 long long render_last_entries(const Render *r) { return r->dev->last_entries; }
 
-int render_last_stats(const Render *r, long long *out8) { return ggbk_render_stats(&r->dev->k, out8); }
+int render_last_stats(const Render *r, long long *out16) { return ggbk_render_stats(&r->dev->k, out16); }
 
 static void refresh(const Render *r)
 {

@@ -557,6 +557,7 @@ extern "C" void ggbk_cg_start(ggbk_grid g, void *omega, const void *psi, void *r
 extern "C" void ggbk_cg_Ap(ggbk_grid g, const void *z, int TZ, const void *p_in, void *p_out, void *Ap, int T,
 			   int first, const ggbk_scalars *s, ggbk_red red)
 {
+	ggbk_probe_begin(GGBK_PROBE_CG_AP);
 	if (T == GGBK_F64 && TZ == GGBK_F64)
 		GGBK_LAUNCH((k_cg_Ap<double, double>), NB(g.n), BS, 0, g, (const double *) z, (const double *) p_in,
 			    (double *) p_out, (double *) Ap, first, s, red);
@@ -566,17 +567,20 @@ extern "C" void ggbk_cg_Ap(ggbk_grid g, const void *z, int TZ, const void *p_in,
 	else
 		GGBK_LAUNCH((k_cg_Ap<float, float>), NB(g.n), BS, 0, g, (const float *) z, (const float *) p_in,
 			    (float *) p_out, (float *) Ap, first, s, red);
+	ggbk_probe_end(GGBK_PROBE_CG_AP);
 }
 
 extern "C" void ggbk_cg_update(ggbk_grid g, void *psi, void *r, const void *p, const void *Ap, int T,
 			       const ggbk_scalars *s, ggbk_red red)
 {
+	ggbk_probe_begin(GGBK_PROBE_CG_UPDATE);
 	if (T == GGBK_F64)
 		GGBK_LAUNCH(k_cg_update<double>, NB(g.n), BS, 0, g.n, (double *) psi, (double *) r, (const double *) p,
 			    (const double *) Ap, s, red);
 	else
 		GGBK_LAUNCH(k_cg_update<float>, NB(g.n), BS, 0, g.n, (float *) psi, (float *) r, (const float *) p,
 			    (const float *) Ap, s, red);
+	ggbk_probe_end(GGBK_PROBE_CG_UPDATE);
 }
 
 extern "C" void ggbk_jacobi_precond(ggbk_grid g, const void *r, void *z, int T, const ggbk_scalars *s, ggbk_red red)
@@ -617,12 +621,14 @@ extern "C" void ggbk_max_speed2(ggbk_grid g, const void *vel, int T, ggbk_red re
 extern "C" void ggbk_departures(ggbk_grid g, const void *vel, const void *omega, void *q0, void *dep_b, void *dep_f,
 				int T, double dt, double twoOmega, int fwd)
 {
+	ggbk_probe_begin(GGBK_PROBE_DEPARTURES);
 	if (T == GGBK_F64)
 		GGBK_LAUNCH(k_departures<double>, NB(g.n), BS, 0, g, (const double *) vel, (const double *) omega,
 			    (double *) q0, (double *) dep_b, (double *) dep_f, dt, twoOmega, fwd);
 	else
 		GGBK_LAUNCH(k_departures<float>, NB(g.n), BS, 0, g, (const float *) vel, (const float *) omega,
 			    (float *) q0, (float *) dep_b, (float *) dep_f, dt, twoOmega, fwd);
+	ggbk_probe_end(GGBK_PROBE_DEPARTURES);
 }
 
 extern "C" void ggbk_advect(ggbk_grid g, const void *src, const void *dep, void *dst, int T)

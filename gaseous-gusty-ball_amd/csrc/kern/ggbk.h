@@ -83,6 +83,25 @@ float ggbk_event_ms(void *a, void *b);     /* synchronises on b */
 void ggbk_event_free(void *ev);
 void *ggbk_stream_handle(void);
 
+/* ---- launch probes (HIP events around selected kernels, for measurement) ---- */
+enum {
+	GGBK_PROBE_RENDER_DRAW = 1,
+	GGBK_PROBE_RENDER_BIN = 2,   /* count + scan + fill */
+	GGBK_PROBE_MG_UP0 = 3,       /* level-0 prolongation + post-smooth */
+	GGBK_PROBE_MG_DOWN0 = 4,     /* level-0 pre-smooth residual */
+	GGBK_PROBE_CG_AP = 5,
+	GGBK_PROBE_CG_UPDATE = 6,
+	GGBK_PROBE_PARTICLES = 7,
+	GGBK_PROBE_DEPARTURES = 8,
+};
+void ggbk_probe_enable(int id, int on);
+int ggbk_probe_collect(int id, double *total_ms);
+void ggbk_copy_kernel(void *dst, const void *src, size_t bytes);
+#ifdef __cplusplus
+int ggbk_probe_begin(int id);
+void ggbk_probe_end(int id);
+#endif
+
 /* ---- conversions ---- */
 void ggbk_convert(void *dst, int tdst, const void *src, int tsrc, size_t count);
 
@@ -188,11 +207,11 @@ typedef struct {
 	long long cap;
 	int *sort_scratch;          /* [scratch_cap] for tiles whose list exceeds the LDS sort */
 	long long scratch_cap;
-	long long *total;           /* [8] entries, longest list, draw diagnostics (device) */
+	long long *total;           /* [16] entries, longest list, draw diagnostics (device) */
 } ggbk_render;
 void ggbk_render_scale(float *buf, size_t count, float g);
-/* copies the 8 diagnostics of the last deposit (synchronises) */
-int ggbk_render_stats(const ggbk_render *r, long long *out8);
+/* copies the 16 diagnostics of the last deposit (synchronises) */
+int ggbk_render_stats(const ggbk_render *r, long long *out16);
 /* bin + ordered blend; returns the number of (particle, tile) entries, or -1 on error */
 long long ggbk_render_deposit(ggbk_render *r, int np, const double *prev, const double *pos,
 			      const float *col, float invg);

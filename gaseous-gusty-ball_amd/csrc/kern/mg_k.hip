@@ -258,11 +258,19 @@ static void vcycle_level(const ggbk_level *lv, int nlev, int l, const TB *b, TM 
 		return;
 	}
 	const ggbk_level &C = lv[l + 1];
+	if (l == 0)
+		ggbk_probe_begin(GGBK_PROBE_MG_DOWN0);
 	GGBK_LAUNCH((k_mg_down<TM, TB>), NB(g.n), BS, 0, g, b, r, s);
+	if (l == 0)
+		ggbk_probe_end(GGBK_PROBE_MG_DOWN0);
 	GGBK_LAUNCH((k_mg_restrict<TM>), NB(L.xf.nc), BS, 0, L.xf, (const TM *) r, (TM *) C.b, s);
 	ggbk_red none = { nullptr, 0 };
 	vcycle_level<TM, TM>(lv, nlev, l + 1, (const TM *) C.b, (TM *) C.x, s, none);
+	if (l == 0)
+		ggbk_probe_begin(GGBK_PROBE_MG_UP0);
 	GGBK_LAUNCH((k_mg_up<TM, TB>), NB(g.n), BS, 0, g, L.xf, b, (const TM *) C.x, x, s, l == 0 ? red : none);
+	if (l == 0)
+		ggbk_probe_end(GGBK_PROBE_MG_UP0);
 }
 
 extern "C" void ggbk_vcycle(const ggbk_level *levels, int nlev, const void *rin, int T, void *zout, int TM,

@@ -88,10 +88,12 @@ extern "C" void ggbk_particles_color(int n, const double *pos, float *col, const
 extern "C" void ggbk_particles_advect(ggbk_grid g, int np, double *pos, double *prev, const void *vel, int T,
 				      double dt, int round32)
 {
+	ggbk_probe_begin(GGBK_PROBE_PARTICLES);
 	if (T == GGBK_F64)
 		GGBK_LAUNCH(k_particles_advect<double>, ggbk_blocks(np, BS), BS, 0, g, np, pos, prev,
 			    (const double *) vel, dt, round32);
 	else
 		GGBK_LAUNCH(k_particles_advect<float>, ggbk_blocks(np, BS), BS, 0, g, np, pos, prev,
 			    (const float *) vel, dt, round32);
+	ggbk_probe_end(GGBK_PROBE_PARTICLES);
 }

@@ -230,7 +230,7 @@ __global__ __launch_bounds__(SCAN_BS) void k_render_scan(int ntiles, const int *
 		starts[ntiles] = (int) part[SCAN_BS - 1];
 		stats[0] = part[SCAN_BS - 1];
 		stats[1] = pmax[SCAN_BS - 1];
-		for (int k = 2; k < 8; k++)
+		for (int k = 2; k < 16; k++)
 			stats[k] = 0;
 	}
 }
@@ -323,6 +323,7 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, const double
 	__shared__ int tcnt[NTEX];
 	__shared__ unsigned short frag[CAPT][NTEX];
 	__shared__ int serial;
+	__shared__ unsigned tmask[NTEX / 32];
 	if (stats[0] > r.cap)
 		return; // list overflowed: the host grows it before this launch
 	long long c0 = clock64();
@@ -358,6 +359,8 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, const double
 		if (ly < tr.h && lx < tr.w)
 			tex[i] = gbuf[((size_t) tr.f * r.R + tr.y0 + ly) * r.R + tr.x0 + lx];
 	}
+	for (int i = threadIdx.x; i < NTEX / 32; i += DBS)
+		tmask[i] = 0;
 	long long csamp = 0, cblend = 0;
 	int windows = 0, serial_windows = 0;
 	// windows of W particles; a window whose fragments overflow is retried with W / 4
@@ -395,6 +398,7 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, const double
 				samp[nsamp][rank] = make_float2(px, py);
 				unsigned short key = (unsigned short) ((rank << 4) | nsamp);
 				splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
+					atomicOr(&tmask[tx >> 5], 1u << (tx & 31));
 					int slot = atomicAdd(&tcnt[tx], 1);
 					if (slot < CAPT)
 						frag[slot][tx] = key;
@@ -461,6 +465,7 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, const double
 						if (!face_pixel(tr.f, arc_point(a, b, k, ns), sg.R, px, py))
 							continue;
 						splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
+							tmask[tx >> 5] |= 1u << (tx & 31);
 							blend_texel(tex[tx], tx, tr, px, py, invrad, r.alpha * c.w, c, invg);
 						});
 					}
@@ -473,13 +478,20 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, const double
 		cblend += clock64() - cb;
 	}
 
+	// write back the texels a splat touched
 	for (int i = threadIdx.x; i < NTEX; i += DBS) {
 		int ly = i / TILE_MAX, lx = i % TILE_MAX;
-		if (ly < tr.h && lx < tr.w)
+		if ((tmask[i >> 5] >> (i & 31)) & 1u)
 			gbuf[((size_t) tr.f * r.R + tr.y0 + ly) * r.R + tr.x0 + lx] = tex[i];
 	}
 	// per-launch diagnostics: [2] windows, [3] serial windows, [4] sort cycles,
-	// [5] sample cycles, [6] blend cycles, [7] total cycles (sums over tiles)
+	// [5] sample cycles, [6] blend cycles, [7] total cycles (sums over tiles),
+	// [8] touched texels
+	if (threadIdx.x < NTEX / 32) {
+		int c = __popc(tmask[threadIdx.x]);
+		if (c)
+			atomicAdd((unsigned long long *) &((long long *) stats)[8], (unsigned long long) c);
+	}
 	if (threadIdx.x == 0) {
 		long long *st = (long long *) stats;
 		atomicAdd((unsigned long long *) &st[2], (unsigned long long) windows);
@@ -510,9 +522,9 @@ extern "C" void ggbk_render_scale(float *buf, size_t count, float g)
 // Host side of one deposit.  Returns the number of (particle, tile) entries, or -1.
 // The only synchronisation is the read-back of the entry total after the scan
 // (the list is grown when the total exceeds its capacity).
-extern "C" int ggbk_render_stats(const ggbk_render *r, long long *out8)
+extern "C" int ggbk_render_stats(const ggbk_render *r, long long *out16)
 {
-	return ggbk_d2h(out8, r->total, 8 * sizeof(long long));
+	return ggbk_d2h(out16, r->total, 16 * sizeof(long long));
 }
 
 extern "C" long long ggbk_render_deposit(ggbk_render *r, int np, const double *prev, const double *pos,
@@ -529,9 +541,11 @@ extern "C" long long ggbk_render_deposit(ggbk_render *r, int np, const double *p
 	unsigned pb = ggbk_blocks(np, PBS);
 	if (ggbk_zero(r->tile_count, sizeof(int) * (size_t) r->ntiles))
 		return -1;
+	int pr_bin = ggbk_probe_begin(GGBK_PROBE_RENDER_BIN);
 	GGBK_LAUNCH(k_render_count, pb, PBS, 0, np, prev, pos, sg, r->tile, r->ntx, r->tile_count);
 	GGBK_LAUNCH(k_render_scan, 1, SCAN_BS, 0, r->ntiles, (const int *) r->tile_count, r->tile_start,
 		    r->tile_cursor, r->total);
+	ggbk_probe_end(GGBK_PROBE_RENDER_BIN);
 	long long stats[2] = { 0, 0 };
 	if (ggbk_d2h(stats, r->total, sizeof(stats)))
 		return -1;
@@ -548,10 +562,15 @@ extern "C" long long ggbk_render_deposit(ggbk_render *r, int np, const double *p
 		r->cap = ncap;
 		r->scratch_cap = 2 * ncap;
 	}
+	if (pr_bin)
+		ggbk_probe_begin(GGBK_PROBE_RENDER_BIN);
 	GGBK_LAUNCH(k_render_fill, pb, PBS, 0, np, prev, pos, sg, r->tile, r->ntx, (const int *) r->tile_start,
 		    r->tile_cursor, r->list, r->cap);
+	ggbk_probe_end(GGBK_PROBE_RENDER_BIN);
+	ggbk_probe_begin(GGBK_PROBE_RENDER_DRAW);
 	GGBK_LAUNCH(k_render_draw, (unsigned) r->ntiles, DBS, 0, *r, prev, pos, col, sg, invg,
 		    (const long long *) r->total);
+	ggbk_probe_end(GGBK_PROBE_RENDER_DRAW);
 	if (ggbk_check("render_deposit"))
 		return -1;
 	return stats[0];

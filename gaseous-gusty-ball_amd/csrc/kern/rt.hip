@@ -179,6 +179,73 @@ extern "C" void ggbk_event_free(void *ev)
 		(void) hipEventDestroy((hipEvent_t) ev);
 }
 
+// ---------------- launch probes ----------------
+// Selected launchers bracket their kernel with HIP events on the library stream when
+// the probe of their id is enabled; the measurement tools sum the intervals.
+#define PROBE_IDS 16
+#define PROBE_CAP 8192
+static unsigned g_probe_mask = 0;
+static hipEvent_t g_probe_ev[PROBE_IDS][PROBE_CAP][2];
+static int g_probe_n[PROBE_IDS];
+static int g_probe_open[PROBE_IDS];
+
+extern "C" void ggbk_probe_enable(int id, int on)
+{
+	if (id < 0 || id >= PROBE_IDS)
+		return;
+	if (on)
+		g_probe_mask |= 1u << id;
+	else
+		g_probe_mask &= ~(1u << id);
+}
+
+int ggbk_probe_begin(int id)
+{
+	if (!(g_probe_mask & (1u << id)) || g_probe_n[id] >= PROBE_CAP)
+		return 0;
+	hipEvent_t *e = g_probe_ev[id][g_probe_n[id]];
+	if (!e[0]) {
+		(void) hipEventCreate(&e[0]);
+		(void) hipEventCreate(&e[1]);
+	}
+	(void) hipEventRecord(e[0], g_stream);
+	g_probe_open[id] = 1;
+	return 1;
+}
+
+void ggbk_probe_end(int id)
+{
+	if (!g_probe_open[id])
+		return;
+	(void) hipEventRecord(g_probe_ev[id][g_probe_n[id]][1], g_stream);
+	g_probe_n[id]++;
+	g_probe_open[id] = 0;
+}
+
+// Number of probed launches and their summed duration (ms) since the last collect;
+// synchronises and resets the probe.
+extern "C" int ggbk_probe_collect(int id, double *total_ms)
+{
+	*total_ms = 0;
+	if (id < 0 || id >= PROBE_IDS)
+		return 0;
+	(void) hipStreamSynchronize(g_stream);
+	int n = g_probe_n[id];
+	for (int k = 0; k < n; k++) {
+		float ms = 0;
+		(void) hipEventElapsedTime(&ms, g_probe_ev[id][k][0], g_probe_ev[id][k][1]);
+		*total_ms += ms;
+	}
+	g_probe_n[id] = 0;
+	return n;
+}
+
+// A plain copy of `bytes` bytes (float words) for counter calibration.
+extern "C" void ggbk_copy_kernel(void *dst, const void *src, size_t bytes)
+{
+	ggbk_convert(dst, GGBK_F32, src, GGBK_F32, bytes / 4);
+}
+
 template <typename D, typename S>
 __global__ void k_convert(D *dst, const S *src, size_t count)
 {

@@ -162,6 +162,11 @@ def lib():
             "ggb_banded_strip": (C.c_int, [C.c_uint, C.c_int, C.c_int, C.c_int, C.c_int,
                                           C.c_int, P(C.c_ubyte)]),
             "ggb_rgba8_to_float": (None, [P(C.c_ubyte), C.c_size_t, fp]),
+            "ggbk_probe_enable": (None, [C.c_int, C.c_int]),
+            "ggbk_probe_collect": (C.c_int, [C.c_int, P(C.c_double)]),
+            "ggbk_copy_kernel": (None, [C.c_void_p, C.c_void_p, C.c_size_t]),
+            "ggbk_malloc": (C.c_void_p, [C.c_size_t]),
+            "ggbk_free": (None, [C.c_void_p]),
             "ggbk_event_new": (C.c_void_p, []),
             "ggbk_event_record": (None, [C.c_void_p]),
             "ggbk_event_ms": (C.c_float, [C.c_void_p, C.c_void_p]),
@@ -447,7 +452,7 @@ class Render:
         return lib().render_last_entries(C.byref(self.c))
 
     def last_stats(self):
-        out = (C.c_longlong * 8)()
+        out = (C.c_longlong * 16)()
         lib().render_last_stats(C.byref(self.c), out)
         return list(out)
 

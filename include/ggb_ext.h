@@ -62,9 +62,10 @@ void particles_set_round32(Particles *P, int on);
 /* ---- render ---- */
 void render_sync_host(Render *r);             /* device buffer -> r->buf mirror */
 long long render_last_entries(const Render *r); /* (particle, tile) entries of the last deposit */
-/* diagnostics of the last deposit: entries, longest tile list, ownership rounds (sum,
- * max), and per-phase cycle sums of the draw pass (sort, samples, rounds, total) */
-int render_last_stats(const Render *r, long long *out8);
+/* diagnostics of the last deposit (16 values): (particle, tile) entries, longest tile
+ * list, windows, serial windows, draw-pass cycle sums (sort, samples, blend, total),
+ * texels touched */
+int render_last_stats(const Render *r, long long *out16);
 /* 8-bit faces [6][R][R][4] (bg NULL: un-premultiplied RGBA) computed on the host mirror. */
 void render_faces8(Render *r, const float *bg, unsigned char *out);
 
