@@ -43,24 +43,24 @@ HBM_PEAK = 8.0e12
 # stored float tables.  kbar = mean Voronoi degree, n = cells, P = particles,
 # E = (particle, tile) entries, T = texels touched, Ts/Tm = state / multigrid word.
 PROBES = {
-    "k_render_draw": 1, "render_binning": 2, "k_mg_up(level 0)": 3, "k_mg_down(level 0)": 4,
-    "k_cg_Ap": 5, "k_cg_update": 6, "k_particles_advect": 7, "k_departures": 8,
+    "k_render_draw": 1, "render_binning": 2, "mg_up(level 0)": 3, "k_mg_down(level 0)": 4,
+    "k_cg_Ap": 5, "k_cg_update_down": 6, "k_particles_advect": 7, "k_departures": 8,
 }
 
 
 def kernel_bytes(name, n, kbar, P, E, T, Ts, Tm, Tz):
-    if name == "k_render_draw":      # texels read+written, list entry, FP64 endpoints, colour
-        return 32 * T + E * (4 + 48 + 16)
-    if name == "render_binning":     # two passes over the FP64 endpoints, list entries, counters
-        return P * 2 * 48 + E * 4
-    if name == "k_mg_up(level 0)":   # b, dinv, nnb, stencil, taps+weights, coarse x, z out
+    if name == "k_render_draw":      # texels read+written, list entry, FP64 endpoints, colour, index
+        return 32 * T + E * (4 + 48 + 16 + 4)
+    if name == "render_binning":     # count: endpoints; fill: particle read + reordered copy, list
+        return P * (48 + 68 + 68) + E * 4
+    if name == "mg_up(level 0)":     # b, dinv, nnb, stencil, taps+weights, coarse x, z out
         return n * (Ts + 4 + 1 + 8 * kbar + 32 + Tm / 4 + Tm)
     if name == "k_mg_down(level 0)":  # b, dinv, nnb, stencil, r1 out
         return n * (Ts + 4 + 1 + 8 * kbar + Tm)
     if name == "k_cg_Ap":            # z, p in, nnb, stencil, p out, Ap out
         return n * (Tz + Ts + 1 + 8 * kbar + 2 * Ts)
-    if name == "k_cg_update":        # psi, r, p, Ap in; psi, r out
-        return n * 6 * Ts
+    if name == "k_cg_update_down":   # psi, r, p, Ap, dinv, nnb, stencil in; psi, r, r1 out
+        return n * (6 * Ts + 5 + 8 * kbar + Tm)
     if name == "k_particles_advect":  # pos in, pos+prev out (FP64), velocity field once
         return P * 72 + n * 3 * Ts
     if name == "k_departures":       # vel, omega, centre in; q0, dep_b, dep_f out

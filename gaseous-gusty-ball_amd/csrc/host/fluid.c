@@ -181,7 +181,7 @@ struct ggb_fluid_dev {
 	HostLevel *hl;          /* [nlev] */
 	void **xbufs;
 	int nxbufs;
-	void *omega, *omega_band, *psi, *psi_alt, *vel, *q0, *q1, *q2, *dep_b, *dep_f, *r, *pbuf[2], *Ap, *z;
+	void *omega, *omega_band, *psi, *psi_alt, *vel, *q0, *q1, *q2, *dep_b, *dep_f, *rbuf[2], *pbuf[2], *Ap, *z;
 	ggbk_scalars *s;        /* device */
 	ggbk_scalars *hs;       /* pinned host copy */
 	ggbk_red red;
@@ -505,7 +505,8 @@ int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
 	D->q2 = xbuf(D, ts * n, "fluid");
 	D->dep_b = xbuf(D, 3 * ts * n, "fluid");
 	D->dep_f = xbuf(D, 3 * ts * n, "fluid");
-	D->r = xbuf(D, ts * n, "fluid");
+	D->rbuf[0] = xbuf(D, ts * n, "fluid");
+	D->rbuf[1] = xbuf(D, ts * n, "fluid");
 	D->pbuf[0] = xbuf(D, ts * n, "fluid");
 	D->pbuf[1] = xbuf(D, ts * n, "fluid");
 	D->Ap = xbuf(D, ts * n, "fluid");
@@ -748,13 +749,35 @@ int fluid_set_f64(Fluid *F, int field, const double *in)
 static int mean_op(const struct ggb_fluid_dev *D) { return D->T == GGBK_F64 ? GGBK_FIN_MEAN_F64 : GGBK_FIN_MEAN_F32; }
 
 /* z = M^-1 r and rz (fluid.c:148-160, 315-318, 344-348) */
-static void precondition(struct ggb_fluid_dev *D, int first)
+static void precondition(struct ggb_fluid_dev *D, const void *r, int first)
 {
 	if (D->nlev > 0)
-		ggbk_vcycle(D->levels, D->nlev, D->r, D->T, D->z, D->TM, D->s, D->red);
+		ggbk_vcycle(D->levels, D->nlev, r, D->T, D->z, D->TM, D->s, D->red);
 	else
-		ggbk_jacobi_precond(D->g0, D->r, D->z, D->T, D->s, D->red);
+		ggbk_jacobi_precond(D->g0, r, D->z, D->T, D->s, D->red);
 	ggbk_fin(first ? GGBK_FIN_RZ_FIRST : GGBK_FIN_RZ, D->s, D->red, D->nb0, 0, 0, 0);
+}
+
+/* One PCG iteration after the first preconditioning (fluid.c:318-350): Ap and pAp, alpha,
+ * then the update and the preconditioner.  With a multigrid hierarchy the update is fused
+ * into the V-cycle's first pass and r alternates between two arrays. */
+static void cg_iteration(struct ggb_fluid_dev *D, int k)
+{
+	ggbk_grid g = D->g0;
+	void *pin = D->pbuf[k & 1], *pout = D->pbuf[(k + 1) & 1];
+	void *rin = D->rbuf[k & 1], *rout = D->rbuf[(k + 1) & 1];
+	ggbk_cg_Ap(g, D->z, D->TZ, pin, pout, D->Ap, D->T, k == 0, D->s, D->red);
+	ggbk_fin(GGBK_FIN_PAP, D->s, D->red, D->nb0, 0, 0, 0);
+	ggbk_cg_fuse u = { D->psi, rin, pout, D->Ap };
+	if (ggbk_vcycle_update(D->levels, D->nlev, &u, rout, D->T, D->z, D->TM, D->s, D->red) == 0) {
+		ggbk_fin(GGBK_FIN_RZ, D->s, D->red, D->nb0, 0, 0, 0);
+		return;
+	}
+	/* no hierarchy to fuse into: separate update in place */
+	ggbk_cg_update(g, D->psi, rin, pout, D->Ap, D->T, D->s, D->red);
+	ggbk_fin(GGBK_FIN_RR, D->s, D->red, D->nb0, 0, 0, 0);
+	ggbk_d2d(rout, rin, D->tsz * (size_t) D->n);
+	precondition(D, rout, 0);
 }
 
 static void read_scalars(struct ggb_fluid_dev *D)
@@ -769,24 +792,21 @@ static int solve_device(Fluid *F, int maxit, int sub_mean)
 {
 	struct ggb_fluid_dev *D = F->dev;
 	ggbk_grid g = D->g0;
-	ggbk_cg_start(g, D->omega, D->psi, D->r, D->T, sub_mean, D->s, D->red);
+	ggbk_cg_start(g, D->omega, D->psi, D->rbuf[0], D->T, sub_mean, D->s, D->red);
 	ggbk_fin(GGBK_FIN_START, D->s, D->red, D->nb0, F->prm.cg_tol, maxit, 0);
-	precondition(D, 1);
-	int batch = D->predict > 1 ? D->predict - 1 : 1;
+	precondition(D, D->rbuf[0], 1);
+	/* Iterations are enqueued in batches and the host reads the loop flag after each; the
+	 * first batch is the previous solve's count, later batches grow geometrically.  Passes
+	 * of iterations past convergence find the flag cleared and store nothing. */
+	int batch = D->predict > 1 ? D->predict : 1;
 	int k = 0;
 	for (;;) {
-		for (int b = 0; b < batch; b++, k++) {
-			void *pin = D->pbuf[k & 1], *pout = D->pbuf[(k + 1) & 1];
-			ggbk_cg_Ap(g, D->z, D->TZ, pin, pout, D->Ap, D->T, k == 0, D->s, D->red);
-			ggbk_fin(GGBK_FIN_PAP, D->s, D->red, D->nb0, 0, 0, 0);
-			ggbk_cg_update(g, D->psi, D->r, pout, D->Ap, D->T, D->s, D->red);
-			ggbk_fin(GGBK_FIN_RR, D->s, D->red, D->nb0, 0, 0, 0);
-			precondition(D, 0);
-		}
+		for (int b = 0; b < batch && k < maxit; b++, k++)
+			cg_iteration(D, k);
 		read_scalars(D);
 		if (!D->hs->active || k >= maxit)
 			break;
-		batch = 1;
+		batch = batch < 64 ? 2 * batch : 64;
 	}
 	ggbk_cg_zero_if_bb0(g, D->psi, D->T, D->s);
 	F->cg_iters = D->hs->it;

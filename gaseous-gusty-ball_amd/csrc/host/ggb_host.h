@@ -6,6 +6,7 @@
 
 #include "cubesphere.h"
 #include "fluid.h"
+#include "particles.h"
 #include "../kern/ggbk.h"
 
 /* Device copy of a grid's tables (column-major ELL), uploaded on first use and
@@ -30,5 +31,10 @@ int ggb_ensure_device(void);
 const Fluid *ggb_fluid_of_vel(const float *vel);
 /* device velocity and its element type of a live fluid */
 const void *ggb_fluid_dev_vel(const Fluid *F, int *T);
+
+/* A particle set's current device arrays, and those the renderer writes the next
+ * (tile) storage order into; ggb_particles_dev_swap adopts the written order. */
+void ggb_particles_dev_parts(const Particles *P, ggbk_parts *cur, ggbk_parts *nxt);
+void ggb_particles_dev_swap(const Particles *P);
 
 #endif

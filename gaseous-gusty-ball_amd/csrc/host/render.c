@@ -14,10 +14,6 @@
 #include "ggb_host.h"
 #include "render.h"
 
-void ggb_particles_dev_arrays(const Particles *P, const double **prev, const double **pos, const float **col);
-
-#define RENDER_TILE 32
-
 struct ggb_render_dev {
 	ggbk_render k;
 	long long last_entries;
@@ -60,8 +56,8 @@ int render_init(Render *r, int R, float radius, float alpha)
 
 	ggbk_render *k = &D->k;
 	k->R = R;
-	k->tile = RENDER_TILE;
-	k->ntx = (R + RENDER_TILE - 1) / RENDER_TILE;
+	k->tile = ggbk_render_tile();
+	k->ntx = (R + k->tile - 1) / k->tile;
 	k->ntiles = 6 * k->ntx * k->ntx;
 	k->radius = r->radius;
 	k->alpha = alpha;
@@ -69,15 +65,23 @@ int render_init(Render *r, int R, float radius, float alpha)
 	k->tile_count = ggbk_malloc(sizeof(int) * ((size_t) k->ntiles + 1));
 	k->tile_start = ggbk_malloc(sizeof(int) * ((size_t) k->ntiles + 1));
 	k->tile_cursor = ggbk_malloc(sizeof(int) * ((size_t) k->ntiles + 1));
+	k->home_count = ggbk_malloc(sizeof(int) * ((size_t) k->ntiles + 1));
+	k->home_start = ggbk_malloc(sizeof(int) * ((size_t) k->ntiles + 1));
+	k->home_cursor = ggbk_malloc(sizeof(int) * ((size_t) k->ntiles + 1));
 	k->total = ggbk_malloc(sizeof(long long) * 16);
+	k->tile_diag = ggbk_malloc(sizeof(int) * 8 * (size_t) k->ntiles);
+	k->scan_part = ggbk_malloc(sizeof(long long) * 3 * (size_t) ggbk_render_scan_blocks(k->ntiles));
 	k->cap = 0;
 	k->list = NULL;
 	k->sort_scratch = NULL;
-	if (!k->buf || !k->tile_count || !k->tile_start || !k->tile_cursor || !k->total) {
+	if (!k->buf || !k->tile_count || !k->tile_start || !k->tile_cursor || !k->home_count || !k->home_start ||
+	    !k->home_cursor || !k->total || !k->tile_diag ||
+	    !k->scan_part) {
 		fprintf(stderr, "render: out of device memory\n");
 		return -1;
 	}
-	if (ggbk_zero(k->buf, sizeof(float) * count))
+	if (ggbk_zero(k->buf, sizeof(float) * count) || ggbk_zero(k->tile_diag, sizeof(int) * 8 * (size_t) k->ntiles) ||
+	    ggbk_zero(k->total, sizeof(long long) * 16))
 		return -1;
 	D->gen_dev = 1;
 	D->gen_host = 1; /* both zero */
@@ -93,7 +97,12 @@ void render_free(Render *r)
 		ggbk_free(D->k.tile_count);
 		ggbk_free(D->k.tile_start);
 		ggbk_free(D->k.tile_cursor);
+		ggbk_free(D->k.home_count);
+		ggbk_free(D->k.home_start);
+		ggbk_free(D->k.home_cursor);
 		ggbk_free(D->k.total);
+		ggbk_free(D->k.tile_diag);
+		ggbk_free(D->k.scan_part);
 		ggbk_free(D->k.list);
 		ggbk_free(D->k.sort_scratch);
 		pthread_mutex_destroy(&D->mu);
@@ -117,15 +126,15 @@ void render_fade(Render *r, double fade)
 
 void render_deposit(Render *r, const Particles *P)
 {
-	const double *prev, *pos;
-	const float *col;
+	ggbk_parts cur, nxt;
 	if (P->n <= 0)
 		return;
-	ggb_particles_dev_arrays(P, &prev, &pos, &col);
+	ggb_particles_dev_parts(P, &cur, &nxt);
 	float invg = (float) (1.0 / r->g);
-	long long e = ggbk_render_deposit(&r->dev->k, P->n, prev, pos, col, invg);
+	long long e = ggbk_render_deposit(&r->dev->k, cur, nxt, invg);
 	if (e < 0)
 		ggb_die("render: deposit failed (%s)", ggbk_last_error());
+	ggb_particles_dev_swap(P);
 	r->dev->last_entries = e;
 	r->dev->gen_dev++;
 }

@@ -250,57 +250,18 @@ __device__ __forceinline__ double block_max(double v)
 	return r;
 }
 
-// Last-block election: every block has stored its partial (thread 0) before the call.
-// Returns true in all threads of the block that arrives last; the ticket is reset for
-// the next use.  Device-scope fences make the other blocks' partials (written through
-// other XCDs' L2s) visible.
-__device__ __forceinline__ bool last_block(unsigned int *ticket)
+// Logical block index with each XCD's share of the grid contiguous.  The dispatcher
+// deals workgroups to the 8 XCDs round-robin (workgroup k runs on XCD k mod 8) and each
+// XCD has its own L2, so with the plain index the cells of rows i +- N, which a stencil
+// gathers, are fetched by three L2s.  Here XCD x runs logical blocks
+// [x q + min(x, r), ...) with q = nb / 8, r = nb % 8: a bijection onto [0, nb).
+#define GGBK_NXCD 8
+__device__ __forceinline__ int xblock(void)
 {
-	__shared__ int is_last;
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		__threadfence();
-		unsigned int t = atomicAdd(ticket, 1u);
-		is_last = (t == gridDim.x - 1);
-	}
-	__syncthreads();
-	if (is_last) {
-		__threadfence();
-		if (threadIdx.x == 0)
-			__hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-	}
-	return is_last != 0;
-}
-
-__device__ __forceinline__ double load_coherent(const double *p)
-{
-	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Fold nb partials in block order (thread 0 result; the fold is sequential in index
-// order within chunks of BS, then chunks in order -- fixed for a given nb).
-template <int BS>
-__device__ __forceinline__ double fold_partials(const double *part, int nb)
-{
-	double acc = 0;
-	for (int base = 0; base < nb; base += BS) {
-		int i = base + threadIdx.x;
-		double v = i < nb ? load_coherent(part + i) : 0.0;
-		acc += block_sum<BS>(v);
-	}
-	return acc;
-}
-
-template <int BS>
-__device__ __forceinline__ double fold_partials_max(const double *part, int nb)
-{
-	double acc = 0;
-	for (int base = 0; base < nb; base += BS) {
-		int i = base + threadIdx.x;
-		double v = i < nb ? load_coherent(part + i) : 0.0;
-		acc = fmax(acc, block_max<BS>(v));
-	}
-	return acc;
+	int k = blockIdx.x, nb = gridDim.x;
+	int x = k % GGBK_NXCD, idx = k / GGBK_NXCD;
+	int q = nb / GGBK_NXCD, r = nb % GGBK_NXCD;
+	return x * q + min(x, r) + idx;
 }
 
 // Kernel launch error check helper for host launchers.

@@ -17,7 +17,7 @@
 #include "ggbk.h"
 
 #define BS 256
-#define FIN_BS 256
+#define FIN_BS 1024
 
 template <typename T>
 __device__ __forceinline__ double applyK_at(const ggbk_grid &g, const T *x, int i)
@@ -36,24 +36,33 @@ __device__ __forceinline__ void store_partial(ggbk_red red, int slot, double v)
 {
 	double b = block_sum<BS>(v);
 	if (threadIdx.x == 0)
-		red.part[(size_t) slot * red.stride + blockIdx.x] = b;
+		red.part[(size_t) slot * red.stride + xblock()] = b;
 }
 
 __device__ __forceinline__ void store_partial_max(ggbk_red red, int slot, double v)
 {
 	double b = block_max<BS>(v);
 	if (threadIdx.x == 0)
-		red.part[(size_t) slot * red.stride + blockIdx.x] = b;
+		red.part[(size_t) slot * red.stride + xblock()] = b;
 }
 
 // ---------------- fold kernel ----------------
 
 // Fold nb partials: thread t sums p[t], p[t + FIN_BS], ... in order, then the block
-// tree; fixed order for a given nb.
+// tree; fixed order for a given nb.  Loads are issued four at a time so a fold costs
+// about one round trip to L2 per four partials per thread.
 __device__ __forceinline__ double fold_sum(const double *p, int nb)
 {
 	double acc = 0;
-	for (int i = threadIdx.x; i < nb; i += FIN_BS)
+	int i = threadIdx.x;
+	for (; i + 3 * FIN_BS < nb; i += 4 * FIN_BS) {
+		double a0 = p[i], a1 = p[i + FIN_BS], a2 = p[i + 2 * FIN_BS], a3 = p[i + 3 * FIN_BS];
+		acc += a0;
+		acc += a1;
+		acc += a2;
+		acc += a3;
+	}
+	for (; i < nb; i += FIN_BS)
 		acc += p[i];
 	return block_sum<FIN_BS>(acc);
 }
@@ -61,7 +70,12 @@ __device__ __forceinline__ double fold_sum(const double *p, int nb)
 __device__ __forceinline__ double fold_max(const double *p, int nb)
 {
 	double acc = 0;
-	for (int i = threadIdx.x; i < nb; i += FIN_BS)
+	int i = threadIdx.x;
+	for (; i + 3 * FIN_BS < nb; i += 4 * FIN_BS) {
+		double a0 = p[i], a1 = p[i + FIN_BS], a2 = p[i + 2 * FIN_BS], a3 = p[i + 3 * FIN_BS];
+		acc = fmax(fmax(acc, a0), fmax(a1, fmax(a2, a3)));
+	}
+	for (; i < nb; i += FIN_BS)
 		acc = fmax(acc, p[i]);
 	return block_max<FIN_BS>(acc);
 }
@@ -163,7 +177,7 @@ __global__ __launch_bounds__(FIN_BS) void k_fin(int op, ggbk_scalars *s, ggbk_re
 template <typename T>
 __global__ __launch_bounds__(BS) void k_curl(ggbk_grid g, const T *vel, T *out)
 {
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	if (i >= g.n)
 		return;
 	dv3 p = cell_pos(g, i);
@@ -183,7 +197,7 @@ __global__ __launch_bounds__(BS) void k_curl(ggbk_grid g, const T *vel, T *out)
 template <typename T>
 __global__ __launch_bounds__(BS) void k_divergence(ggbk_grid g, const T *vel, T *out)
 {
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	if (i >= g.n)
 		return;
 	dv3 p = cell_pos(g, i);
@@ -202,7 +216,7 @@ __global__ __launch_bounds__(BS) void k_divergence(ggbk_grid g, const T *vel, T 
 template <typename T>
 __global__ __launch_bounds__(BS) void k_laplacian(ggbk_grid g, const T *x, T *y)
 {
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	if (i >= g.n)
 		return;
 	double s = 0, xi = (double) x[i];
@@ -215,7 +229,7 @@ __global__ __launch_bounds__(BS) void k_laplacian(ggbk_grid g, const T *x, T *y)
 template <typename T>
 __global__ __launch_bounds__(BS) void k_applyK(ggbk_grid g, const T *x, double *y)
 {
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	if (i < g.n)
 		y[i] = applyK_at(g, x, i);
 }
@@ -224,7 +238,7 @@ __global__ __launch_bounds__(BS) void k_applyK(ggbk_grid g, const T *x, double *
 template <typename T>
 __global__ __launch_bounds__(BS) void k_mean_partial(ggbk_grid g, const T *f, ggbk_red red)
 {
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	double v = i < g.n ? (double) g.area[i] * (double) f[i] : 0.0;
 	store_partial(red, GGBK_SL_MEAN, v);
 }
@@ -232,7 +246,7 @@ __global__ __launch_bounds__(BS) void k_mean_partial(ggbk_grid g, const T *f, gg
 template <typename T>
 __global__ __launch_bounds__(BS) void k_sub_mean(int n, T *f, const ggbk_scalars *s)
 {
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	if (i < n)
 		f[i] -= (T) s->mean;
 }
@@ -243,7 +257,7 @@ template <typename T>
 __global__ __launch_bounds__(BS) void k_cg_start(ggbk_grid g, T *omega, const T *psi, T *r, int sub_mean,
 						  const ggbk_scalars *s, ggbk_red red)
 {
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	double bbl = 0, rrl = 0;
 	if (i < g.n) {
 		T om = omega[i];
@@ -270,7 +284,7 @@ __global__ __launch_bounds__(BS) void k_cg_Ap(ggbk_grid g, const TZ *z, const T 
 	if (!s->active)
 		return;
 	double beta = first ? 0.0 : s->beta;
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	double v = 0;
 	if (i < g.n) {
 		T pi = first ? (T) z[i] : (T) ((double) z[i] + beta * (double) p_in[i]);
@@ -295,7 +309,7 @@ __global__ __launch_bounds__(BS) void k_cg_update(int n, T *psi, T *r, const T *
 	if (!s->active)
 		return;
 	double alpha = s->alpha;
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	double v = 0;
 	if (i < n) {
 		psi[i] += (T) (alpha * (double) p[i]);
@@ -311,7 +325,7 @@ __global__ __launch_bounds__(BS) void k_jacobi_precond(ggbk_grid g, const T *r, 
 {
 	if (!s->active)
 		return;
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	double v = 0;
 	if (i < g.n) {
 		T zi = g.dinv[i] * r[i];
@@ -326,7 +340,7 @@ __global__ __launch_bounds__(BS) void k_zero_if_bb0(int n, T *psi, const ggbk_sc
 {
 	if (s->status != 2)
 		return;
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	if (i < n)
 		psi[i] = (T) 0;
 }
@@ -339,7 +353,7 @@ template <typename T>
 __global__ __launch_bounds__(BS) void k_velocity(ggbk_grid g, const T *psi_in, T *psi_out, int sub_mean, T *vel,
 						  const ggbk_scalars *s, ggbk_red red)
 {
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	double m = 0;
 	if (i < g.n) {
 		T mean = sub_mean ? (T) s->mean : (T) 0;
@@ -375,7 +389,7 @@ __global__ __launch_bounds__(BS) void k_velocity(ggbk_grid g, const T *psi_in, T
 template <typename T>
 __global__ __launch_bounds__(BS) void k_max_speed2(int n, const T *vel, ggbk_red red)
 {
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	double m = 0;
 	if (i < n) {
 		const T *v = vel + 3 * (size_t) i;
@@ -391,7 +405,7 @@ template <typename T>
 __global__ __launch_bounds__(BS) void k_departures(ggbk_grid g, const T *vel, const T *omega, T *q0, T *dep_b, T *dep_f,
 						    double dt, double twoOmega, int fwd)
 {
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	if (i >= g.n)
 		return;
 	dv3 p = cell_pos(g, i);
@@ -411,7 +425,7 @@ __global__ __launch_bounds__(BS) void k_departures(ggbk_grid g, const T *vel, co
 template <typename T>
 __global__ __launch_bounds__(BS) void k_advect(ggbk_grid g, const T *src, const T *dep, T *dst)
 {
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	if (i < g.n)
 		dst[i] = (T) sample_scalar(g, src, ld3(dep + 3 * (size_t) i));
 }
@@ -421,7 +435,7 @@ __global__ __launch_bounds__(BS) void k_advect(ggbk_grid g, const T *src, const 
 template <typename T>
 __global__ __launch_bounds__(BS) void k_advect_bfecc(ggbk_grid g, const T *q1, const T *q, const T *dep_f, T *q2)
 {
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	if (i >= g.n)
 		return;
 	T a = (T) sample_scalar(g, q1, ld3(dep_f + 3 * (size_t) i));
@@ -436,7 +450,7 @@ __global__ __launch_bounds__(BS) void k_limited_relax(ggbk_grid g, const T *q2, 
 						       int bfecc, const T *band, T *omega, double twoOmega, double relax,
 						       ggbk_red red)
 {
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	double part = 0;
 	if (i < g.n) {
 		T qn;

@@ -169,7 +169,16 @@ typedef struct {
 } ggbk_level;
 /* z = V(r); with red.part != NULL also the r.z partial (slot RZ) of ggbk_nblocks(n0) blocks */
 void ggbk_vcycle(const ggbk_level *levels, int nlev, const void *rin, int T, void *zout, int TM,
-		 const ggbk_scalars *s, ggbk_red red);
+		 ggbk_scalars *s, ggbk_red red);
+/* PCG update fused into the V-cycle's first pass: psi += alpha p; r_out = r_in - alpha Ap
+ * (fold FIN_RR enqueued after it); then z = V(r_out) as ggbk_vcycle.  Needs nlev >= 2;
+ * returns -1 (nothing enqueued) otherwise.  r_out must not alias r_in. */
+typedef struct {
+	void *psi;
+	const void *r_in, *p, *Ap;
+} ggbk_cg_fuse;
+int ggbk_vcycle_update(const ggbk_level *levels, int nlev, const ggbk_cg_fuse *u, void *r_out, int T,
+		       void *zout, int TM, ggbk_scalars *s, ggbk_red red);
 
 /* ---- velocity (reference fluid.c:360-376) + max speed (fluid.c:480-491) ---- */
 /* [psi_out = psi_in - mean] vel = p x grad(psi_out); vmax partial (slot VMAX) */
@@ -189,8 +198,16 @@ void ggbk_advect_bfecc(ggbk_grid g, const void *q1, const void *q, const void *d
 void ggbk_limited_relax(ggbk_grid g, const void *q2, const void *q, const void *dep_b, const void *q1, int bfecc,
 			const void *band, void *omega, int T, double twoOmega, double relax, ggbk_red red);
 
-/* ---- particles (reference particles.c) ---- */
-void ggbk_particles_init(int n, double *pos, double *prev, float *col, int round32);
+/* ---- particles (reference particles.c) ----
+ * Particle arrays in storage order: slot s holds particle pid[s].  The renderer keeps
+ * the slots in tile order (each deposit writes the next order into a second set). */
+typedef struct {
+	int n, pad_;
+	double *pos, *prev;   /* [3n] */
+	float *col;           /* [4n] */
+	int *pid;             /* [n] */
+} ggbk_parts;
+void ggbk_particles_init(ggbk_parts p, int round32);  /* identity order */
 void ggbk_particles_color(int n, const double *pos, float *col, const float *img, int w, int h);
 void ggbk_particles_advect(ggbk_grid g, int np, double *pos, double *prev, const void *vel, int T,
 			   double dt, int round32);
@@ -203,18 +220,26 @@ typedef struct {
 	int *tile_count;            /* [ntiles + 1] */
 	int *tile_start;            /* [ntiles + 1] */
 	int *tile_cursor;           /* [ntiles] */
-	int *list;                  /* [cap] particle indices per tile */
+	int *home_count;            /* [ntiles + 1] particles whose current position is on the tile */
+	int *home_start;            /* [ntiles + 1] */
+	int *home_cursor;           /* [ntiles] */
+	int *list;                  /* [cap] particle slots per tile */
 	long long cap;
-	int *sort_scratch;          /* [scratch_cap] for tiles whose list exceeds the LDS sort */
+	unsigned long long *sort_scratch; /* [scratch_cap] for tiles whose list exceeds the LDS sort */
 	long long scratch_cap;
 	long long *total;           /* [16] entries, longest list, draw diagnostics (device) */
+	int *tile_diag;             /* [8 * ntiles] per-tile draw diagnostics */
+	long long *scan_part;       /* [3 * ggbk_render_scan_blocks(ntiles)] */
 } ggbk_render;
+int ggbk_render_tile(void);     /* tile edge the kernels are built for */
+int ggbk_render_scan_blocks(int ntiles);
 void ggbk_render_scale(float *buf, size_t count, float g);
-/* copies the 16 diagnostics of the last deposit (synchronises) */
+/* sums the per-tile diagnostics of the last deposit and copies the 16 values
+ * (synchronises) */
 int ggbk_render_stats(const ggbk_render *r, long long *out16);
-/* bin + ordered blend; returns the number of (particle, tile) entries, or -1 on error */
-long long ggbk_render_deposit(ggbk_render *r, int np, const double *prev, const double *pos,
-			      const float *col, float invg);
+/* bin + ordered blend of the particles `in`; writes them to `out` in the next storage
+ * (tile) order.  Returns the number of (particle, tile) entries, or -1 on error. */
+long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_parts out, float invg);
 
 #ifdef __cplusplus
 }

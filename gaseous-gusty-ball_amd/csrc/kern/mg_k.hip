@@ -7,9 +7,14 @@
 //            the neighbours, so x1 is never stored;
 //   restrict b_coarse = R r1 as a padded-row gather in fine-index order (deterministic);
 //   up       x = x1 + P x_coarse, then x_out = x + 0.8 dinv (b - K x), with x recomputed at
-//            the neighbours (out of place), plus the r.z partial on level 0 for PCG.
+//            the neighbours (out of place), plus the r.z partial on level 0 for PCG; on
+//            large levels x is stored by a prolongation pass and smoothed by a second.
 // Below the first level with at most DENSE_MAX cells the V-cycle is a fixed linear map;
 // it is precomputed on the host and applied as one dense matrix-vector product.
+//
+// The kernels of a speculative CG iteration after convergence must not store anything:
+// they read the device flag `active` together with their data and check it only before
+// storing, so the flag's load adds no latency to these short, latency-bound passes.
 //
 // TM is the level storage type (double in FP64 mode, float in the mixed and reference
 // modes); TB is the type of the level-0 right-hand side (the CG residual).  Per-cell
@@ -26,20 +31,25 @@
 #define COARSE_BS 1024
 #define DENSE_BS 256
 
-// pre-sweep value at cell j: r = (TM) b, x1 = 0 + (TM)(0.8 dinv r)
+// pre-sweep value of a cell with right-hand side bj: r = (TM) b, x1 = 0 + (TM)(0.8 dinv r)
+template <typename TM, typename TB>
+__device__ __forceinline__ TM x1_of(TB bj, float dinv)
+{
+	TM rj = (TM) (double) bj;
+	return (TM) 0 + (TM) (MG_OMEGA * (double) dinv * (double) rj);
+}
+
 template <typename TM, typename TB>
 __device__ __forceinline__ TM x1_at(const ggbk_grid &g, const TB *b, int j)
 {
-	TM rj = (TM) (double) b[j];
-	return (TM) 0 + (TM) (MG_OMEGA * (double) g.dinv[j] * (double) rj);
+	return x1_of<TM, TB>(b[j], g.dinv[j]);
 }
 
 template <typename TM, typename TB>
 __global__ __launch_bounds__(BS) void k_mg_down(ggbk_grid g, const TB *b, TM *r1, const ggbk_scalars *s)
 {
-	if (!s->active)
-		return;
-	int i = blockIdx.x * BS + threadIdx.x;
+	int act = s->active; // waited for only at the store
+	int i = xblock() * BS + threadIdx.x;
 	if (i >= g.n)
 		return;
 	double xi = (double) x1_at<TM, TB>(g, b, i), sum = 0;
@@ -48,7 +58,46 @@ __global__ __launch_bounds__(BS) void k_mg_down(ggbk_grid g, const TB *b, TM *r1
 		int j = g.nbr[(size_t) k * g.n + i];
 		sum += (double) g.w[(size_t) k * g.n + i] * (xi - (double) x1_at<TM, TB>(g, b, j));
 	}
-	r1[i] = (TM) ((double) b[i] - sum);
+	if (act)
+		r1[i] = (TM) ((double) b[i] - sum);
+}
+
+// PCG update fused with the level-0 pre-smooth residual (fluid.c:336-345 then the first
+// pass of the V-cycle): psi += alpha p, r_out = r_in - alpha Ap, and r1 = r_out - K x1
+// with x1 from r_out, where the neighbours' r_out is recomputed from r_in and Ap (r_out
+// is a separate array, so no cell reads a value another has already replaced).  Stores
+// the same values as k_cg_update followed by k_mg_down, plus the r.r partial.
+template <typename T, typename TM>
+__global__ __launch_bounds__(BS) void k_cg_update_down(ggbk_grid g, T *psi, const T *r_in, T *r_out, const T *p,
+							const T *Ap, TM *r1, const ggbk_scalars *s, ggbk_red red)
+{
+	int act = s->active;
+	double alpha = s->alpha;
+	int i = xblock() * BS + threadIdx.x;
+	double v = 0;
+	if (i < g.n) {
+		double ri = (double) r_in[i] - alpha * (double) Ap[i];
+		T rt = (T) ri;
+		T ps = psi[i] + (T) (alpha * (double) p[i]);
+		double xi = (double) x1_of<TM, T>(rt, g.dinv[i]), sum = 0;
+		int nn = g.nnb[i];
+		for (int k = 0; k < nn; k++) {
+			int j = g.nbr[(size_t) k * g.n + i];
+			T rj = (T) ((double) r_in[j] - alpha * (double) Ap[j]);
+			sum += (double) g.w[(size_t) k * g.n + i] * (xi - (double) x1_of<TM, T>(rj, g.dinv[j]));
+		}
+		if (act) {
+			psi[i] = ps;
+			r_out[i] = rt;
+			r1[i] = (TM) ((double) rt - sum);
+		}
+		v = ri * ri;
+	}
+	if (act) {
+		double bsum = block_sum<BS>(v);
+		if (threadIdx.x == 0)
+			red.part[(size_t) GGBK_SL_RR * red.stride + xblock()] = bsum;
+	}
 }
 
 // coarse b = sum over the row of R of rw * r_fine (product in storage precision,
@@ -57,9 +106,8 @@ __global__ __launch_bounds__(BS) void k_mg_down(ggbk_grid g, const TB *b, TM *r1
 template <typename TM>
 __global__ __launch_bounds__(BS) void k_mg_restrict(ggbk_xfer xf, const TM *rf, TM *bc, const ggbk_scalars *s)
 {
-	if (!s->active)
-		return;
-	int c = blockIdx.x * BS + threadIdx.x;
+	int act = s->active;
+	int c = xblock() * BS + threadIdx.x;
 	if (c >= xf.nc)
 		return;
 	size_t nc = (size_t) xf.nc;
@@ -76,7 +124,8 @@ __global__ __launch_bounds__(BS) void k_mg_restrict(ggbk_xfer xf, const TM *rf, 
 	for (int k = 0; k < GGBK_RMAX; k++)
 		if (k < xf.rmax)
 			acc += (double) (w[k] * rf[idx[k]]);
-	bc[c] = (TM) acc;
+	if (act)
+		bc[c] = (TM) acc;
 }
 
 // x after prolongation at cell j: x1 + (TM)(1.0 * sum_k pw_k xc[tap_k]) with the tap
@@ -94,9 +143,8 @@ template <typename TM, typename TB>
 __global__ __launch_bounds__(BS) void k_mg_up(ggbk_grid g, ggbk_xfer xf, const TB *b, const TM *xc, TM *xout,
 					       const ggbk_scalars *s, ggbk_red red)
 {
-	if (!s->active)
-		return;
-	int i = blockIdx.x * BS + threadIdx.x;
+	int act = s->active;
+	int i = xblock() * BS + threadIdx.x;
 	double dot = 0;
 	if (i < g.n) {
 		TM xi = xcorr_at<TM, TB>(g, xf, b, xc, i);
@@ -108,22 +156,70 @@ __global__ __launch_bounds__(BS) void k_mg_up(ggbk_grid g, ggbk_xfer xf, const T
 		}
 		TM ri = (TM) ((double) b[i] - sum);
 		TM xo = xi + (TM) (MG_OMEGA * (double) g.dinv[i] * (double) ri);
-		xout[i] = xo;
+		if (act)
+			xout[i] = xo;
 		dot = (double) b[i] * (double) xo;
 	}
-	if (red.part) {
+	if (red.part && act) {
 		double bsum = block_sum<BS>(dot);
 		if (threadIdx.x == 0)
-			red.part[(size_t) GGBK_SL_RZ * red.stride + blockIdx.x] = bsum;
+			red.part[(size_t) GGBK_SL_RZ * red.stride + xblock()] = bsum;
 	}
 }
+
+// Two-pass form of k_mg_up for large levels, where recomputing the prolongation at every
+// neighbour costs more load instructions than storing it once:
+//   prolong  xp = x1 + P x_coarse (into the level's residual array, free after restrict)
+//   smooth   x_out = xp + 0.8 dinv (b - K xp), plus the r.z partial on level 0.
+// Both store the same TM values as k_mg_up, so the results are identical.
+template <typename TM, typename TB>
+__global__ __launch_bounds__(BS) void k_mg_prolong(ggbk_grid g, ggbk_xfer xf, const TB *b, const TM *xc, TM *xp,
+						    const ggbk_scalars *s)
+{
+	int act = s->active;
+	int i = xblock() * BS + threadIdx.x;
+	if (i < g.n) {
+		TM v = xcorr_at<TM, TB>(g, xf, b, xc, i);
+		if (act)
+			xp[i] = v;
+	}
+}
+
+template <typename TM, typename TB>
+__global__ __launch_bounds__(BS) void k_mg_smooth(ggbk_grid g, const TB *b, const TM *xp, TM *xout,
+						   const ggbk_scalars *s, ggbk_red red)
+{
+	int act = s->active;
+	int i = xblock() * BS + threadIdx.x;
+	double dot = 0;
+	if (i < g.n) {
+		TM xi = xp[i];
+		double sum = 0;
+		int nn = g.nnb[i];
+		for (int k = 0; k < nn; k++) {
+			int j = g.nbr[(size_t) k * g.n + i];
+			sum += (double) g.w[(size_t) k * g.n + i] * ((double) xi - (double) xp[j]);
+		}
+		double bi = (double) b[i];
+		TM ri = (TM) (bi - sum);
+		TM xo = xi + (TM) (MG_OMEGA * (double) g.dinv[i] * (double) ri);
+		if (act)
+			xout[i] = xo;
+		dot = bi * (double) xo;
+	}
+	if (red.part && act) {
+		double bsum = block_sum<BS>(dot);
+		if (threadIdx.x == 0)
+			red.part[(size_t) GGBK_SL_RZ * red.stride + xblock()] = bsum;
+	}
+}
+#define TWO_PASS_MIN_CELLS 262144
 
 // x = M b, one wave per row, rows split over blocks; the row sum is a fixed tree.
 template <typename TM, typename TD, typename TB>
 __global__ __launch_bounds__(DENSE_BS) void k_mg_dense(const TD *M, int n, const TB *b, TM *x, const ggbk_scalars *s)
 {
-	if (!s->active)
-		return;
+	int act = s->active;
 	int row = blockIdx.x * (DENSE_BS / 64) + (threadIdx.x >> 6);
 	int lane = threadIdx.x & 63;
 	if (row >= n)
@@ -133,7 +229,7 @@ __global__ __launch_bounds__(DENSE_BS) void k_mg_dense(const TD *M, int n, const
 	for (int c = lane; c < n; c += 64)
 		acc += (double) m[c] * (double) b[c];
 	acc = wave_sum(acc);
-	if (lane == 0)
+	if (lane == 0 && act)
 		x[row] = (TM) acc;
 }
 
@@ -143,7 +239,7 @@ __global__ __launch_bounds__(BS) void k_mg_first(ggbk_grid g, const TB *b, TM *x
 {
 	if (!s->active)
 		return;
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	if (i < g.n)
 		x[i] = x1_at<TM, TB>(g, b, i);
 }
@@ -154,7 +250,7 @@ __global__ __launch_bounds__(BS) void k_mg_residual(ggbk_grid g, const TB *b, co
 {
 	if (!s->active)
 		return;
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	if (i >= g.n)
 		return;
 	double xi = (double) x[i], sum = 0;
@@ -171,7 +267,7 @@ __global__ __launch_bounds__(BS) void k_mg_jacobi(ggbk_grid g, TM *x, const TM *
 {
 	if (!s->active)
 		return;
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	if (i < g.n)
 		x[i] += (TM) (MG_OMEGA * (double) g.dinv[i] * (double) r[i]);
 }
@@ -213,11 +309,11 @@ __global__ __launch_bounds__(BS) void k_dot_partial(int n, const TB *r, const TM
 {
 	if (!s->active)
 		return;
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	double v = i < n ? (double) r[i] * (double) z[i] : 0.0;
 	double bsum = block_sum<BS>(v);
 	if (threadIdx.x == 0)
-		red.part[(size_t) GGBK_SL_RZ * red.stride + blockIdx.x] = bsum;
+		red.part[(size_t) GGBK_SL_RZ * red.stride + xblock()] = bsum;
 }
 
 #define NB(n) ggbk_blocks((n), BS)
@@ -237,8 +333,11 @@ static void coarsest(const ggbk_level &L, const TB *b, TM *x, TM *r, const ggbk_
 	}
 }
 
+// With u != NULL the level-0 pre-smooth pass is the fused PCG update, writing b itself
+// (b = u->r_in - alpha Ap), and the r.r fold follows it.
 template <typename TM, typename TB>
-static void vcycle_level(const ggbk_level *lv, int nlev, int l, const TB *b, TM *x, const ggbk_scalars *s, ggbk_red red)
+static void vcycle_level(const ggbk_level *lv, int nlev, int l, const TB *b, TM *x, ggbk_scalars *s, ggbk_red red,
+			 const ggbk_cg_fuse *u = nullptr)
 {
 	const ggbk_level &L = lv[l];
 	const ggbk_grid &g = L.g;
@@ -258,23 +357,37 @@ static void vcycle_level(const ggbk_level *lv, int nlev, int l, const TB *b, TM 
 		return;
 	}
 	const ggbk_level &C = lv[l + 1];
-	if (l == 0)
-		ggbk_probe_begin(GGBK_PROBE_MG_DOWN0);
-	GGBK_LAUNCH((k_mg_down<TM, TB>), NB(g.n), BS, 0, g, b, r, s);
-	if (l == 0)
-		ggbk_probe_end(GGBK_PROBE_MG_DOWN0);
+	if (u) {
+		ggbk_probe_begin(GGBK_PROBE_CG_UPDATE);
+		GGBK_LAUNCH((k_cg_update_down<TB, TM>), NB(g.n), BS, 0, g, (TB *) u->psi, (const TB *) u->r_in,
+			    (TB *) b, (const TB *) u->p, (const TB *) u->Ap, r, s, red);
+		ggbk_probe_end(GGBK_PROBE_CG_UPDATE);
+		ggbk_fin(GGBK_FIN_RR, s, red, (int) NB(g.n), 0, 0, 0);
+	} else {
+		if (l == 0)
+			ggbk_probe_begin(GGBK_PROBE_MG_DOWN0);
+		GGBK_LAUNCH((k_mg_down<TM, TB>), NB(g.n), BS, 0, g, b, r, s);
+		if (l == 0)
+			ggbk_probe_end(GGBK_PROBE_MG_DOWN0);
+	}
 	GGBK_LAUNCH((k_mg_restrict<TM>), NB(L.xf.nc), BS, 0, L.xf, (const TM *) r, (TM *) C.b, s);
 	ggbk_red none = { nullptr, 0 };
 	vcycle_level<TM, TM>(lv, nlev, l + 1, (const TM *) C.b, (TM *) C.x, s, none);
 	if (l == 0)
 		ggbk_probe_begin(GGBK_PROBE_MG_UP0);
-	GGBK_LAUNCH((k_mg_up<TM, TB>), NB(g.n), BS, 0, g, L.xf, b, (const TM *) C.x, x, s, l == 0 ? red : none);
+	if (g.n >= TWO_PASS_MIN_CELLS) {
+		GGBK_LAUNCH((k_mg_prolong<TM, TB>), NB(g.n), BS, 0, g, L.xf, b, (const TM *) C.x, r, s);
+		GGBK_LAUNCH((k_mg_smooth<TM, TB>), NB(g.n), BS, 0, g, b, (const TM *) r, x, s, l == 0 ? red : none);
+	} else {
+		GGBK_LAUNCH((k_mg_up<TM, TB>), NB(g.n), BS, 0, g, L.xf, b, (const TM *) C.x, x, s,
+			    l == 0 ? red : none);
+	}
 	if (l == 0)
 		ggbk_probe_end(GGBK_PROBE_MG_UP0);
 }
 
 extern "C" void ggbk_vcycle(const ggbk_level *levels, int nlev, const void *rin, int T, void *zout, int TM,
-			    const ggbk_scalars *s, ggbk_red red)
+			    ggbk_scalars *s, ggbk_red red)
 {
 	if (TM == GGBK_F64)
 		vcycle_level<double, double>(levels, nlev, 0, (const double *) rin, (double *) zout, s, red);
@@ -282,4 +395,18 @@ extern "C" void ggbk_vcycle(const ggbk_level *levels, int nlev, const void *rin,
 		vcycle_level<float, double>(levels, nlev, 0, (const double *) rin, (float *) zout, s, red);
 	else
 		vcycle_level<float, float>(levels, nlev, 0, (const float *) rin, (float *) zout, s, red);
+}
+
+extern "C" int ggbk_vcycle_update(const ggbk_level *levels, int nlev, const ggbk_cg_fuse *u, void *r_out, int T,
+				  void *zout, int TM, ggbk_scalars *s, ggbk_red red)
+{
+	if (nlev < 2 || levels[0].M)
+		return -1;
+	if (TM == GGBK_F64)
+		vcycle_level<double, double>(levels, nlev, 0, (const double *) r_out, (double *) zout, s, red, u);
+	else if (T == GGBK_F64)
+		vcycle_level<float, double>(levels, nlev, 0, (const double *) r_out, (float *) zout, s, red, u);
+	else
+		vcycle_level<float, float>(levels, nlev, 0, (const float *) r_out, (float *) zout, s, red, u);
+	return 0;
 }

@@ -3,7 +3,8 @@
 // circles through the bilinear velocity field (reference src/particles.c:8-74).
 //
 // Positions are stored in double.  With round32 set, every stored position is rounded
-// to float, which reproduces the reference's float particle arrays.
+// to float, which reproduces the reference's float particle arrays.  The kernels work
+// per storage slot; the slot order (pid) only matters to the host's index-ordered views.
 #include <hip/hip_runtime.h>
 
 #include "dev.h"
@@ -18,9 +19,10 @@ __device__ __forceinline__ void store_pos(double *p, dv3 a, int round32)
 	st3(p, a);
 }
 
-__global__ __launch_bounds__(BS) void k_particles_init(int n, double *pos, double *prev, float *col, int round32)
+__global__ __launch_bounds__(BS) void k_particles_init(ggbk_parts ps, int round32)
 {
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
+	int n = ps.n;
 	if (i >= n)
 		return;
 	const double golden = GGB_PI * (3.0 - sqrt(5.0));
@@ -28,17 +30,18 @@ __global__ __launch_bounds__(BS) void k_particles_init(int n, double *pos, doubl
 	double r = sqrt(1.0 - y * y);
 	double phi = golden * i;
 	dv3 p = d3(r * cos(phi), y, r * sin(phi));
-	store_pos(pos + 3 * (size_t) i, p, round32);
-	store_pos(prev + 3 * (size_t) i, p, round32);
+	store_pos(ps.pos + 3 * (size_t) i, p, round32);
+	store_pos(ps.prev + 3 * (size_t) i, p, round32);
 	for (int c = 0; c < 4; c++)
-		col[4 * (size_t) i + c] = 1.0f;
+		ps.col[4 * (size_t) i + c] = 1.0f;
+	ps.pid[i] = i;
 }
 
 // bilinear colour at the particle's longitude/latitude; x wraps, y clamps
 __global__ __launch_bounds__(BS) void k_particles_color(int n, const double *pos, float *col, const float *img, int w,
 							int h)
 {
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	if (i >= n)
 		return;
 	dv3 d = ld3(pos + 3 * (size_t) i);
@@ -64,7 +67,7 @@ template <typename T>
 __global__ __launch_bounds__(BS) void k_particles_advect(ggbk_grid g, int np, double *pos, double *prev, const T *vel,
 							 double dt, int round32)
 {
-	int i = blockIdx.x * BS + threadIdx.x;
+	int i = xblock() * BS + threadIdx.x;
 	if (i >= np)
 		return;
 	dv3 p = ld3(pos + 3 * (size_t) i);
@@ -75,9 +78,9 @@ __global__ __launch_bounds__(BS) void k_particles_advect(ggbk_grid g, int np, do
 	store_pos(pos + 3 * (size_t) i, gc_move(p, u2, dt), round32);
 }
 
-extern "C" void ggbk_particles_init(int n, double *pos, double *prev, float *col, int round32)
+extern "C" void ggbk_particles_init(ggbk_parts p, int round32)
 {
-	GGBK_LAUNCH(k_particles_init, ggbk_blocks(n, BS), BS, 0, n, pos, prev, col, round32);
+	GGBK_LAUNCH(k_particles_init, ggbk_blocks(p.n, BS), BS, 0, p, round32);
 }
 
 extern "C" void ggbk_particles_color(int n, const double *pos, float *col, const float *img, int w, int h)

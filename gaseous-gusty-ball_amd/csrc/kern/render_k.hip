@@ -3,22 +3,27 @@
 // The reference emits one record per (sample, face, 32-row strip), sorts them
 // bin-major / thread-minor and blends each bin in order, so every texel sees its
 // splats in (particle index, sample) order.  Here the unit of work is a face TILE of
-// TILE x TILE texels held in LDS:
+// TILE x TILE texels held in LDS, and the particle arrays are kept in tile order
+// (storage slot s holds particle pid[s]):
 //   1. count  -- each particle adds itself once to every tile that the bounding box of
-//                its arc's footprints (per face) overlaps; for short arcs the box comes
-//                from the two endpoints (the arc is a straight segment on the face);
-//   2. scan   -- exclusive scan of the tile counts (one workgroup);
-//   3. fill   -- the same boxes write the particle index into each tile's list;
-//   4. draw   -- one workgroup per tile: sort the list (bitonic, LDS), load the tile,
-//                then per window of WIN particles (ascending index): each particle
-//                computes its samples on the tile once (cached in LDS) and appends a
-//                2-byte fragment (window rank, sample) to every texel it touches; each
-//                texel then sorts its few fragments and blends them in (particle,
-//                sample) order -- the reference's per-texel order, with no
-//                cross-particle synchronisation.  A window in which a texel collects
-//                more than CAPT fragments is retried with a quarter of the particles;
-//                a single particle with more than SMAX samples on the tile is blended
-//                in order by one thread.
+//                its arc's footprints (per face) overlaps, and once to its home tile
+//                (the tile of the texel nearest its current position).  For short arcs
+//                the box comes from the two endpoints (the arc is a straight segment on
+//                the face).  Atomics are aggregated per wave, one per distinct tile:
+//                neighbouring slots hold neighbouring particles;
+//   2. scan   -- exclusive scans of both counts;
+//   3. fill   -- the same boxes write the particle's slot into each tile's list, and
+//                each particle is copied to its home tile's range of the next storage
+//                order, which the caller adopts after the draw;
+//   4. draw   -- one workgroup per tile: load the tile, then per window of particles
+//                each one computes its samples on the tile once (cached in LDS) and
+//                appends a fragment to every texel it touches; each texel sorts its few
+//                fragments by (particle index, sample) and blends them in that order --
+//                the reference's per-texel order.  A tile with more than WIN particles
+//                is sorted by particle index and processed in windows; a window in
+//                which a texel collects more than CAPT fragments is retried with a
+//                quarter of the particles, and a single particle with more than SMAX
+//                samples on the tile is blended in order by one thread.
 // The texel arithmetic is the reference's (render.c:113-154) in float with no FMA
 // contraction, so texels match the CPU path bit for bit for identical positions.
 #include <hip/hip_runtime.h>
@@ -27,21 +32,24 @@
 #include "dev.h"
 #include "ggbk.h"
 
-#define PBS 256          // particle kernels
-#define DBS 256          // draw workgroup
-#define TILE_MAX 32
-#define SORT_CAP 1024    // LDS sort capacity per tile
+#define PBS 256           // particle kernels
+#define TILE 16
+#define NTEX (TILE * TILE)
+#define WIN 128           // particles per window = draw workgroup size
+#define DBS WIN
+#define SMAX 12           // samples per particle cached in LDS (sample index fits 4 bits)
+#define CAPT 12           // fragments one texel can hold per window
+#define SORT_CAP 512      // LDS sort capacity per tile
 #define SCAN_BS 1024
 
 struct SplatGeo {
-	int R;
+	int R, ntx;
 	float rad;
 	double spacing;
 };
 
-__device__ __forceinline__ int arc_samples(dv3 a, dv3 b, double spacing)
+__device__ __forceinline__ int arc_samples(double ang, double spacing)
 {
-	double ang = vangle(a, b);
 	int ns = (int) ceil(ang / spacing);
 	if (ns < 1)
 		ns = 1;
@@ -60,10 +68,10 @@ __device__ __forceinline__ dv3 arc_point(dv3 a, dv3 b, int k, int ns)
 // not within 60 degrees (dc < 0.5).
 __device__ __forceinline__ bool face_pixel(int f, dv3 d, int R, float &px, float &py)
 {
-	double dc, u, v;
-	gnomonic(f, d, dc, u, v);
 	if (!(face_dot(f, d) >= 0.5))
 		return false;
+	double dc, u, v;
+	gnomonic(f, d, dc, u, v);
 	px = (float) ((u + 1.0) * 0.5 * R - 0.5);
 	py = (float) ((1.0 - v) * 0.5 * R - 0.5);
 	return true;
@@ -83,166 +91,276 @@ __device__ __forceinline__ bool footprint(float px, float py, float rad, int R, 
 	return xlo <= xhi && ylo <= yhi;
 }
 
-// Per-face union of the footprint boxes of a particle's samples (conservative).
-struct FaceBoxes {
-	int xlo[6], xhi[6], ylo[6], yhi[6];
-};
-
-__device__ __forceinline__ void box_add(FaceBoxes &fb, int f, int xlo, int xhi, int ylo, int yhi)
+// Tile of the texel nearest the particle's current position, on the face it lies on.
+__device__ __forceinline__ int home_tile(dv3 p, int R, int ntx)
 {
-	fb.xlo[f] = min(fb.xlo[f], xlo);
-	fb.xhi[f] = max(fb.xhi[f], xhi);
-	fb.ylo[f] = min(fb.ylo[f], ylo);
-	fb.yhi[f] = max(fb.yhi[f], yhi);
+	int f = face_of(p);
+	double dc, u, v;
+	gnomonic(f, p, dc, u, v);
+	double px = (u + 1.0) * 0.5 * R - 0.5, py = (1.0 - v) * 0.5 * R - 0.5;
+	int ix = (int) fmin(fmax(floor(px + 0.5), 0.0), (double) (R - 1));
+	int iy = (int) fmin(fmax(floor(py + 0.5), 0.0), (double) (R - 1));
+	return (f * ntx + iy / TILE) * ntx + ix / TILE;
 }
 
-// exact per-sample boxes of face f
-__device__ __forceinline__ void face_box_samples(dv3 a, dv3 b, int ns, int f, const SplatGeo &sg, FaceBoxes &fb)
-{
-	for (int k = 1; k <= ns; k++) {
-		dv3 d = arc_point(a, b, k, ns);
-		if (!(face_dot(f, d) >= 0.5))
-			continue;
-		float px, py;
-		face_pixel(f, d, sg.R, px, py);
-		int xlo, xhi, ylo, yhi;
-		if (footprint(px, py, sg.rad, sg.R, xlo, xhi, ylo, yhi))
-			box_add(fb, f, xlo, xhi, ylo, yhi);
-	}
-}
-
+// Union over the arc's samples of their footprint boxes on face f (conservative);
+// false if no sample reaches the face.  `ns` is the sample count, computed on first use
+// (0 = not yet).
+//
 // Short arcs: every sample d(t) = normalize((1-t) a + t b) has face_dot between
 // min(da, db) and max(da, db) / cos(angle / 2), so with both endpoints at dc >= 0.45 the
 // arc projects onto face f as the straight segment between the endpoint pixels (great
 // circles are gnomonic lines), and with both below 0.45 no sample reaches dc = 0.5.
 // Otherwise the samples are walked.  The box is widened by 0.01 texel for rounding.
 #define SHORT_ARC 0.05
-__device__ __forceinline__ void arc_boxes(dv3 a, dv3 b, const SplatGeo &sg, FaceBoxes &fb)
+__device__ __forceinline__ bool face_box(dv3 a, dv3 b, double ang, int &ns, int f, const SplatGeo &sg, int &xlo,
+					 int &xhi, int &ylo, int &yhi)
 {
-	for (int f = 0; f < 6; f++) {
-		fb.xlo[f] = INT_MAX;
-		fb.ylo[f] = INT_MAX;
-		fb.xhi[f] = INT_MIN;
-		fb.yhi[f] = INT_MIN;
-	}
-	double ang = vangle(a, b);
-	int ns = 0;
 	bool shortarc = ang < SHORT_ARC;
-	for (int f = 0; f < 6; f++) {
-		double da = face_dot(f, a), db = face_dot(f, b);
-		if (shortarc && da < 0.45 && db < 0.45)
-			continue;
-		if (shortarc && da >= 0.45 && db >= 0.45) {
-			double dc, u, v;
-			gnomonic(f, a, dc, u, v);
-			double pxa = (u + 1.0) * 0.5 * sg.R - 0.5, pya = (1.0 - v) * 0.5 * sg.R - 0.5;
-			gnomonic(f, b, dc, u, v);
-			double pxb = (u + 1.0) * 0.5 * sg.R - 0.5, pyb = (1.0 - v) * 0.5 * sg.R - 0.5;
-			double x0 = fmin(pxa, pxb) - 0.01 - sg.rad, x1 = fmax(pxa, pxb) + 0.01 + sg.rad;
-			double y0 = fmin(pya, pyb) - 0.01 - sg.rad, y1 = fmax(pya, pyb) + 0.01 + sg.rad;
-			int xlo = max((int) ceil(x0), 0), xhi = min((int) floor(x1), sg.R - 1);
-			int ylo = max((int) ceil(y0), 0), yhi = min((int) floor(y1), sg.R - 1);
-			if (xlo <= xhi && ylo <= yhi)
-				box_add(fb, f, xlo, xhi, ylo, yhi);
-			continue;
-		}
-		if (ns == 0) {
-			ns = (int) ceil(ang / sg.spacing);
-			ns = ns < 1 ? 1 : (ns > 4096 ? 4096 : ns);
-		}
-		face_box_samples(a, b, ns, f, sg, fb);
+	double da = face_dot(f, a), db = face_dot(f, b);
+	if (shortarc && da < 0.45 && db < 0.45)
+		return false;
+	if (shortarc && da >= 0.45 && db >= 0.45) {
+		double dc, u, v;
+		gnomonic(f, a, dc, u, v);
+		double pxa = (u + 1.0) * 0.5 * sg.R - 0.5, pya = (1.0 - v) * 0.5 * sg.R - 0.5;
+		gnomonic(f, b, dc, u, v);
+		double pxb = (u + 1.0) * 0.5 * sg.R - 0.5, pyb = (1.0 - v) * 0.5 * sg.R - 0.5;
+		double x0 = fmin(pxa, pxb) - 0.01 - sg.rad, x1 = fmax(pxa, pxb) + 0.01 + sg.rad;
+		double y0 = fmin(pya, pyb) - 0.01 - sg.rad, y1 = fmax(pya, pyb) + 0.01 + sg.rad;
+		// clamp in double before converting (far-off projections of a grazing endpoint)
+		xlo = (int) fmax(ceil(x0), 0.0);
+		xhi = (int) fmin(floor(x1), (double) (sg.R - 1));
+		ylo = (int) fmax(ceil(y0), 0.0);
+		yhi = (int) fmin(floor(y1), (double) (sg.R - 1));
+		return xlo <= xhi && ylo <= yhi;
 	}
+	if (ns == 0)
+		ns = arc_samples(ang, sg.spacing);
+	xlo = ylo = INT_MAX;
+	xhi = yhi = INT_MIN;
+	for (int k = 1; k <= ns; k++) {
+		float px, py;
+		if (!face_pixel(f, arc_point(a, b, k, ns), sg.R, px, py))
+			continue;
+		int x0, x1, y0, y1;
+		if (footprint(px, py, sg.rad, sg.R, x0, x1, y0, y1)) {
+			xlo = min(xlo, x0);
+			xhi = max(xhi, x1);
+			ylo = min(ylo, y0);
+			yhi = max(yhi, y1);
+		}
+	}
+	return xlo <= xhi;
 }
 
-__global__ __launch_bounds__(PBS) void k_render_count(int np, const double *prev, const double *pos, SplatGeo sg,
-						      int tile, int ntx, int *counts)
+// Adds one per participating lane to cnt[key] with one atomic per distinct key in the
+// wave and returns each participating lane's slot (previous count + rank among the lanes
+// with the same key, in lane order); -1 for lanes not participating.  Called by every
+// lane of the wave.
+__device__ __forceinline__ int wave_atomic_inc(int *cnt, int key, bool on)
 {
-	int i = blockIdx.x * PBS + threadIdx.x;
-	if (i >= np)
-		return;
-	FaceBoxes fb;
-	arc_boxes(ld3(prev + 3 * (size_t) i), ld3(pos + 3 * (size_t) i), sg, fb);
+	unsigned long long todo = __ballot(on);
+	int lane = __lane_id();
+	int slot = -1;
+	while (todo) {
+		int leader = __ffsll((long long) todo) - 1;
+		int lk = __shfl(key, leader);
+		unsigned long long m = __ballot(on && key == lk) & todo;
+		int base = 0;
+		if (lane == leader)
+			base = atomicAdd(&cnt[lk], __popcll(m));
+		base = __shfl(base, leader);
+		if ((m >> lane) & 1ull)
+			slot = base + __popcll(m & ((1ull << lane) - 1ull));
+		todo &= ~m;
+	}
+	return slot;
+}
+
+// Visits every tile of every face box of particle i's arc: fn(has, tile) is called by
+// all lanes of the wave together, once per tile index step, with has false for lanes
+// that have no tile at this step.
+template <typename Fn>
+__device__ __forceinline__ void for_arc_tiles(bool on, dv3 a, dv3 b, const SplatGeo &sg, Fn fn)
+{
+	double ang = on ? vangle(a, b) : 0.0;
+	int ns = 0;
 	for (int f = 0; f < 6; f++) {
-		if (fb.xlo[f] > fb.xhi[f])
-			continue;
-		for (int ty = fb.ylo[f] / tile; ty <= fb.yhi[f] / tile; ty++)
-			for (int tx = fb.xlo[f] / tile; tx <= fb.xhi[f] / tile; tx++)
-				atomicAdd(&counts[(f * ntx + ty) * ntx + tx], 1);
+		int xlo = 0, xhi = -1, ylo = 0, yhi = -1;
+		bool any = on && face_box(a, b, ang, ns, f, sg, xlo, xhi, ylo, yhi);
+		int tx0 = xlo / TILE, ty0 = ylo / TILE;
+		int w = any ? xhi / TILE - tx0 + 1 : 0;
+		int nt = any ? w * (yhi / TILE - ty0 + 1) : 0;
+		for (int s = 0; __any(s < nt); s++) {
+			bool has = s < nt;
+			int t = has ? (f * sg.ntx + ty0 + s / w) * sg.ntx + tx0 + s % w : 0;
+			fn(has, t);
+		}
 	}
 }
 
-__global__ __launch_bounds__(PBS) void k_render_fill(int np, const double *prev, const double *pos, SplatGeo sg,
-						     int tile, int ntx, const int *starts, int *cursor, int *list,
+__global__ __launch_bounds__(PBS) void k_render_count(ggbk_parts ps, SplatGeo sg, int *counts, int *hcounts)
+{
+	int i = xblock() * PBS + threadIdx.x;
+	bool on = i < ps.n;
+	dv3 a = d3(0, 0, 1), b = d3(0, 0, 1);
+	if (on) {
+		a = ld3(ps.prev + 3 * (size_t) i);
+		b = ld3(ps.pos + 3 * (size_t) i);
+	}
+	int home = on ? home_tile(b, sg.R, sg.ntx) : 0;
+	wave_atomic_inc(hcounts, home, on);
+	for_arc_tiles(on, a, b, sg, [&](bool has, int t) { wave_atomic_inc(counts, t, has); });
+}
+
+__global__ __launch_bounds__(PBS) void k_render_fill(ggbk_parts ps, ggbk_parts out, SplatGeo sg, const int *starts,
+						     int *cursor, const int *hstarts, int *hcursor, int *list,
 						     long long cap)
 {
-	int i = blockIdx.x * PBS + threadIdx.x;
-	if (i >= np)
-		return;
-	FaceBoxes fb;
-	arc_boxes(ld3(prev + 3 * (size_t) i), ld3(pos + 3 * (size_t) i), sg, fb);
-	for (int f = 0; f < 6; f++) {
-		if (fb.xlo[f] > fb.xhi[f])
-			continue;
-		for (int ty = fb.ylo[f] / tile; ty <= fb.yhi[f] / tile; ty++)
-			for (int tx = fb.xlo[f] / tile; tx <= fb.xhi[f] / tile; tx++) {
-				int t = (f * ntx + ty) * ntx + tx;
-				long long slot = (long long) starts[t] + atomicAdd(&cursor[t], 1);
-				if (slot < cap)
-					list[slot] = i;
-			}
+	int i = xblock() * PBS + threadIdx.x;
+	bool on = i < ps.n;
+	dv3 a = d3(0, 0, 1), b = d3(0, 0, 1);
+	if (on) {
+		a = ld3(ps.prev + 3 * (size_t) i);
+		b = ld3(ps.pos + 3 * (size_t) i);
+	}
+	// the particle's place in the next storage order
+	int home = on ? home_tile(b, sg.R, sg.ntx) : 0;
+	int hr = wave_atomic_inc(hcursor, home, on);
+	if (on) {
+		size_t o = (size_t) hstarts[home] + hr;
+		st3(out.pos + 3 * o, b);
+		st3(out.prev + 3 * o, a);
+		*(float4 *) (out.col + 4 * o) = *(const float4 *) (ps.col + 4 * (size_t) i);
+		out.pid[o] = ps.pid[i];
+	}
+	for_arc_tiles(on, a, b, sg, [&](bool has, int t) {
+		int r = wave_atomic_inc(cursor, t, has);
+		if (has) {
+			long long slot = (long long) starts[t] + r;
+			if (slot < cap)
+				list[slot] = i;
+		}
+	});
+}
+
+// Exclusive scans of the tile list counts and the home counts, in two passes over
+// blocks of SCAN_BS tiles: local scans plus per-block totals, then each block adds the
+// totals of the blocks before it.  stats[0] = total list entries, stats[1] = longest
+// tile list; stats[2..15] are cleared.
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+	int lane = __lane_id();
+	for (int off = 1; off < 64; off <<= 1) {
+		int u = __shfl_up(v, off);
+		if (lane >= off)
+			v += u;
+	}
+	return v;
+}
+
+__global__ __launch_bounds__(SCAN_BS) void k_render_scan_local(int ntiles, const int *counts, int *starts,
+							       int *cursor, const int *hcounts, int *hstarts,
+							       int *hcursor, long long *part)
+{
+	__shared__ int wsum[SCAN_BS / 64], hwsum[SCAN_BS / 64], wmax[SCAN_BS / 64];
+	int t = blockIdx.x * SCAN_BS + threadIdx.x;
+	int c = t < ntiles ? counts[t] : 0;
+	int hc = t < ntiles ? hcounts[t] : 0;
+	int ic = wave_incl_scan(c), ihc = wave_incl_scan(hc);
+	int m = c;
+	for (int off = 32; off > 0; off >>= 1)
+		m = max(m, __shfl_xor(m, off));
+	int w = threadIdx.x / 64, lane = __lane_id();
+	if (lane == 63) {
+		wsum[w] = ic;
+		hwsum[w] = ihc;
+		wmax[w] = m;
+	}
+	__syncthreads();
+	int before = 0, hbefore = 0;
+	for (int k = 0; k < w; k++) {
+		before += wsum[k];
+		hbefore += hwsum[k];
+	}
+	if (t < ntiles) {
+		starts[t] = before + ic - c;
+		hstarts[t] = hbefore + ihc - hc;
+		cursor[t] = 0;
+		hcursor[t] = 0;
+	}
+	if (threadIdx.x == 0) {
+		long long s = 0, hs = 0;
+		int mm = 0;
+		for (int k = 0; k < SCAN_BS / 64; k++) {
+			s += wsum[k];
+			hs += hwsum[k];
+			mm = max(mm, wmax[k]);
+		}
+		part[3 * (size_t) blockIdx.x] = s;
+		part[3 * (size_t) blockIdx.x + 1] = hs;
+		part[3 * (size_t) blockIdx.x + 2] = mm;
 	}
 }
 
-// Exclusive scan of tile counts (one workgroup).  stats[0] = total entries,
-// stats[1] = longest tile list.
-__global__ __launch_bounds__(SCAN_BS) void k_render_scan(int ntiles, const int *counts, int *starts, int *cursor,
-							 long long *stats)
+__global__ __launch_bounds__(SCAN_BS) void k_render_scan_fix(int ntiles, int *starts, int *hstarts,
+							     const long long *part, long long *stats)
 {
-	__shared__ long long part[SCAN_BS];
-	__shared__ int pmax[SCAN_BS];
-	int per = (ntiles + SCAN_BS - 1) / SCAN_BS;
-	int lo = threadIdx.x * per, hi = min(lo + per, ntiles);
-	long long s = 0;
-	int m = 0;
-	for (int t = lo; t < hi; t++) {
-		s += counts[t];
-		m = max(m, counts[t]);
+	__shared__ long long red[3][SCAN_BS / 64];
+	int b = blockIdx.x, nb = gridDim.x;
+	bool last = b == nb - 1;
+	long long s = 0, hs = 0, m = 0;
+	for (int j = threadIdx.x; j < (last ? nb : b); j += SCAN_BS) {
+		if (j < b) {
+			s += part[3 * (size_t) j];
+			hs += part[3 * (size_t) j + 1];
+		}
+		m = max(m, part[3 * (size_t) j + 2]);
 	}
-	part[threadIdx.x] = s;
-	pmax[threadIdx.x] = m;
+	for (int off = 32; off > 0; off >>= 1) {
+		s += __shfl_xor(s, off);
+		hs += __shfl_xor(hs, off);
+		m = max(m, __shfl_xor(m, off));
+	}
+	int w = threadIdx.x / 64;
+	if (__lane_id() == 0) {
+		red[0][w] = s;
+		red[1][w] = hs;
+		red[2][w] = m;
+	}
 	__syncthreads();
-	// Hillis-Steele inclusive scan of the per-thread sums
-	for (int off = 1; off < SCAN_BS; off <<= 1) {
-		long long v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
-		int mv = threadIdx.x >= off ? pmax[threadIdx.x - off] : 0;
-		__syncthreads();
-		part[threadIdx.x] += v;
-		pmax[threadIdx.x] = max(pmax[threadIdx.x], mv);
-		__syncthreads();
+	s = hs = m = 0;
+	for (int k = 0; k < SCAN_BS / 64; k++) {
+		s += red[0][k];
+		hs += red[1][k];
+		m = max(m, red[2][k]);
 	}
-	long long acc = threadIdx.x ? part[threadIdx.x - 1] : 0;
-	for (int t = lo; t < hi; t++) {
-		starts[t] = (int) acc;
-		acc += counts[t];
-		cursor[t] = 0;
+	int t = b * SCAN_BS + threadIdx.x;
+	if (t < ntiles) {
+		starts[t] += (int) s;
+		hstarts[t] += (int) hs;
 	}
-	if (threadIdx.x == SCAN_BS - 1) {
-		starts[ntiles] = (int) part[SCAN_BS - 1];
-		stats[0] = part[SCAN_BS - 1];
-		stats[1] = pmax[SCAN_BS - 1];
+	if (last && threadIdx.x == 0) {
+		long long tot = s + part[3 * (size_t) b], htot = hs + part[3 * (size_t) b + 1];
+		starts[ntiles] = (int) tot;
+		hstarts[ntiles] = (int) htot;
+		stats[0] = tot;
+		stats[1] = m;
 		for (int k = 2; k < 16; k++)
 			stats[k] = 0;
 	}
 }
 
-__device__ __forceinline__ void bitonic_sort(int *a, int P)
+// bitonic sort of P (a power of two) 64-bit keys, ascending
+__device__ __forceinline__ void bitonic_sort64(unsigned long long *a, int P)
 {
 	for (int k = 2; k <= P; k <<= 1)
 		for (int j = k >> 1; j > 0; j >>= 1) {
 			for (int i = threadIdx.x; i < P; i += DBS) {
 				int ixj = i ^ j;
 				if (ixj > i) {
-					int x = a[i], y = a[ixj];
+					unsigned long long x = a[i], y = a[ixj];
 					bool up = (i & k) == 0;
 					if ((x > y) == up) {
 						a[i] = y;
@@ -271,12 +389,7 @@ __device__ __forceinline__ bool tile_clip(const TileRect &tr, float px, float py
 	return xlo <= xhi && ylo <= yhi;
 }
 
-#define WIN 128        // particles per window (one thread each)
-#define SMAX 12        // samples per particle cached in LDS (k fits 4 bits)
-#define CAPT 12        // fragments one texel can hold per window
-#define NTEX (TILE_MAX * TILE_MAX)
-
-// Texels of one splat on the tile with positive tent weight (render.c:129-144 tests).
+// Texels of one splat on the tile with positive tent weight (render.c:129-144 tests);
 // fn(local texel index) is called for each, in y-then-x order.
 template <typename Fn>
 __device__ __forceinline__ void splat_texels(const TileRect &tr, float px, float py, float rad, float invrad, int R,
@@ -293,7 +406,7 @@ __device__ __forceinline__ void splat_texels(const TileRect &tr, float px, float
 			float wx = 1.0f - fabsf((float) x - px) * invrad;
 			if (wx <= 0)
 				continue;
-			fn((y - tr.y0) * TILE_MAX + (x - tr.x0));
+			fn((y - tr.y0) * TILE + (x - tr.x0));
 		}
 	}
 }
@@ -302,7 +415,7 @@ __device__ __forceinline__ void splat_texels(const TileRect &tr, float px, float
 __device__ __forceinline__ void blend_texel(float4 &dd, int t, const TileRect &tr, float px, float py, float invrad,
 					    float w0, float4 c, float invg)
 {
-	int x = tr.x0 + t % TILE_MAX, y = tr.y0 + t / TILE_MAX;
+	int x = tr.x0 + t % TILE, y = tr.y0 + t / TILE;
 	float wy = 1.0f - fabsf((float) y - py) * invrad;
 	float wx = 1.0f - fabsf((float) x - px) * invrad;
 	float w = w0 * wx * wy;
@@ -313,78 +426,94 @@ __device__ __forceinline__ void blend_texel(float4 &dd, int t, const TileRect &t
 	dd.w = dd.w * keep + add;
 }
 
-__global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, const double *prev, const double *pos,
-						     const float *col, SplatGeo sg, float invg, const long long *stats)
+__global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts ps, SplatGeo sg, float invg)
 {
 	__shared__ float4 tex[NTEX];
-	__shared__ int sbuf[SORT_CAP];
-	__shared__ float2 samp[SMAX][WIN];
-	__shared__ float4 pcol[WIN];
 	__shared__ int tcnt[NTEX];
 	__shared__ unsigned short frag[CAPT][NTEX];
-	__shared__ int serial;
-	__shared__ unsigned tmask[NTEX / 32];
-	if (stats[0] > r.cap)
-		return; // list overflowed: the host grows it before this launch
+	__shared__ float2 samp[SMAX][WIN];
+	__shared__ float4 pcol[WIN];
+	__shared__ int wpid[WIN];
+	__shared__ unsigned long long sbuf[SORT_CAP];
+	__shared__ unsigned char touched[NTEX];
+	__shared__ int overflow, ntouched;
 	long long c0 = clock64();
-	int t = blockIdx.x;
+	int t = xblock();
+	const int lid = threadIdx.x;
+	int *diag = r.tile_diag + 8 * (size_t) t;
 	int L = r.tile_start[t + 1] - r.tile_start[t];
-	if (L == 0)
+	if (L == 0 || r.total[0] > r.cap) {
+		// nothing to draw (or the list did not fit: the host grows it before launching)
+		if (lid < 8)
+			diag[lid] = 0;
 		return;
-	const int T = r.tile;
+	}
 	TileRect tr;
 	tr.f = t / (r.ntx * r.ntx);
 	int rem = t % (r.ntx * r.ntx);
-	tr.y0 = (rem / r.ntx) * T;
-	tr.x0 = (rem % r.ntx) * T;
-	tr.w = min(T, r.R - tr.x0);
-	tr.h = min(T, r.R - tr.y0);
+	tr.y0 = (rem / r.ntx) * TILE;
+	tr.x0 = (rem % r.ntx) * TILE;
+	tr.w = min(TILE, r.R - tr.x0);
+	tr.h = min(TILE, r.R - tr.y0);
 	const int *lst = r.list + r.tile_start[t];
 	const float rad = sg.rad, invrad = 1.0f / rad;
 
-	// sorted particle list: in LDS when it fits, else in this tile's scratch region
-	int P = 1;
-	while (P < L)
-		P <<= 1;
-	int *sorted = P <= SORT_CAP ? sbuf : r.sort_scratch + 2 * (size_t) r.tile_start[t];
-	for (int i = threadIdx.x; i < P; i += DBS)
-		sorted[i] = i < L ? lst[i] : INT_MAX;
-	__syncthreads();
-	bitonic_sort(sorted, P);
-
-	long long c1 = clock64();
 	float4 *gbuf = (float4 *) r.buf;
-	for (int i = threadIdx.x; i < NTEX; i += DBS) {
-		int ly = i / TILE_MAX, lx = i % TILE_MAX;
+	for (int i = lid; i < NTEX; i += DBS) {
+		int ly = i / TILE, lx = i % TILE;
 		if (ly < tr.h && lx < tr.w)
 			tex[i] = gbuf[((size_t) tr.f * r.R + tr.y0 + ly) * r.R + tr.x0 + lx];
 	}
-	for (int i = threadIdx.x; i < NTEX / 32; i += DBS)
-		tmask[i] = 0;
+	for (int i = lid; i < NTEX; i += DBS)
+		touched[i] = 0;
+	if (lid == 0)
+		ntouched = 0;
+
+	// A tile with at most WIN particles is one window in list order (fragments carry the
+	// particle index); otherwise (pid, slot) pairs are sorted and cut into windows.
+	unsigned long long *sorted = sbuf;
+	bool is_sorted = false;
+	auto sort_list = [&]() {
+		int P = 1;
+		while (P < L)
+			P <<= 1;
+		sorted = P <= SORT_CAP ? sbuf : r.sort_scratch + 2 * (size_t) r.tile_start[t];
+		for (int i = lid; i < P; i += DBS) {
+			unsigned long long key = ~0ull;
+			if (i < L) {
+				int slot = lst[i];
+				key = ((unsigned long long) (unsigned) ps.pid[slot] << 32) | (unsigned) slot;
+			}
+			sorted[i] = key;
+		}
+		__syncthreads();
+		bitonic_sort64(sorted, P);
+		is_sorted = true;
+	};
+	if (L > WIN)
+		sort_list();
+	__syncthreads();
+
 	long long csamp = 0, cblend = 0;
 	int windows = 0, serial_windows = 0;
-	// windows of W particles; a window whose fragments overflow is retried with W / 4
-	// particles (the tile is only written by a window that fits), down to one particle,
-	// which the serial path always handles
-	int W = WIN;
+	int W = min(L, WIN);
 	for (int base = 0; base < L;) {
 		long long ca = clock64();
 		windows++;
-		for (int i = threadIdx.x; i < NTEX; i += DBS)
+		for (int i = lid; i < NTEX; i += DBS)
 			tcnt[i] = 0;
-		if (threadIdx.x == 0)
-			serial = 0;
+		if (lid == 0)
+			overflow = 0;
 		__syncthreads();
-		// samples of this window's particles on the tile (one thread per particle); every
-		// (particle rank, sample, texel) fragment goes to its texel's slot list
-		const int rank = threadIdx.x;
-		if (rank < W && base + rank < L) {
-			int pid = sorted[base + rank];
-			dv3 a = ld3(prev + 3 * (size_t) pid), b = ld3(pos + 3 * (size_t) pid);
-			int ns = arc_samples(a, b, sg.spacing);
+		// samples of this window's particles on the tile (one thread per particle);
+		// every (particle, sample, texel) fragment goes to its texel's list
+		if (lid < W) {
+			int slot = is_sorted ? (int) (unsigned) sorted[base + lid] : lst[base + lid];
+			dv3 a = ld3(ps.prev + 3 * (size_t) slot), b = ld3(ps.pos + 3 * (size_t) slot);
+			int ns = arc_samples(vangle(a, b), sg.spacing);
 			int nsamp = 0;
 			bool over = false;
-			for (int k = 1; k <= ns; k++) {
+			for (int k = 1; k <= ns && !over; k++) {
 				float px, py;
 				if (!face_pixel(tr.f, arc_point(a, b, k, ns), sg.R, px, py))
 					continue;
@@ -395,111 +524,155 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, const double
 					over = true;
 					break;
 				}
-				samp[nsamp][rank] = make_float2(px, py);
-				unsigned short key = (unsigned short) ((rank << 4) | nsamp);
+				samp[nsamp][lid] = make_float2(px, py);
+				unsigned short key = (unsigned short) ((lid << 4) | nsamp);
 				splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
-					atomicOr(&tmask[tx >> 5], 1u << (tx & 31));
-					int slot = atomicAdd(&tcnt[tx], 1);
-					if (slot < CAPT)
-						frag[slot][tx] = key;
+					int s = atomicAdd(&tcnt[tx], 1);
+					if (s < CAPT)
+						frag[s][tx] = key;
 					else
 						over = true;
 				});
 				nsamp++;
 			}
-			const float *cp = col + 4 * (size_t) pid;
-			pcol[rank] = make_float4(cp[0], cp[1], cp[2], cp[3]);
+			pcol[lid] = *(const float4 *) (ps.col + 4 * (size_t) slot);
+			wpid[lid] = ps.pid[slot];
 			if (over)
-				serial = 1;
+				overflow = 1;
 		}
 		__syncthreads();
 		long long cb = clock64();
 		csamp += cb - ca;
-		if (!serial) {
-			// each texel applies its fragments in (particle rank, sample) order
-			for (int tx = threadIdx.x; tx < NTEX; tx += DBS) {
+		if (!overflow) {
+			// each texel applies its fragments in (particle index, sample) order
+			for (int tx = lid; tx < NTEX; tx += DBS) {
 				int n = tcnt[tx];
 				if (n == 0)
 					continue;
-				unsigned short keys[CAPT];
+				touched[tx] = 1;
+				unsigned long long keys[CAPT];
 #pragma unroll
-				for (int a = 0; a < CAPT; a++)
-					keys[a] = a < n ? frag[a][tx] : (unsigned short) 0xffff;
+				for (int q = 0; q < CAPT; q++) {
+					unsigned long long kk = ~0ull;
+					if (q < n) {
+						unsigned f = frag[q][tx];
+						kk = ((unsigned long long) (unsigned) wpid[f >> 4] << 12) | f;
+					}
+					keys[q] = kk;
+				}
 #pragma unroll
-				for (int a = 1; a < CAPT; a++) {
-					unsigned short key = keys[a];
+				for (int q = 1; q < CAPT; q++) {
 #pragma unroll
-					for (int bpos = a; bpos > 0; bpos--) {
-						if (keys[bpos - 1] > key) {
-							keys[bpos] = keys[bpos - 1];
-							keys[bpos - 1] = key;
-						}
+					for (int p = q; p > 0; p--) {
+						unsigned long long x = keys[p - 1], y = keys[p];
+						bool sw = x > y;
+						keys[p - 1] = sw ? y : x;
+						keys[p] = sw ? x : y;
 					}
 				}
 				float4 dd = tex[tx];
-				for (int a = 0; a < n; a++) {
-					int pr = keys[a] >> 4, k = keys[a] & 15;
-					float2 q = samp[k][pr];
+				for (int q = 0; q < n; q++) {
+					unsigned f = (unsigned) (keys[q] & 0xfffull);
+					int pr = f >> 4, k = f & 15;
+					float2 s2 = samp[k][pr];
 					float4 c = pcol[pr];
-					blend_texel(dd, tx, tr, q.x, q.y, invrad, r.alpha * c.w, c, invg);
+					blend_texel(dd, tx, tr, s2.x, s2.y, invrad, r.alpha * c.w, c, invg);
 				}
 				tex[tx] = dd;
 			}
 			base += W;
-			W = WIN;
+			W = min(WIN, L - base);
+		} else if (!is_sorted) {
+			// the single unsorted window overflowed before anything was blended: sort
+			// and retry in index-ordered windows
+			__syncthreads();
+			sort_list();
+			W = max(W / 4, 1);
 		} else if (W > 1) {
-			W = W >= 4 ? W / 4 : 1;
+			W = max(W / 4, 1);
 		} else {
 			// one particle with more than SMAX samples on the tile, or more than CAPT
 			// samples on one texel: one thread blends it in order
 			serial_windows++;
-			if (threadIdx.x == 0) {
-				for (int rr = 0; rr < 1 && base + rr < L; rr++) {
-					int pid = sorted[base + rr];
-					dv3 a = ld3(prev + 3 * (size_t) pid), b = ld3(pos + 3 * (size_t) pid);
-					int ns = arc_samples(a, b, sg.spacing);
-					const float *cp = col + 4 * (size_t) pid;
-					float4 c = make_float4(cp[0], cp[1], cp[2], cp[3]);
-					for (int k = 1; k <= ns; k++) {
-						float px, py;
-						if (!face_pixel(tr.f, arc_point(a, b, k, ns), sg.R, px, py))
-							continue;
-						splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
-							tmask[tx >> 5] |= 1u << (tx & 31);
-							blend_texel(tex[tx], tx, tr, px, py, invrad, r.alpha * c.w, c, invg);
-						});
-					}
+			if (lid == 0) {
+				int slot = (int) (unsigned) sorted[base];
+				dv3 a = ld3(ps.prev + 3 * (size_t) slot), b = ld3(ps.pos + 3 * (size_t) slot);
+				int ns = arc_samples(vangle(a, b), sg.spacing);
+				float4 c = *(const float4 *) (ps.col + 4 * (size_t) slot);
+				for (int k = 1; k <= ns; k++) {
+					float px, py;
+					if (!face_pixel(tr.f, arc_point(a, b, k, ns), sg.R, px, py))
+						continue;
+					splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
+						touched[tx] = 1;
+						blend_texel(tex[tx], tx, tr, px, py, invrad, r.alpha * c.w, c, invg);
+					});
 				}
 			}
 			base += 1;
-			W = WIN;
+			W = min(WIN, L - base);
 		}
 		__syncthreads();
 		cblend += clock64() - cb;
 	}
 
-	// write back the texels a splat touched
-	for (int i = threadIdx.x; i < NTEX; i += DBS) {
-		int ly = i / TILE_MAX, lx = i % TILE_MAX;
-		if ((tmask[i >> 5] >> (i & 31)) & 1u)
+	// write back the texels a splat touched (NTEX is a multiple of DBS: whole waves)
+	int cnt = 0;
+	for (int i = lid; i < NTEX; i += DBS) {
+		int ly = i / TILE, lx = i % TILE;
+		bool w = touched[i] != 0;
+		if (w)
 			gbuf[((size_t) tr.f * r.R + tr.y0 + ly) * r.R + tr.x0 + lx] = tex[i];
+		cnt += __popcll(__ballot(w));
 	}
-	// per-launch diagnostics: [2] windows, [3] serial windows, [4] sort cycles,
-	// [5] sample cycles, [6] blend cycles, [7] total cycles (sums over tiles),
-	// [8] touched texels
-	if (threadIdx.x < NTEX / 32) {
-		int c = __popc(tmask[threadIdx.x]);
-		if (c)
-			atomicAdd((unsigned long long *) &((long long *) stats)[8], (unsigned long long) c);
+	if (__lane_id() == 0)
+		atomicAdd(&ntouched, cnt);
+	__syncthreads();
+	// this tile's diagnostics (summed over tiles on request): touched texels, windows,
+	// serial windows, sorted, sample / blend / total cycles in units of 1024
+	if (lid == 0) {
+		diag[0] = ntouched;
+		diag[1] = windows;
+		diag[2] = serial_windows;
+		diag[3] = is_sorted ? 1 : 0;
+		diag[4] = (int) (csamp >> 10);
+		diag[5] = (int) (cblend >> 10);
+		diag[6] = (int) ((clock64() - c0) >> 10);
+		diag[7] = L;
 	}
-	if (threadIdx.x == 0) {
-		long long *st = (long long *) stats;
-		atomicAdd((unsigned long long *) &st[2], (unsigned long long) windows);
-		atomicAdd((unsigned long long *) &st[3], (unsigned long long) serial_windows);
-		atomicAdd((unsigned long long *) &st[4], (unsigned long long) (c1 - c0));
-		atomicAdd((unsigned long long *) &st[5], (unsigned long long) csamp);
-		atomicAdd((unsigned long long *) &st[6], (unsigned long long) cblend);
-		atomicAdd((unsigned long long *) &st[7], (unsigned long long) (clock64() - c0));
+}
+
+// Sums the per-tile diagnostics into stats[2..8] (one workgroup):
+// [2] windows, [3] serial windows, [4] sorted tiles, [5] sample cycles, [6] blend cycles,
+// [7] total cycles, [8] touched texels.
+__global__ __launch_bounds__(SCAN_BS) void k_render_diag_sum(int ntiles, const int *tile_diag, long long *stats)
+{
+	__shared__ long long part[7][SCAN_BS / 64];
+	long long acc[7] = { 0, 0, 0, 0, 0, 0, 0 };
+	for (int t = threadIdx.x; t < ntiles; t += SCAN_BS) {
+		const int *d = tile_diag + 8 * (size_t) t;
+		acc[0] += d[1];
+		acc[1] += d[2];
+		acc[2] += d[3];
+		acc[3] += (long long) d[4] << 10;
+		acc[4] += (long long) d[5] << 10;
+		acc[5] += (long long) d[6] << 10;
+		acc[6] += d[0];
+	}
+	int w = threadIdx.x / 64;
+	for (int k = 0; k < 7; k++) {
+		long long v = acc[k];
+		for (int off = 32; off > 0; off >>= 1)
+			v += __shfl_down(v, off);
+		if (__lane_id() == 0)
+			part[k][w] = v;
+	}
+	__syncthreads();
+	if (threadIdx.x < 7) {
+		long long v = 0;
+		for (int i = 0; i < SCAN_BS / 64; i++)
+			v += part[threadIdx.x][i];
+		stats[2 + threadIdx.x] = v;
 	}
 }
 
@@ -519,44 +692,54 @@ extern "C" void ggbk_render_scale(float *buf, size_t count, float g)
 	GGBK_LAUNCH(k_render_scale, (unsigned) blocks, 256, 0, buf, count, g);
 }
 
-// Host side of one deposit.  Returns the number of (particle, tile) entries, or -1.
-// The only synchronisation is the read-back of the entry total after the scan
-// (the list is grown when the total exceeds its capacity).
+extern "C" int ggbk_render_tile(void) { return TILE; }
+
+extern "C" int ggbk_render_scan_blocks(int ntiles) { return (ntiles + SCAN_BS - 1) / SCAN_BS; }
+
 extern "C" int ggbk_render_stats(const ggbk_render *r, long long *out16)
 {
+	GGBK_LAUNCH(k_render_diag_sum, 1, SCAN_BS, 0, r->ntiles, (const int *) r->tile_diag, r->total);
 	return ggbk_d2h(out16, r->total, 16 * sizeof(long long));
 }
 
-extern "C" long long ggbk_render_deposit(ggbk_render *r, int np, const double *prev, const double *pos,
-					 const float *col, float invg)
+// Host side of one deposit.  Returns the number of (particle, tile) entries, or -1.
+// The only synchronisation is the read-back of the entry total after the scan (the list
+// grows when the total exceeds its capacity).  `out` receives the particles in the next
+// storage order; the caller adopts it once this returns.
+extern "C" long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_parts out, float invg)
 {
-	if (np <= 0)
+	if (in.n <= 0)
 		return 0;
-	if (r->tile > TILE_MAX || r->tile < 1)
+	if (r->tile != TILE || out.n != in.n)
 		return -1;
 	SplatGeo sg;
 	sg.R = r->R;
+	sg.ntx = r->ntx;
 	sg.rad = r->radius;
 	sg.spacing = 0.7 / r->R;
-	unsigned pb = ggbk_blocks(np, PBS);
-	if (ggbk_zero(r->tile_count, sizeof(int) * (size_t) r->ntiles))
-		return -1;
+	unsigned pb = ggbk_blocks(in.n, PBS);
 	int pr_bin = ggbk_probe_begin(GGBK_PROBE_RENDER_BIN);
-	GGBK_LAUNCH(k_render_count, pb, PBS, 0, np, prev, pos, sg, r->tile, r->ntx, r->tile_count);
-	GGBK_LAUNCH(k_render_scan, 1, SCAN_BS, 0, r->ntiles, (const int *) r->tile_count, r->tile_start,
-		    r->tile_cursor, r->total);
+	if (ggbk_zero(r->tile_count, sizeof(int) * (size_t) r->ntiles) ||
+	    ggbk_zero(r->home_count, sizeof(int) * (size_t) r->ntiles))
+		return -1;
+	GGBK_LAUNCH(k_render_count, pb, PBS, 0, in, sg, r->tile_count, r->home_count);
+	unsigned sb = ggbk_blocks(r->ntiles, SCAN_BS);
+	GGBK_LAUNCH(k_render_scan_local, sb, SCAN_BS, 0, r->ntiles, (const int *) r->tile_count, r->tile_start,
+		    r->tile_cursor, (const int *) r->home_count, r->home_start, r->home_cursor, r->scan_part);
+	GGBK_LAUNCH(k_render_scan_fix, sb, SCAN_BS, 0, r->ntiles, r->tile_start, r->home_start,
+		    (const long long *) r->scan_part, r->total);
 	ggbk_probe_end(GGBK_PROBE_RENDER_BIN);
 	long long stats[2] = { 0, 0 };
 	if (ggbk_d2h(stats, r->total, sizeof(stats)))
 		return -1;
-	if (stats[0] > r->cap || stats[0] > INT_MAX / 2) {
-		if (stats[0] > INT_MAX / 2)
-			return -1;
+	if (stats[0] > INT_MAX / 2)
+		return -1;
+	if (stats[0] > r->cap) {
 		long long ncap = stats[0] + stats[0] / 4 + 1024;
 		ggbk_free(r->list);
 		ggbk_free(r->sort_scratch);
 		r->list = (int *) ggbk_malloc(sizeof(int) * (size_t) ncap);
-		r->sort_scratch = (int *) ggbk_malloc(sizeof(int) * 2 * (size_t) ncap);
+		r->sort_scratch = (unsigned long long *) ggbk_malloc(sizeof(unsigned long long) * 2 * (size_t) ncap);
 		if (!r->list || !r->sort_scratch)
 			return -1;
 		r->cap = ncap;
@@ -564,12 +747,11 @@ extern "C" long long ggbk_render_deposit(ggbk_render *r, int np, const double *p
 	}
 	if (pr_bin)
 		ggbk_probe_begin(GGBK_PROBE_RENDER_BIN);
-	GGBK_LAUNCH(k_render_fill, pb, PBS, 0, np, prev, pos, sg, r->tile, r->ntx, (const int *) r->tile_start,
-		    r->tile_cursor, r->list, r->cap);
+	GGBK_LAUNCH(k_render_fill, pb, PBS, 0, in, out, sg, (const int *) r->tile_start, r->tile_cursor,
+		    (const int *) r->home_start, r->home_cursor, r->list, r->cap);
 	ggbk_probe_end(GGBK_PROBE_RENDER_BIN);
 	ggbk_probe_begin(GGBK_PROBE_RENDER_DRAW);
-	GGBK_LAUNCH(k_render_draw, (unsigned) r->ntiles, DBS, 0, *r, prev, pos, col, sg, invg,
-		    (const long long *) r->total);
+	GGBK_LAUNCH(k_render_draw, (unsigned) r->ntiles, DBS, 0, *r, in, sg, invg);
 	ggbk_probe_end(GGBK_PROBE_RENDER_DRAW);
 	if (ggbk_check("render_deposit"))
 		return -1;
