@@ -156,6 +156,44 @@ def cpu_baseline(cfg, steps):
     return res
 
 
+def adjoint_leg(N=128, steps=50, K=10, seed=1):
+    """Discrete adjoint at N=128 over 50 FP64 steps with checkpoints every 10: timed
+    forward and backward, and one central-difference check of the gradient."""
+    g = ggb.Grid(N)
+    vel0, vband = g.initial_velocity(seed=seed, octaves=3)
+    om0 = g.curl_f64(vel0.astype(np.float64))
+    omb = g.curl_f64(vband.astype(np.float64))
+    omb -= np.sum(g.area * omb) / g.total_area
+    F = ggb.Fluid(g, forcing=0.5, rotation=10.0, cg_tol=1e-10, cg_maxit=2000, precision=ggb.PREC_FP64)
+    F.set(ggb.FIELD_OMEGA_BAND, omb)
+    F.set_state(vel0, vband)
+    dts = np.full(steps, g.h / max(F.max_speed(), 1e-6))
+    rng = np.random.default_rng(seed)
+    target = 0.5 * om0[rng.permutation(g.n)]
+    A = ggb.FluidAdjoint(F, K)
+    area = g.area.astype(np.float64)
+
+    def loss(w):
+        return 0.5 * float(np.sum(area * (w - target) ** 2))
+
+    A.forward(om0, dts)                     # warm-up (allocations, first solves)
+    t0 = time.perf_counter()
+    wT = A.forward(om0, dts)
+    t1 = time.perf_counter()
+    g0, gk = A.backward(area * (wT - target))
+    t2 = time.perf_counter()
+    d = rng.standard_normal(g.n)
+    eps = 1e-6
+    fd = (loss(A.forward(om0 + eps * d, dts)) - loss(A.forward(om0 - eps * d, dts))) / (2 * eps)
+    ad = float(g0 @ d)
+    A.free()
+    F.free()
+    return {"workload": "N=%d (%d cells), %d FP64 steps, checkpoint every %d, CG tol 1e-10" % (N, g.n, steps, K),
+            "forward_ms": round((t1 - t0) * 1000, 2), "backward_ms": round((t2 - t1) * 1000, 2),
+            "backward_over_forward": round((t2 - t1) / (t1 - t0), 2),
+            "fd_directional_rel_err": abs(fd - ad) / abs(ad), "dL_dforcing": gk}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +203,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain copy kernel of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -298,6 +337,11 @@ def main():
         out["fp64_true_relres"] = F2.true_relres()
         out["mixed_over_fp64"] = round(fluid_sps / fp64_sps, 3)
         F2.free()
+
+    if not args.no_adjoint:
+        t0 = time.perf_counter()
+        out["adjoint"] = adjoint_leg()
+        log("adjoint leg %.1fs" % (time.perf_counter() - t0))
 
     if not args.no_cpu:
         t0 = time.perf_counter()

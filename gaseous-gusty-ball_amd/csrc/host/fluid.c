@@ -761,20 +761,20 @@ static void precondition(struct ggb_fluid_dev *D, const void *r, int first)
 /* One PCG iteration after the first preconditioning (fluid.c:318-350): Ap and pAp, alpha,
  * then the update and the preconditioner.  With a multigrid hierarchy the update is fused
  * into the V-cycle's first pass and r alternates between two arrays. */
-static void cg_iteration(struct ggb_fluid_dev *D, int k)
+static void cg_iteration(struct ggb_fluid_dev *D, void *psi, int k)
 {
 	ggbk_grid g = D->g0;
 	void *pin = D->pbuf[k & 1], *pout = D->pbuf[(k + 1) & 1];
 	void *rin = D->rbuf[k & 1], *rout = D->rbuf[(k + 1) & 1];
 	ggbk_cg_Ap(g, D->z, D->TZ, pin, pout, D->Ap, D->T, k == 0, D->s, D->red);
 	ggbk_fin(GGBK_FIN_PAP, D->s, D->red, D->nb0, 0, 0, 0);
-	ggbk_cg_fuse u = { D->psi, rin, pout, D->Ap };
+	ggbk_cg_fuse u = { psi, rin, pout, D->Ap };
 	if (ggbk_vcycle_update(D->levels, D->nlev, &u, rout, D->T, D->z, D->TM, D->s, D->red) == 0) {
 		ggbk_fin(GGBK_FIN_RZ, D->s, D->red, D->nb0, 0, 0, 0);
 		return;
 	}
 	/* no hierarchy to fuse into: separate update in place */
-	ggbk_cg_update(g, D->psi, rin, pout, D->Ap, D->T, D->s, D->red);
+	ggbk_cg_update(g, psi, rin, pout, D->Ap, D->T, D->s, D->red);
 	ggbk_fin(GGBK_FIN_RR, D->s, D->red, D->nb0, 0, 0, 0);
 	ggbk_d2d(rout, rin, D->tsz * (size_t) D->n);
 	precondition(D, rout, 0);
@@ -786,13 +786,14 @@ static void read_scalars(struct ggb_fluid_dev *D)
 		ggb_die("device error: %s", ggbk_last_error());
 }
 
-/* PCG on the device state (reference fluid.c:288-358).  With sub_mean the pending mean
- * of omega (in s->mean) is subtracted by the first pass.  psi's mean is NOT removed. */
-static int solve_device(Fluid *F, int maxit, int sub_mean)
+/* PCG for K psi = -area * omega on device arrays (reference fluid.c:288-358), psi
+ * warm-started from its contents.  With sub_mean the pending mean of omega (in s->mean)
+ * is subtracted by the first pass.  psi's mean is NOT removed. */
+static int solve_on(Fluid *F, void *omega, void *psi, int maxit, int sub_mean)
 {
 	struct ggb_fluid_dev *D = F->dev;
 	ggbk_grid g = D->g0;
-	ggbk_cg_start(g, D->omega, D->psi, D->rbuf[0], D->T, sub_mean, D->s, D->red);
+	ggbk_cg_start(g, omega, psi, D->rbuf[0], D->T, sub_mean, D->s, D->red);
 	ggbk_fin(GGBK_FIN_START, D->s, D->red, D->nb0, F->prm.cg_tol, maxit, 0);
 	precondition(D, D->rbuf[0], 1);
 	/* Iterations are enqueued in batches and the host reads the loop flag after each; the
@@ -802,17 +803,22 @@ static int solve_device(Fluid *F, int maxit, int sub_mean)
 	int k = 0;
 	for (;;) {
 		for (int b = 0; b < batch && k < maxit; b++, k++)
-			cg_iteration(D, k);
+			cg_iteration(D, psi, k);
 		read_scalars(D);
 		if (!D->hs->active || k >= maxit)
 			break;
 		batch = batch < 64 ? 2 * batch : 64;
 	}
-	ggbk_cg_zero_if_bb0(g, D->psi, D->T, D->s);
+	ggbk_cg_zero_if_bb0(g, psi, D->T, D->s);
 	F->cg_iters = D->hs->it;
 	F->cg_relres = D->hs->bb == 0.0 ? 0.0 : sqrt(D->hs->rr / D->hs->bb);
 	D->predict = D->hs->it > 0 ? D->hs->it : 1;
 	return D->hs->it;
+}
+
+static int solve_device(Fluid *F, int maxit, int sub_mean)
+{
+	return solve_on(F, F->dev->omega, F->dev->psi, maxit, sub_mean);
 }
 
 /* psi -= mean(psi) fused into the velocity pass (fluid.c:354 then :361-376) */
@@ -1064,4 +1070,215 @@ void fluid_remove_mean(const CubeSphere *cs, float *f)
 	ggbk_free(d);
 	ggbk_free(s);
 	ggbk_free(red.part);
+}
+
+/* ---------------- discrete adjoint ----------------
+ *
+ * Gradient of a loss L(omega_T) with respect to the initial vorticity and the forcing
+ * strength for a rollout of fluid_step with a prescribed dt schedule.  The forward pass
+ * keeps (omega, psi) every K steps; the backward pass recomputes each K-step segment
+ * from its checkpoint, keeping (omega_s, psi_s) of every step of the segment, then walks
+ * the segment backwards through the adjoint kernels.  The Poisson solve is treated as
+ * exact, so its adjoint is one solve with the same MG-PCG (K is symmetric).
+ */
+struct FluidAdjoint {
+	Fluid *F;
+	int K, nsteps, nck_cap;
+	double *dts;           /* host [nsteps] */
+	double *ckpt;          /* device [nck_cap][2][n]: omega, psi at steps 0, K, 2K, ... */
+	double *tape;          /* device [K][2][n] */
+	double *lam, *lq, *lq1, *lq2, *ldb, *ldf, *lvel, *lpsi, *om, *z, *dk;
+	unsigned char *rslot;  /* device [maxnb][n]: slot of cell j in the list of its m-th neighbour */
+	double sum_area;
+};
+
+FluidAdjoint *fluid_adjoint_new(Fluid *F, int checkpoint_every)
+{
+	struct ggb_fluid_dev *D = F->dev;
+	if (!D || D->T != GGBK_F64 || checkpoint_every < 1) {
+		fprintf(stderr, "fluid_adjoint_new: needs an FP64-state fluid and K >= 1\n");
+		return NULL;
+	}
+	const CubeSphere *cs = F->cs;
+	size_t n = (size_t) D->n;
+	int maxnb = D->g0.maxnb;
+	unsigned char *rs = calloc((size_t) maxnb * n, 1);
+	if (!rs)
+		return NULL;
+	for (size_t j = 0; j < n; j++)
+		for (int m = 0; m < cs->nnb[j]; m++) {
+			int i = cs->nbr[CS_MAXNB * j + m], k;
+			for (k = 0; k < cs->nnb[i]; k++)
+				if (cs->nbr[CS_MAXNB * (size_t) i + k] == (int) j)
+					break;
+			if (k == cs->nnb[i]) {
+				fprintf(stderr, "fluid_adjoint_new: stencil not reciprocal at cells %zu, %d\n", j, i);
+				free(rs);
+				return NULL;
+			}
+			rs[(size_t) m * n + j] = (unsigned char) k;
+		}
+	FluidAdjoint *A = calloc(1, sizeof(*A));
+	if (!A) {
+		free(rs);
+		return NULL;
+	}
+	A->F = F;
+	A->K = checkpoint_every;
+	for (size_t i = 0; i < n; i++)
+		A->sum_area += cs->area[i];
+	A->rslot = ggb_dmalloc((size_t) maxnb * n, "adjoint");
+	upload(A->rslot, rs, (size_t) maxnb * n);
+	free(rs);
+	A->tape = ggb_dmalloc(sizeof(double) * 2 * n * (size_t) A->K, "adjoint tape");
+	double **v1[] = { &A->lam, &A->lq, &A->lq1, &A->lq2, &A->lpsi, &A->om, &A->z };
+	for (size_t k = 0; k < sizeof(v1) / sizeof(v1[0]); k++)
+		*v1[k] = ggb_dmalloc(sizeof(double) * n, "adjoint");
+	double **v3[] = { &A->ldb, &A->ldf, &A->lvel };
+	for (size_t k = 0; k < sizeof(v3) / sizeof(v3[0]); k++)
+		*v3[k] = ggb_dmalloc(sizeof(double) * 3 * n, "adjoint");
+	A->dk = ggb_dmalloc(sizeof(double), "adjoint");
+	return A;
+}
+
+void fluid_adjoint_free(FluidAdjoint *A)
+{
+	if (!A)
+		return;
+	ggb_sync();
+	void *bufs[] = { A->ckpt, A->tape, A->lam, A->lq, A->lq1, A->lq2, A->ldb, A->ldf,
+			 A->lvel, A->lpsi, A->om, A->z, A->dk, A->rslot };
+	for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); k++)
+		ggbk_free(bufs[k]);
+	free(A->dts);
+	free(A);
+}
+
+static void adj_save(struct ggb_fluid_dev *D, double *dst)
+{
+	size_t n = (size_t) D->n;
+	ggbk_d2d(dst, D->omega, 8 * n);
+	ggbk_d2d(dst + n, D->psi, 8 * n);
+}
+
+int fluid_adjoint_forward(FluidAdjoint *A, const double *omega0, const double *dts, int nsteps, double *omega_T)
+{
+	Fluid *F = A->F;
+	struct ggb_fluid_dev *D = F->dev;
+	size_t n = (size_t) D->n;
+	if (nsteps < 0)
+		return -1;
+	int nck = (nsteps + A->K - 1) / A->K;
+	if (nck > A->nck_cap) {
+		ggb_sync();
+		ggbk_free(A->ckpt);
+		A->ckpt = ggb_dmalloc(sizeof(double) * 2 * n * (size_t) nck, "adjoint checkpoints");
+		A->nck_cap = nck;
+	}
+	free(A->dts);
+	A->dts = malloc(sizeof(double) * (size_t) (nsteps ? nsteps : 1));
+	if (!A->dts)
+		return -1;
+	memcpy(A->dts, dts, sizeof(double) * (size_t) nsteps);
+	A->nsteps = nsteps;
+	/* set-up as fluid_set_state: omega - mean, cold solve, psi mean, velocity */
+	upload(D->omega, omega0, 8 * n);
+	ggbk_remove_mean(D->g0, D->omega, D->T, D->s, D->red);
+	ggbk_zero(D->psi, 8 * n);
+	int save = F->prm.cg_maxit;
+	int cold = save * 20 > 20000 ? save * 20 : 20000;
+	D->predict = 1;
+	solve_device(F, cold, 0);
+	D->predict = 1;
+	psi_mean_and_velocity(F);
+	for (int s = 0; s < nsteps; s++) {
+		if (s % A->K == 0)
+			adj_save(D, A->ckpt + 2 * n * (size_t) (s / A->K));
+		fluid_step(F, dts[s]);
+	}
+	if (omega_T)
+		download(omega_T, D->omega, 8 * n);
+	return ggbk_check("fluid_adjoint_forward") ? -1 : 0;
+}
+
+/* one backward step: lam = dL/d omega_{s+1} in, dL/d omega_s out; dL/dk accumulated */
+static void adj_step(FluidAdjoint *A, double dt, const double *om_s, const double *psi_s)
+{
+	Fluid *F = A->F;
+	struct ggb_fluid_dev *D = F->dev;
+	ggbk_grid g = D->g0;
+	size_t n = (size_t) D->n;
+	int bfecc = F->prm.bfecc;
+	double c = 2.0 * F->prm.rotation;
+	double relax = F->prm.forcing > 0 ? 1.0 - exp(-F->prm.forcing * dt) : 0.0;
+	double drelax = F->prm.forcing > 0 ? dt * exp(-F->prm.forcing * dt) : 0.0;
+	/* forward intermediates of step s */
+	ggbk_velocity(g, psi_s, NULL, 0, D->vel, D->T, D->s, D->red);
+	ggbk_departures(g, D->vel, om_s, D->q0, D->dep_b, D->dep_f, D->T, dt, c, bfecc);
+	ggbk_advect(g, D->q0, D->dep_b, D->q1, D->T);
+	if (bfecc)
+		ggbk_advect_bfecc(g, D->q1, D->q0, D->dep_f, D->q2, D->T);
+	/* mean removal, relaxation, limiter */
+	ggbk_adj_sum(g, A->lam, D->red);
+	ggbk_fin(GGBK_FIN_MEAN_F64, D->s, D->red, D->nb0, 0, 0, g.total_area);
+	ggbk_zero(A->lq, 8 * n);
+	ggbk_zero(A->lq1, 8 * n);
+	ggbk_zero(A->lq2, 8 * n);
+	ggbk_zero(A->lvel, 24 * n);
+	ggbk_adj_limiter(g, D->q0, D->q2, D->dep_b, D->omega_band, A->lam, D->s, relax, drelax, c, bfecc, A->lq,
+			 A->lq2, A->ldb, D->red);
+	ggbk_fin(GGBK_FIN_MEAN_F64, D->s, D->red, D->nb0, 0, 0, g.total_area);
+	ggbk_adj_accum(D->s, A->dk);
+	/* BFECC advections */
+	if (bfecc) {
+		ggbk_adj_bfecc_f(g, A->lq2, D->q1, D->dep_f, A->lq, A->lq1, A->ldf);
+		ggbk_adj_advect_b(g, A->lq1, D->q0, D->dep_b, A->lq, A->ldb);
+	}
+	/* departure points, velocity, Poisson */
+	ggbk_adj_departures(g, D->vel, A->ldb, A->ldf, dt, bfecc, A->lvel);
+	ggbk_adj_velocity(g, A->lvel, A->rslot, A->lpsi, D->red);
+	ggbk_fin(GGBK_FIN_MEAN_F64, D->s, D->red, D->nb0, 0, 0, g.total_area);
+	ggbk_adj_rhs(g, A->lpsi, D->s, A->sum_area, A->om);
+	solve_on(F, A->om, A->z, F->prm.cg_maxit, 0);
+	ggbk_adj_combine(g, A->lq, A->z, A->lam);
+}
+
+int fluid_adjoint_backward(FluidAdjoint *A, const double *g_omega_T, double *g_omega0, double *g_forcing)
+{
+	Fluid *F = A->F;
+	struct ggb_fluid_dev *D = F->dev;
+	ggbk_grid g = D->g0;
+	size_t n = (size_t) D->n;
+	if (!A->dts)
+		return -1;
+	upload(A->lam, g_omega_T, 8 * n);
+	ggbk_zero(A->dk, sizeof(double));
+	ggbk_zero(A->z, 8 * n);
+	int nseg = (A->nsteps + A->K - 1) / A->K;
+	for (int seg = nseg - 1; seg >= 0; seg--) {
+		int s0 = seg * A->K, s1 = s0 + A->K < A->nsteps ? s0 + A->K : A->nsteps;
+		/* recompute the segment from its checkpoint, keeping every step's state */
+		const double *ck = A->ckpt + 2 * n * (size_t) seg;
+		ggbk_d2d(D->omega, ck, 8 * n);
+		ggbk_d2d(D->psi, ck + n, 8 * n);
+		velocity_device(F);
+		for (int s = s0; s < s1; s++) {
+			adj_save(D, A->tape + 2 * n * (size_t) (s - s0));
+			if (s + 1 < s1)
+				fluid_step(F, A->dts[s]);
+		}
+		for (int s = s1 - 1; s >= s0; s--) {
+			const double *t = A->tape + 2 * n * (size_t) (s - s0);
+			adj_step(A, A->dts[s], t, t + n);
+		}
+	}
+	/* the control passes through set-up's mean removal */
+	ggbk_adj_sum(g, A->lam, D->red);
+	ggbk_fin(GGBK_FIN_MEAN_F64, D->s, D->red, D->nb0, 0, 0, g.total_area);
+	ggbk_adj_project(g, A->lam, D->s, A->lq);
+	if (g_omega0)
+		download(g_omega0, A->lq, 8 * n);
+	if (g_forcing)
+		download(g_forcing, A->dk, sizeof(double));
+	return ggbk_check("fluid_adjoint_backward") ? -1 : 0;
 }

@@ -1,0 +1,374 @@
+// adj_k.hip -- discrete adjoint of the fluid step (FP64 state), for the gradient of a
+// loss of the final vorticity with respect to the initial vorticity and the forcing
+// strength.  Each kernel is the transpose of one forward pass (reference src/fluid.c):
+//
+//   relaxation + limiter (fluid.c:449-474)  k_adj_limiter    (limiter active set re-evaluated)
+//   BFECC source and forward advect          k_adj_bfecc_f
+//   backward advect (fluid.c:416-422)        k_adj_advect_b
+//   departure points (fluid.c:397-414)       k_adj_departures
+//   velocity u = p x grad psi (:360-376)     k_adj_velocity   (gather over reverse slots)
+//   Poisson solve                            k_adj_rhs + the forward MG-PCG + k_adj_combine
+//
+// The transpose of an interpolation scatters w_k * lambda to the four taps with FP64
+// atomics; the derivative of the bilinear weights with respect to the sample point
+// gives the departure-point adjoint.  Great-circle moves are differentiated in closed
+// form (Rodrigues step, then normalisation).  Forward intermediates are recomputed by
+// the forward kernels from the taped (omega, psi) of the step.
+#include <hip/hip_runtime.h>
+
+#include "dev.h"
+#include "ggbk.h"
+
+#define BS 256
+
+__device__ __forceinline__ void store_sum(ggbk_red red, double v)
+{
+	double b = block_sum<BS>(v);
+	if (threadIdx.x == 0)
+		red.part[(size_t) GGBK_SL_MEAN * red.stride + xblock()] = b;
+}
+
+__device__ __forceinline__ void face_basis(int f, dv3 &C, dv3 &R, dv3 &U)
+{
+	switch (f) {
+	case 0: C = d3(0, 0, 1); R = d3(1, 0, 0); U = d3(0, 1, 0); break;
+	case 1: C = d3(1, 0, 0); R = d3(0, 0, -1); U = d3(0, 1, 0); break;
+	case 2: C = d3(0, 0, -1); R = d3(-1, 0, 0); U = d3(0, 1, 0); break;
+	case 3: C = d3(-1, 0, 0); R = d3(0, 0, 1); U = d3(0, 1, 0); break;
+	case 4: C = d3(0, 1, 0); R = d3(1, 0, 0); U = d3(0, 0, -1); break;
+	default: C = d3(0, -1, 0); R = d3(1, 0, 0); U = d3(0, 0, 1); break;
+	}
+}
+
+// Taps at p, their bilinear weights, and the gradients of the weights with respect to
+// the 3-vector p (taps fixed): x = (atan(u) + pi/4)/h - 1/2 with u = p.R / p.C gives
+// dx/dp = (R - u C) / (p.C h (1 + u^2)), likewise y.
+__device__ __forceinline__ void taps_grad(const ggbk_grid &g, dv3 p, Taps &t, double wt[4], dv3 dw[4])
+{
+	t = taps_at(g, p);
+	tap_weights(t, wt);
+	int f = face_of(p);
+	dv3 C, R, U;
+	face_basis(f, C, R, U);
+	double dc, u, v;
+	gnomonic(f, p, dc, u, v);
+	dv3 gx = vscl(vsub(R, vscl(C, u)), 1.0 / (dc * g.h * (1 + u * u)));
+	dv3 gy = vscl(vsub(U, vscl(C, v)), 1.0 / (dc * g.h * (1 + v * v)));
+	double fx = t.fx, fy = t.fy;
+	dw[0] = vadd(vscl(gx, -(1 - fy)), vscl(gy, -(1 - fx)));
+	dw[1] = vadd(vscl(gx, 1 - fy), vscl(gy, -fx));
+	dw[2] = vadd(vscl(gx, -fy), vscl(gy, 1 - fx));
+	dw[3] = vadd(vscl(gx, fy), vscl(gy, fx));
+}
+
+// lambda_v of out = gc_move(p, v, s), p fixed (vec3.h:33-42 differentiated).
+__device__ __forceinline__ dv3 gc_move_adj(dv3 p, dv3 v, double s, dv3 lo)
+{
+	dv3 t = vtan(v, p);
+	double sp = vlen(t);
+	if (sp < 1e-300)
+		return d3(0, 0, 0);
+	double th = sp * s, sn, cs;
+	sincos(th, &sn, &cs);
+	dv3 r = vadd(vscl(p, cs), vscl(t, sn / sp));
+	double l = vlen(r);
+	dv3 out = vscl(r, 1.0 / l);
+	dv3 lr = vscl(vsub(lo, vscl(out, vdot(out, lo))), 1.0 / l);
+	double tl = vdot(t, lr);
+	dv3 lt = vscl(lr, sn / sp);
+	double lth = -sn * vdot(p, lr) + cs / sp * tl;
+	double lsp = -sn / (sp * sp) * tl + s * lth;
+	lt = vadd(lt, vscl(t, lsp / sp));
+	return vtan(lt, p);
+}
+
+__device__ __forceinline__ void atomic_add3(double *a, dv3 v)
+{
+	atomicAdd(a, v.x);
+	atomicAdd(a + 1, v.y);
+	atomicAdd(a + 2, v.z);
+}
+
+// Sum of x (slot MEAN partials; fold with FIN_MEAN_F64: s->sum, s->mean = sum / area).
+__global__ __launch_bounds__(BS) void k_adj_sum(int n, const double *x, ggbk_red red)
+{
+	int i = xblock() * BS + threadIdx.x;
+	store_sum(red, i < n ? x[i] : 0.0);
+}
+
+// Relaxation toward the band (fluid.c:466-474) after the mean removal (:475), and the
+// limited final advection (:449-464).  lam = dL/d omega_{n+1}; s->mean = sum(lam) / area.
+// Out: lq (scatter, atomics), lq2 (scatter, atomics), ldb (store), dL/dk partial.
+__global__ __launch_bounds__(BS) void k_adj_limiter(ggbk_grid g, const double *q, const double *q2,
+						     const double *dep_b, const double *band, const double *lam,
+						     const ggbk_scalars *s, double relax, double drelax, double c,
+						     int bfecc, double *lq, double *lq2, double *ldb, ggbk_red red)
+{
+	int i = xblock() * BS + threadIdx.x;
+	double dk = 0;
+	if (i < g.n) {
+		double lpp = lam[i] - (double) g.area[i] * s->mean;
+		Taps t;
+		double wt[4];
+		dv3 dw[4];
+		taps_grad(g, ld3(dep_b + 3 * (size_t) i), t, wt, dw);
+		const double *src = bfecc ? q2 : q;
+		double v = 0, lo = 1e30, hi = -1e30;
+		int klo = 0, khi = 0;
+#pragma unroll
+		for (int k = 0; k < 4; k++) {
+			v += wt[k] * src[t.idx[k]];
+			double o = q[t.idx[k]];
+			if (o < lo) {
+				lo = o;
+				klo = k;
+			}
+			if (o > hi) {
+				hi = o;
+				khi = k;
+			}
+		}
+		int state = 0;
+		double qn = v;
+		if (bfecc) {
+			if (v < lo) {
+				qn = lo;
+				state = 1;
+			}
+			if (qn > hi) {
+				qn = hi;
+				state = 2;
+			}
+		}
+		double w = qn - c * (double) g.pos[3 * (size_t) i + 1];
+		dk = lpp * (band[i] - w) * drelax;
+		double lqn = (1 - relax) * lpp;
+		dv3 d = d3(0, 0, 0);
+		if (state == 1) {
+			atomicAdd(&lq[t.idx[klo]], lqn);
+		} else if (state == 2) {
+			atomicAdd(&lq[t.idx[khi]], lqn);
+		} else {
+			double *lt = bfecc ? lq2 : lq;
+#pragma unroll
+			for (int k = 0; k < 4; k++) {
+				atomicAdd(&lt[t.idx[k]], wt[k] * lqn);
+				d = vadd(d, vscl(dw[k], src[t.idx[k]] * lqn));
+			}
+		}
+		st3(ldb + 3 * (size_t) i, d);
+	}
+	store_sum(red, dk);
+}
+
+// q2 = q + 0.5 (q - A_f q1) (fluid.c:446-448): lq += 1.5 lq2; then the transpose of
+// A_f applied to -0.5 lq2 scatters into lq1, and its weight derivative gives ldf.
+__global__ __launch_bounds__(BS) void k_adj_bfecc_f(ggbk_grid g, const double *lq2, const double *q1,
+						     const double *dep_f, double *lq, double *lq1, double *ldf)
+{
+	int i = xblock() * BS + threadIdx.x;
+	if (i >= g.n)
+		return;
+	double l2 = lq2[i];
+	lq[i] += 1.5 * l2;
+	double l = -0.5 * l2;
+	Taps t;
+	double wt[4];
+	dv3 dw[4];
+	taps_grad(g, ld3(dep_f + 3 * (size_t) i), t, wt, dw);
+	dv3 d = d3(0, 0, 0);
+#pragma unroll
+	for (int k = 0; k < 4; k++) {
+		atomicAdd(&lq1[t.idx[k]], wt[k] * l);
+		d = vadd(d, vscl(dw[k], q1[t.idx[k]] * l));
+	}
+	st3(ldf + 3 * (size_t) i, d);
+}
+
+// q1 = A_b q: transpose into lq, weight derivative added to ldb.
+__global__ __launch_bounds__(BS) void k_adj_advect_b(ggbk_grid g, const double *lq1, const double *q,
+						      const double *dep_b, double *lq, double *ldb)
+{
+	int i = xblock() * BS + threadIdx.x;
+	if (i >= g.n)
+		return;
+	double l = lq1[i];
+	Taps t;
+	double wt[4];
+	dv3 dw[4];
+	taps_grad(g, ld3(dep_b + 3 * (size_t) i), t, wt, dw);
+	dv3 d = ld3(ldb + 3 * (size_t) i);
+#pragma unroll
+	for (int k = 0; k < 4; k++) {
+		atomicAdd(&lq[t.idx[k]], wt[k] * l);
+		d = vadd(d, vscl(dw[k], q[t.idx[k]] * l));
+	}
+	st3(ldb + 3 * (size_t) i, d);
+}
+
+// One departure point of cell i with steps (s1, s2) (fluid.c:404-412) differentiated
+// with respect to the velocity field: pm = move(p, u_i, s1); r = sum w_k(pm) u[tap_k];
+// um = tan_p(tan_pm(r)); dep = move(p, um, s2).
+__device__ __forceinline__ void departure_adj(const ggbk_grid &g, const double *vel, int i, double s1, double s2,
+					      dv3 ldep, double *lvel)
+{
+	dv3 p = cell_pos(g, i), u = ld3(vel + 3 * (size_t) i);
+	dv3 pm = gc_move(p, u, s1);
+	Taps t;
+	double wt[4];
+	dv3 dw[4];
+	taps_grad(g, pm, t, wt, dw);
+	dv3 uk[4];
+	dv3 r = d3(0, 0, 0);
+#pragma unroll
+	for (int k = 0; k < 4; k++) {
+		uk[k] = ld3(vel + 3 * (size_t) t.idx[k]);
+		r = vadd(r, vscl(uk[k], wt[k]));
+	}
+	dv3 sv = vtan(r, pm);
+	dv3 um = vtan(sv, p);
+	dv3 lum = gc_move_adj(p, um, s2, ldep);
+	dv3 lsv = vtan(lum, p);
+	dv3 lr = vtan(lsv, pm);
+	dv3 lpm = vsub(vscl(lsv, -vdot(pm, r)), vscl(r, vdot(pm, lsv)));
+#pragma unroll
+	for (int k = 0; k < 4; k++) {
+		atomic_add3(lvel + 3 * (size_t) t.idx[k], vscl(lr, wt[k]));
+		lpm = vadd(lpm, vscl(dw[k], vdot(uk[k], lr)));
+	}
+	atomic_add3(lvel + 3 * (size_t) i, gc_move_adj(p, u, s1, lpm));
+}
+
+__global__ __launch_bounds__(BS) void k_adj_departures(ggbk_grid g, const double *vel, const double *ldb,
+							const double *ldf, double dt, int bfecc, double *lvel)
+{
+	int i = xblock() * BS + threadIdx.x;
+	if (i >= g.n)
+		return;
+	departure_adj(g, vel, i, -0.5 * dt, -dt, ld3(ldb + 3 * (size_t) i), lvel);
+	if (bfecc)
+		departure_adj(g, vel, i, 0.5 * dt, dt, ld3(ldf + 3 * (size_t) i), lvel);
+}
+
+// vel_i = p_i x sum_k gc_ik (psi_nbr(i,k) - psi_i): with c_ik = gc_ik . (lvel_i x p_i),
+// lpsi_j = sum over the cells i that list j (slot rslot) of c_i,slot - sum_k c_jk.  The
+// stencil is reciprocal, so the transpose is a gather; plus the sum partial of lpsi.
+__global__ __launch_bounds__(BS) void k_adj_velocity(ggbk_grid g, const double *lvel, const unsigned char *rslot,
+						      double *lpsi, ggbk_red red)
+{
+	int j = xblock() * BS + threadIdx.x;
+	double acc = 0;
+	if (j < g.n) {
+		size_t n = (size_t) g.n;
+		dv3 mj = vcross(ld3(lvel + 3 * (size_t) j), cell_pos(g, j));
+		int nn = g.nnb[j];
+		for (int m = 0; m < nn; m++) {
+			int i = g.nbr[m * n + j];
+			int k = rslot[m * n + j];
+			dv3 mi = vcross(ld3(lvel + 3 * (size_t) i), cell_pos(g, i));
+			dv3 gci = d3(g.gc[(k * 3 + 0) * n + i], g.gc[(k * 3 + 1) * n + i], g.gc[(k * 3 + 2) * n + i]);
+			dv3 gcj = d3(g.gc[(m * 3 + 0) * n + j], g.gc[(m * 3 + 1) * n + j], g.gc[(m * 3 + 2) * n + j]);
+			acc += vdot(gci, mi) - vdot(gcj, mj);
+		}
+		lpsi[j] = acc;
+	}
+	store_sum(red, acc);
+}
+
+// Right-hand side of the adjoint Poisson solve in the solver's omega form
+// (b = -area * omega): rhs = lpsi - area * S / total_area - S (1 - sum(area) / total_area) / n,
+// i.e. the transpose of psi's mean removal followed by the removal of the constant
+// component, which K^+ ignores.  S = s->sum.
+__global__ __launch_bounds__(BS) void k_adj_rhs(ggbk_grid g, const double *lpsi, const ggbk_scalars *s, double sum_area,
+						 double *om)
+{
+	int i = xblock() * BS + threadIdx.x;
+	if (i >= g.n)
+		return;
+	double S = s->sum;
+	double a = (double) g.area[i];
+	double rhs = lpsi[i] - a * (S / g.total_area) - S * (1.0 - sum_area / g.total_area) / g.n;
+	om[i] = -rhs / a;
+}
+
+// lam = lq - area * z  (lambda of omega_n: direct path plus the Poisson path)
+__global__ __launch_bounds__(BS) void k_adj_combine(ggbk_grid g, const double *lq, const double *z, double *lam)
+{
+	int i = xblock() * BS + threadIdx.x;
+	if (i < g.n)
+		lam[i] = lq[i] - (double) g.area[i] * z[i];
+}
+
+// out = lam - area * s->mean (transpose of the area-weighted mean removal)
+__global__ __launch_bounds__(BS) void k_adj_project(ggbk_grid g, const double *lam, const ggbk_scalars *s, double *out)
+{
+	int i = xblock() * BS + threadIdx.x;
+	if (i < g.n)
+		out[i] = lam[i] - (double) g.area[i] * s->mean;
+}
+
+__global__ void k_adj_accum(const ggbk_scalars *s, double *acc)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0)
+		acc[0] += s->sum;
+}
+
+#define NB(n) ggbk_blocks((n), BS)
+
+extern "C" void ggbk_adj_sum(ggbk_grid g, const double *x, ggbk_red red)
+{
+	GGBK_LAUNCH(k_adj_sum, NB(g.n), BS, 0, g.n, x, red);
+}
+
+extern "C" void ggbk_adj_limiter(ggbk_grid g, const double *q, const double *q2, const double *dep_b,
+				 const double *band, const double *lam, const ggbk_scalars *s, double relax,
+				 double drelax, double c, int bfecc, double *lq, double *lq2, double *ldb,
+				 ggbk_red red)
+{
+	GGBK_LAUNCH(k_adj_limiter, NB(g.n), BS, 0, g, q, q2, dep_b, band, lam, s, relax, drelax, c, bfecc, lq, lq2,
+		    ldb, red);
+}
+
+extern "C" void ggbk_adj_bfecc_f(ggbk_grid g, const double *lq2, const double *q1, const double *dep_f, double *lq,
+				 double *lq1, double *ldf)
+{
+	GGBK_LAUNCH(k_adj_bfecc_f, NB(g.n), BS, 0, g, lq2, q1, dep_f, lq, lq1, ldf);
+}
+
+extern "C" void ggbk_adj_advect_b(ggbk_grid g, const double *lq1, const double *q, const double *dep_b, double *lq,
+				  double *ldb)
+{
+	GGBK_LAUNCH(k_adj_advect_b, NB(g.n), BS, 0, g, lq1, q, dep_b, lq, ldb);
+}
+
+extern "C" void ggbk_adj_departures(ggbk_grid g, const double *vel, const double *ldb, const double *ldf, double dt,
+				    int bfecc, double *lvel)
+{
+	GGBK_LAUNCH(k_adj_departures, NB(g.n), BS, 0, g, vel, ldb, ldf, dt, bfecc, lvel);
+}
+
+extern "C" void ggbk_adj_velocity(ggbk_grid g, const double *lvel, const unsigned char *rslot, double *lpsi,
+				  ggbk_red red)
+{
+	GGBK_LAUNCH(k_adj_velocity, NB(g.n), BS, 0, g, lvel, rslot, lpsi, red);
+}
+
+extern "C" void ggbk_adj_rhs(ggbk_grid g, const double *lpsi, const ggbk_scalars *s, double sum_area, double *om)
+{
+	GGBK_LAUNCH(k_adj_rhs, NB(g.n), BS, 0, g, lpsi, s, sum_area, om);
+}
+
+extern "C" void ggbk_adj_combine(ggbk_grid g, const double *lq, const double *z, double *lam)
+{
+	GGBK_LAUNCH(k_adj_combine, NB(g.n), BS, 0, g, lq, z, lam);
+}
+
+extern "C" void ggbk_adj_project(ggbk_grid g, const double *lam, const ggbk_scalars *s, double *out)
+{
+	GGBK_LAUNCH(k_adj_project, NB(g.n), BS, 0, g, lam, s, out);
+}
+
+extern "C" void ggbk_adj_accum(const ggbk_scalars *s, double *acc)
+{
+	GGBK_LAUNCH(k_adj_accum, 1, 64, 0, s, acc);
+}

@@ -198,6 +198,24 @@ void ggbk_advect_bfecc(ggbk_grid g, const void *q1, const void *q, const void *d
 void ggbk_limited_relax(ggbk_grid g, const void *q2, const void *q, const void *dep_b, const void *q1, int bfecc,
 			const void *band, void *omega, int T, double twoOmega, double relax, ggbk_red red);
 
+/* ---- discrete adjoint of the fluid step (FP64 state; adj_k.hip) ---- */
+void ggbk_adj_sum(ggbk_grid g, const double *x, ggbk_red red);     /* slot MEAN partials of sum x */
+void ggbk_adj_limiter(ggbk_grid g, const double *q, const double *q2, const double *dep_b,
+		      const double *band, const double *lam, const ggbk_scalars *s, double relax,
+		      double drelax, double c, int bfecc, double *lq, double *lq2, double *ldb,
+		      ggbk_red red);
+void ggbk_adj_bfecc_f(ggbk_grid g, const double *lq2, const double *q1, const double *dep_f, double *lq,
+		      double *lq1, double *ldf);
+void ggbk_adj_advect_b(ggbk_grid g, const double *lq1, const double *q, const double *dep_b, double *lq,
+		       double *ldb);
+void ggbk_adj_departures(ggbk_grid g, const double *vel, const double *ldb, const double *ldf, double dt,
+			 int bfecc, double *lvel);
+void ggbk_adj_velocity(ggbk_grid g, const double *lvel, const unsigned char *rslot, double *lpsi, ggbk_red red);
+void ggbk_adj_rhs(ggbk_grid g, const double *lpsi, const ggbk_scalars *s, double sum_area, double *om);
+void ggbk_adj_combine(ggbk_grid g, const double *lq, const double *z, double *lam);
+void ggbk_adj_project(ggbk_grid g, const double *lam, const ggbk_scalars *s, double *out);
+void ggbk_adj_accum(const ggbk_scalars *s, double *acc);
+
 /* ---- particles (reference particles.c) ----
  * Particle arrays in storage order: slot s holds particle pid[s].  The renderer keeps
  * the slots in tile order (each deposit writes the next order into a second set). */

@@ -91,6 +91,7 @@ PUBLIC_SYMBOLS = [
     "fluid_vcycle_f64", "ggb_laplacian_f64", "ggb_applyK_f64", "ggb_curl_f64",
     "ggb_divergence_f64", "particles_sync_host", "particles_get_f64", "particles_set_f64",
     "particles_set_round32", "render_sync_host", "render_last_entries", "render_last_stats", "render_faces8",
+    "fluid_adjoint_new", "fluid_adjoint_free", "fluid_adjoint_forward", "fluid_adjoint_backward",
     # ggb_inputs.h
     "ggb_initial_velocity", "ggb_banded_strip", "ggb_rgba8_to_float",
 ]
@@ -153,6 +154,10 @@ def lib():
             "render_last_entries": (C.c_longlong, [P(CRender)]),
             "render_faces8": (None, [P(CRender), fp, P(C.c_ubyte)]),
             "render_last_stats": (C.c_int, [P(CRender), P(C.c_longlong)]),
+            "fluid_adjoint_new": (C.c_void_p, [P(CFluid), C.c_int]),
+            "fluid_adjoint_free": (None, [C.c_void_p]),
+            "fluid_adjoint_forward": (C.c_int, [C.c_void_p, dp, dp, C.c_int, dp]),
+            "fluid_adjoint_backward": (C.c_int, [C.c_void_p, dp, dp, dp]),
             "ggb_device_init": (C.c_int, []),
             "ggb_version": (C.c_char_p, []),
             "ggb_sync": (None, []),
@@ -361,11 +366,54 @@ class Fluid:
     def true_relres(self):
         return lib().fluid_true_relres(C.byref(self.c))
 
+    def set_forcing(self, k):
+        """Forcing strength used by later steps (F->prm.forcing)."""
+        self.c.prm.forcing = float(k)
+
     def vcycle(self, b):
         x = np.zeros(self.grid.n)
         if lib().fluid_vcycle_f64(C.byref(self.c), dptr(np.ascontiguousarray(b, np.float64)), dptr(x)):
             raise RuntimeError("vcycle failed")
         return x
+
+
+class FluidAdjoint:
+    """Checkpointed discrete adjoint of a rollout of fluid_step (fluid_adjoint_* ABI)."""
+
+    def __init__(self, fluid, checkpoint_every=10):
+        self.fluid = fluid
+        self.h = lib().fluid_adjoint_new(C.byref(fluid.c), int(checkpoint_every))
+        if not self.h:
+            raise RuntimeError("fluid_adjoint_new failed (needs FP64 state)")
+
+    def free(self):
+        if self.h:
+            lib().fluid_adjoint_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def forward(self, omega0, dts):
+        """omega_T after len(dts) steps from the control omega0."""
+        om = np.ascontiguousarray(omega0, np.float64).ravel()
+        d = np.ascontiguousarray(dts, np.float64).ravel()
+        out = np.zeros(self.fluid.grid.n)
+        if lib().fluid_adjoint_forward(self.h, dptr(om), dptr(d), len(d), dptr(out)) != 0:
+            raise RuntimeError("fluid_adjoint_forward failed")
+        return out
+
+    def backward(self, g_omega_T):
+        """(dL/d omega0, dL/d forcing) from dL/d omega_T, for the last forward."""
+        g = np.ascontiguousarray(g_omega_T, np.float64).ravel()
+        g0 = np.zeros(self.fluid.grid.n)
+        gk = C.c_double()
+        if lib().fluid_adjoint_backward(self.h, dptr(g), dptr(g0), C.byref(gk)) != 0:
+            raise RuntimeError("fluid_adjoint_backward failed")
+        return g0, gk.value
 
 
 class Particles:

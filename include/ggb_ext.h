@@ -62,12 +62,30 @@ void particles_set_round32(Particles *P, int on);
 /* ---- render ---- */
 void render_sync_host(Render *r);             /* device buffer -> r->buf mirror */
 long long render_last_entries(const Render *r); /* (particle, tile) entries of the last deposit */
-/* diagnostics of the last deposit (16 values): (particle, tile) entries, longest tile
- * list, windows, serial windows, draw-pass cycle sums (sort, samples, blend, total),
- * texels touched */
+/* diagnostics of the last deposit (16 values): [0] (particle, tile) entries, [1] longest
+ * tile list, [2] windows, [3] serial windows, [4] tiles sorted by particle index, [5..7]
+ * draw-pass cycle sums (samples, blend, total), [8] texels touched */
 int render_last_stats(const Render *r, long long *out16);
 /* 8-bit faces [6][R][R][4] (bg NULL: un-premultiplied RGBA) computed on the host mirror. */
 void render_faces8(Render *r, const float *bg, unsigned char *out);
+
+/* ---- discrete adjoint of fluid_step ----
+ * Gradient of a loss L(omega_T) of the final vorticity after nsteps steps of fluid_step
+ * with a prescribed dt schedule, with respect to the initial vorticity omega0 (taken
+ * through set-up's mean removal, then a cold solve) and to the forcing strength
+ * F->prm.forcing.  Needs an FP64-state fluid (GGB_PREC_FP64 or GGB_PREC_MIXED) and a
+ * reciprocal stencil.  Checkpoints (omega, psi) every checkpoint_every steps; the backward
+ * pass recomputes one segment at a time.  The Poisson solves (forward and adjoint) use
+ * F->prm.cg_tol; the gradient is exact for exact solves.  Both calls overwrite the
+ * fluid's state; after fluid_adjoint_forward it is the final state. */
+typedef struct FluidAdjoint FluidAdjoint;
+FluidAdjoint *fluid_adjoint_new(Fluid *F, int checkpoint_every);   /* NULL on error */
+void fluid_adjoint_free(FluidAdjoint *A);
+/* omega0 [n] in; omega_T [n] out (may be NULL); 0 on success */
+int fluid_adjoint_forward(FluidAdjoint *A, const double *omega0, const double *dts, int nsteps,
+			  double *omega_T);
+/* g_omega_T = dL/d omega_T [n] in; g_omega0 [n] and *g_forcing out; 0 on success */
+int fluid_adjoint_backward(FluidAdjoint *A, const double *g_omega_T, double *g_omega0, double *g_forcing);
 
 #ifdef __cplusplus
 }

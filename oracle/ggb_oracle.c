@@ -63,6 +63,7 @@ static inline v3d vtan(v3d v, v3d n) { return vsub(v, vscl(n, vdot(v, n))); }
 static inline double vangle(v3d a, v3d b) { return atan2(vlen(vcross(a, b)), vdot(a, b)); }
 static inline v3d ldf(const float *f) { return mk(f[0], f[1], f[2]); }
 static inline v3d ldr(const real *f) { return mk(f[0], f[1], f[2]); }
+static inline v3d ldd(const double *f) { return mk(f[0], f[1], f[2]); }
 static inline void str(real *f, v3d a) { f[0] = (real) a.x; f[1] = (real) a.y; f[2] = (real) a.z; }
 
 /* vec3.h:33-42: great-circle move of p along tangent velocity v for time s */
@@ -783,6 +784,336 @@ int orc_fluid_set(OFluid *F, int which, const double *in)
 		return -1;
 	for (size_t i = 0; i < len; i++)
 		p[i] = (real) in[i];
+	return 0;
+}
+
+/* ---------------- discrete adjoint of the fluid step ----------------
+ *
+ * Reverse-mode derivative of n steps of orc_fluid_step (this build's FP64 state) with a
+ * prescribed dt schedule, for the loss gradient with respect to the initial vorticity
+ * and to the forcing strength k (relax = 1 - exp(-k dt), fluid.c:466).  The forward
+ * rollout keeps (omega_n, psi_n) for every step (a full tape); the backward sweep
+ * recomputes each step's intermediates from them.  The Poisson solve is treated as exact
+ * (psi = Pi_A K^+ (-A omega) with Pi_A the area-weighted mean removal), so its adjoint
+ * is one more solve with the same PCG; the limiter's active set (fluid.c:455-462) is
+ * re-evaluated from the recomputed forward values, ties resolved to the first tap as the
+ * forward loop does.  The control omega0 enters through the mean removal of set-up.
+ */
+
+/* x(p), y(p) of locate() and their gradients with respect to the 3-vector p on face f */
+static void locate_grad(const OGrid *g, v3d p, int f, v3d *gx, v3d *gy)
+{
+	double dc = vdot(p, FC[f]);
+	double u = vdot(p, FR[f]) / dc, v = vdot(p, FU[f]) / dc;
+	v3d du = vscl(vsub(FR[f], vscl(FC[f], u)), 1.0 / dc);
+	v3d dv = vscl(vsub(FU[f], vscl(FC[f], v)), 1.0 / dc);
+	*gx = vscl(du, 1.0 / (g->h * (1 + u * u)));
+	*gy = vscl(dv, 1.0 / (g->h * (1 + v * v)));
+}
+
+/* taps() plus the gradients of the four bilinear weights with respect to p */
+static void taps_grad(const OGrid *g, v3d p, int idx[4], double wt[4], v3d dw[4])
+{
+	taps(g, p, idx, wt);
+	int f;
+	double x, y;
+	locate(g, p, &f, &x, &y);
+	double fx = x - floor(x), fy = y - floor(y);
+	v3d gx, gy;
+	locate_grad(g, p, f, &gx, &gy);
+	dw[0] = vadd(vscl(gx, -(1 - fy)), vscl(gy, -(1 - fx)));
+	dw[1] = vadd(vscl(gx, 1 - fy), vscl(gy, -fx));
+	dw[2] = vadd(vscl(gx, -fy), vscl(gy, 1 - fx));
+	dw[3] = vadd(vscl(gx, fy), vscl(gy, fx));
+}
+
+/* adjoint of out = gc_move(p, v, s) with respect to v (p fixed): returns lambda_v */
+static v3d gc_move_adj(v3d p, v3d v, double s, v3d lo)
+{
+	v3d t = vtan(v, p);
+	double sp = vlen(t);
+	if (sp < 1e-300)
+		return mk(0, 0, 0);
+	double th = sp * s, sn = sin(th), cs = cos(th);
+	v3d r = vadd(vscl(p, cs), vscl(t, sn / sp));
+	double l = vlen(r);
+	v3d out = vscl(r, 1.0 / l);
+	v3d lr = vscl(vsub(lo, vscl(out, vdot(out, lo))), 1.0 / l);
+	double tl = vdot(t, lr);
+	v3d lt = vscl(lr, sn / sp);
+	double lth = -sn * vdot(p, lr) + cs / sp * tl;
+	double lsp = -sn / (sp * sp) * tl + s * lth;
+	lt = vadd(lt, vscl(t, lsp / sp));
+	return vtan(lt, p);
+}
+
+/* adjoint of one departure point (fluid.c:404-412) with steps s1, s2: adds to lvel */
+static void departure_adj(const OGrid *g, const real *vel, int i, double s1, double s2, v3d ldep, double *lvel)
+{
+	v3d p = gpos(g, i), u = ldr(vel + 3 * i);
+	v3d pm = gc_move(p, u, s1);
+	int idx[4];
+	double wt[4];
+	v3d dw[4];
+	taps_grad(g, pm, idx, wt, dw);
+	v3d r = mk(0, 0, 0);
+	for (int k = 0; k < 4; k++)
+		r = vadd(r, vscl(ldr(vel + 3 * idx[k]), wt[k]));
+	v3d sv = vtan(r, pm);
+	v3d um = vtan(sv, p);
+	v3d lum = gc_move_adj(p, um, s2, ldep);
+	v3d lsv = vtan(lum, p);
+	v3d lr = vtan(lsv, pm);
+	v3d lpm = vsub(vscl(lsv, -vdot(pm, r)), vscl(r, vdot(pm, lsv)));
+	for (int k = 0; k < 4; k++) {
+		double *o = lvel + 3 * idx[k];
+		o[0] += wt[k] * lr.x;
+		o[1] += wt[k] * lr.y;
+		o[2] += wt[k] * lr.z;
+		lpm = vadd(lpm, vscl(dw[k], vdot(ldr(vel + 3 * idx[k]), lr)));
+	}
+	v3d lu = gc_move_adj(p, u, s1, lpm);
+	lvel[3 * i] += lu.x;
+	lvel[3 * i + 1] += lu.y;
+	lvel[3 * i + 2] += lu.z;
+}
+
+/* lsrc[taps] += w * ldst (transpose of advect) and ldep += sum dw * src[taps] * ldst */
+static void advect_adj(const OGrid *g, const real *src, const real *dep, const double *ldst, double *lsrc,
+		       double *ldep)
+{
+	int n = g->ncells;
+	for (int i = 0; i < n; i++) {
+		int idx[4];
+		double wt[4];
+		v3d dw[4];
+		taps_grad(g, ldr(dep + 3 * i), idx, wt, dw);
+		v3d d = mk(0, 0, 0);
+		for (int k = 0; k < 4; k++) {
+			lsrc[idx[k]] += wt[k] * ldst[i];
+			d = vadd(d, vscl(dw[k], (double) src[idx[k]] * ldst[i]));
+		}
+		ldep[3 * i] += d.x;
+		ldep[3 * i + 1] += d.y;
+		ldep[3 * i + 2] += d.z;
+	}
+}
+
+typedef struct {
+	int nsteps;
+	double *dts;
+	real *om, *ps;     /* [nsteps][n] omega_n, psi_n */
+} OTape;
+
+/* the tape of the last orc_adj_forward of each fluid (one fluid at a time is enough
+ * for the checker; kept beside the fluid by pointer) */
+typedef struct { OFluid *F; OTape T; } OTapeSlot;
+static OTapeSlot g_tapes[8];
+
+static OTape *tape_of(OFluid *F, int create)
+{
+	for (int k = 0; k < 8; k++)
+		if (g_tapes[k].F == F)
+			return &g_tapes[k].T;
+	if (!create)
+		return NULL;
+	for (int k = 0; k < 8; k++)
+		if (!g_tapes[k].F) {
+			g_tapes[k].F = F;
+			memset(&g_tapes[k].T, 0, sizeof(OTape));
+			return &g_tapes[k].T;
+		}
+	return NULL;
+}
+
+void orc_adj_release(OFluid *F)
+{
+	for (int k = 0; k < 8; k++)
+		if (g_tapes[k].F == F) {
+			free(g_tapes[k].T.dts);
+			free(g_tapes[k].T.om);
+			free(g_tapes[k].T.ps);
+			memset(&g_tapes[k], 0, sizeof(g_tapes[k]));
+		}
+}
+
+void orc_fluid_set_forcing(OFluid *F, double forcing) { F->forcing = forcing; }
+
+/* Forward rollout from the control omega0: omega = omega0 - mean, psi solved from zero,
+ * velocity; then nsteps steps with the given dts.  Records the tape. */
+int orc_adj_forward(OFluid *F, const double *omega0, const double *dts, int nsteps)
+{
+	const OGrid *g = &F->g;
+	int n = g->ncells;
+	orc_adj_release(F);
+	OTape *T = tape_of(F, 1);
+	if (!T || nsteps < 0)
+		return -1;
+	T->nsteps = nsteps;
+	T->dts = xcalloc(nsteps, sizeof(double));
+	T->om = xcalloc((size_t) nsteps * n, sizeof(real));
+	T->ps = xcalloc((size_t) nsteps * n, sizeof(real));
+	memcpy(T->dts, dts, sizeof(double) * nsteps);
+	for (int i = 0; i < n; i++)
+		F->omega[i] = (real) omega0[i];
+	orc_remove_mean(g, F->omega);
+	memset(F->psi, 0, sizeof(real) * n);
+	orc_solve_psi(F);
+	orc_compute_velocity(F);
+	for (int s = 0; s < nsteps; s++) {
+		memcpy(T->om + (size_t) s * n, F->omega, sizeof(real) * n);
+		memcpy(T->ps + (size_t) s * n, F->psi, sizeof(real) * n);
+		orc_fluid_step(F, dts[s]);
+	}
+	return 0;
+}
+
+/* Backward sweep: gT = dL/d omega_T in; g_omega0 = dL/d omega0 and *g_forcing = dL/dk
+ * out.  Leaves the fluid's state arrays overwritten by the recomputation. */
+int orc_adj_backward(OFluid *F, const double *gT, double *g_omega0, double *g_forcing)
+{
+	const OGrid *g = &F->g;
+	int n = g->ncells;
+	OTape *T = tape_of(F, 0);
+	if (!T)
+		return -1;
+	double c = 2.0 * F->rotation;
+	double *lam = xcalloc(n, sizeof(double)), *lpp = xcalloc(n, sizeof(double));
+	double *lq = xcalloc(n, sizeof(double)), *lq1 = xcalloc(n, sizeof(double));
+	double *lq2 = xcalloc(n, sizeof(double)), *lq2p = xcalloc(n, sizeof(double));
+	double *ldbw = xcalloc(3 * (size_t) n, sizeof(double)), *ldfw = xcalloc(3 * (size_t) n, sizeof(double));
+	double *lvel = xcalloc(3 * (size_t) n, sizeof(double)), *lpsi = xcalloc(n, sizeof(double));
+	real *z = xcalloc(n, sizeof(real)), *rhs_om = xcalloc(n, sizeof(real));
+	memcpy(lam, gT, sizeof(double) * n);
+	double dk = 0;
+	for (int s = T->nsteps - 1; s >= 0; s--) {
+		double dt = T->dts[s];
+		/* recompute step s's forward intermediates */
+		memcpy(F->omega, T->om + (size_t) s * n, sizeof(real) * n);
+		memcpy(F->psi, T->ps + (size_t) s * n, sizeof(real) * n);
+		orc_compute_velocity(F);
+		departures(F, dt, F->bfecc);
+		real *q = F->q0, *q1 = F->q1, *q2 = F->q2;
+		for (int i = 0; i < n; i++)
+			q[i] = (real) (F->omega[i] + c * g->pos[3 * i + 1]);
+		advect(g, q, F->dep_b, q1);
+		if (F->bfecc) {
+			advect(g, q1, F->dep_f, q2);
+			for (int i = 0; i < n; i++)
+				q2[i] = q[i] + (real) 0.5f * (q[i] - q2[i]);
+		}
+		double relax = F->forcing > 0 ? 1.0 - exp(-F->forcing * dt) : 0.0;
+		double drelax = F->forcing > 0 ? dt * exp(-F->forcing * dt) : 0.0;
+		/* mean removal (fluid.c:475): lpp = Pi_A^T lam */
+		double sum = 0;
+		for (int i = 0; i < n; i++)
+			sum += lam[i];
+		double m = sum / g->total_area;
+		for (int i = 0; i < n; i++)
+			lpp[i] = lam[i] - (double) g->area[i] * m;
+		memset(lq, 0, sizeof(double) * n);
+		memset(lq1, 0, sizeof(double) * n);
+		memset(lq2, 0, sizeof(double) * n);
+		memset(ldbw, 0, sizeof(double) * 3 * n);
+		memset(ldfw, 0, sizeof(double) * 3 * n);
+		memset(lvel, 0, sizeof(double) * 3 * n);
+		/* relaxation and limiter (fluid.c:449-474) */
+		for (int i = 0; i < n; i++) {
+			int idx[4];
+			double wt[4];
+			v3d dw[4];
+			taps_grad(g, ldr(F->dep_b + 3 * i), idx, wt, dw);
+			const real *src = F->bfecc ? q2 : q;
+			double v = 0, lo = 1e30, hi = -1e30;
+			int klo = 0, khi = 0;
+			for (int k = 0; k < 4; k++) {
+				v += wt[k] * src[idx[k]];
+				double o = q[idx[k]];
+				if (o < lo) { lo = o; klo = k; }
+				if (o > hi) { hi = o; khi = k; }
+			}
+			int state = 0;
+			double qn = v;
+			if (F->bfecc) {
+				if (v < lo) { qn = lo; state = 1; }
+				if (qn > hi) { qn = hi; state = 2; }
+			}
+			double w = qn - c * g->pos[3 * i + 1];
+			dk += lpp[i] * ((double) F->omega_band[i] - w) * drelax;
+			double lqn = (1 - relax) * lpp[i];
+			if (state == 1) {
+				lq[idx[klo]] += lqn;
+			} else if (state == 2) {
+				lq[idx[khi]] += lqn;
+			} else {
+				double *lt = F->bfecc ? lq2 : lq;
+				v3d d = mk(0, 0, 0);
+				for (int k = 0; k < 4; k++) {
+					lt[idx[k]] += wt[k] * lqn;
+					d = vadd(d, vscl(dw[k], (double) src[idx[k]] * lqn));
+				}
+				ldbw[3 * i] += d.x;
+				ldbw[3 * i + 1] += d.y;
+				ldbw[3 * i + 2] += d.z;
+			}
+		}
+		if (F->bfecc) {
+			/* q2 = q + 0.5 (q - A_f q1) (fluid.c:446-448), then q1 = A_b q */
+			for (int i = 0; i < n; i++) {
+				lq[i] += 1.5 * lq2[i];
+				lq2p[i] = -0.5 * lq2[i];
+			}
+			advect_adj(g, q1, F->dep_f, lq2p, lq1, ldfw);
+			advect_adj(g, q, F->dep_b, lq1, lq, ldbw);
+		}
+		/* departures (fluid.c:397-414) */
+		for (int i = 0; i < n; i++) {
+			departure_adj(g, F->vel, i, -0.5 * dt, -dt, ldd(ldbw + 3 * i), lvel);
+			if (F->bfecc)
+				departure_adj(g, F->vel, i, 0.5 * dt, dt, ldd(ldfw + 3 * i), lvel);
+		}
+		/* velocity (fluid.c:360-376): vel_i = p_i x sum_k gc_ik (psi_j - psi_i) */
+		memset(lpsi, 0, sizeof(double) * n);
+		for (int i = 0; i < n; i++) {
+			v3d mvec = vcross(ldd(lvel + 3 * i), gpos(g, i));
+			for (int k = 0; k < g->nnb[i]; k++) {
+				int j = g->nbr[MAXNB * i + k];
+				double cik = vdot(ldf(g->gc + 3 * (MAXNB * i + k)), mvec);
+				lpsi[j] += cik;
+				lpsi[i] -= cik;
+			}
+		}
+		/* Poisson: psi = Pi_A K^+ (-A omega): lam_omega = -A K^+ Pi_A^T lpsi.  K^+ ignores
+		 * the constant component of its argument; it is removed explicitly (Pi_A uses the
+		 * geometric total area, which differs from the sum of the float areas in the last
+		 * bits), so the solve sees a consistent right-hand side. */
+		double ps = 0, sa = 0;
+		for (int i = 0; i < n; i++) {
+			ps += lpsi[i];
+			sa += g->area[i];
+		}
+		double pm = ps / g->total_area, cm = ps * (1.0 - sa / g->total_area) / n;
+		for (int i = 0; i < n; i++)
+			rhs_om[i] = (real) (-(lpsi[i] - (double) g->area[i] * pm - cm) / (double) g->area[i]);
+		real *so = F->omega, *sp = F->psi;
+		F->omega = rhs_om;
+		F->psi = z;
+		orc_solve_psi(F);
+		F->omega = so;
+		F->psi = sp;
+		for (int i = 0; i < n; i++)
+			lam[i] = lq[i] - (double) g->area[i] * z[i];
+	}
+	/* set-up mean removal of the control */
+	double sum = 0;
+	for (int i = 0; i < n; i++)
+		sum += lam[i];
+	double m = sum / g->total_area;
+	for (int i = 0; i < n; i++)
+		g_omega0[i] = lam[i] - (double) g->area[i] * m;
+	*g_forcing = dk;
+	free(lam); free(lpp); free(lq); free(lq1); free(lq2); free(lq2p);
+	free(ldbw); free(ldfw); free(lvel); free(lpsi); free(z); free(rhs_om);
 	return 0;
 }
 

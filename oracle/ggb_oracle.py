@@ -96,6 +96,10 @@ def olib(kind):
             "orc_fluid_get": (C.c_int, [C.c_void_p, C.c_int, dp]),
             "orc_fluid_set": (C.c_int, [C.c_void_p, C.c_int, dp]),
             "orc_vcycle": (None, [C.c_void_p, rp, rp]),
+            "orc_fluid_set_forcing": (None, [C.c_void_p, C.c_double]),
+            "orc_adj_forward": (C.c_int, [C.c_void_p, dp, dp, C.c_int]),
+            "orc_adj_backward": (C.c_int, [C.c_void_p, dp, dp, dp]),
+            "orc_adj_release": (None, [C.c_void_p]),
             "orc_particles_init": (None, [C.c_int, rp, rp, fp]),
             "orc_particles_color": (None, [C.c_int, rp, fp, fp, C.c_int, C.c_int]),
             "orc_particles_advect": (None, [g, C.c_int, rp, rp, rp, C.c_double]),
@@ -207,9 +211,30 @@ class OracleFluid:
 
     def __del__(self):
         try:
+            self.o.L.orc_adj_release(self.h)
             self.o.L.orc_fluid_free(self.h)
         except Exception:
             pass
+
+    def set_forcing(self, k):
+        self.o.L.orc_fluid_set_forcing(self.h, float(k))
+
+    def adj_forward(self, omega0, dts):
+        """Rollout from the control omega0 with the dt schedule; returns omega_T."""
+        om = np.ascontiguousarray(omega0, np.float64).ravel()
+        d = np.ascontiguousarray(dts, np.float64).ravel()
+        if self.o.L.orc_adj_forward(self.h, _dp(om), _dp(d), len(d)) != 0:
+            raise RuntimeError("orc_adj_forward failed")
+        return self.get(0)
+
+    def adj_backward(self, gT):
+        """dL/d omega0 and dL/d forcing from dL/d omega_T (after adj_forward)."""
+        g = np.ascontiguousarray(gT, np.float64).ravel()
+        out = np.zeros(self.n)
+        gk = C.c_double()
+        if self.o.L.orc_adj_backward(self.h, _dp(g), _dp(out), C.byref(gk)) != 0:
+            raise RuntimeError("orc_adj_backward failed")
+        return out, gk.value
 
     def set_state(self, vel0, vband=None):
         v0 = np.ascontiguousarray(vel0, np.float32).ravel()
