@@ -215,21 +215,33 @@ __global__ __launch_bounds__(BS) void k_mg_smooth(ggbk_grid g, const TB *b, cons
 }
 #define TWO_PASS_MIN_CELLS 262144
 
-// x = M b, one wave per row, rows split over blocks; the row sum is a fixed tree.
+// x = M b, one workgroup per row: thread t sums columns t, t + DENSE_BS, ... (loads issued
+// DENSE_UNROLL at a time), then the workgroup tree; a fixed order for a given n.  A row per
+// workgroup keeps enough rows in flight to cover the load latency of this small pass.
+#define DENSE_UNROLL 8
 template <typename TM, typename TD, typename TB>
 __global__ __launch_bounds__(DENSE_BS) void k_mg_dense(const TD *M, int n, const TB *b, TM *x, const ggbk_scalars *s)
 {
 	int act = s->active;
-	int row = blockIdx.x * (DENSE_BS / 64) + (threadIdx.x >> 6);
-	int lane = threadIdx.x & 63;
-	if (row >= n)
-		return;
+	int row = blockIdx.x;
 	const TD *m = M + (size_t) row * n;
 	double acc = 0;
-	for (int c = lane; c < n; c += 64)
+	int c = threadIdx.x;
+	for (; c + (DENSE_UNROLL - 1) * DENSE_BS < n; c += DENSE_UNROLL * DENSE_BS) {
+		double mv[DENSE_UNROLL], bv[DENSE_UNROLL];
+#pragma unroll
+		for (int u = 0; u < DENSE_UNROLL; u++) {
+			mv[u] = (double) m[c + u * DENSE_BS];
+			bv[u] = (double) b[c + u * DENSE_BS];
+		}
+#pragma unroll
+		for (int u = 0; u < DENSE_UNROLL; u++)
+			acc += mv[u] * bv[u];
+	}
+	for (; c < n; c += DENSE_BS)
 		acc += (double) m[c] * (double) b[c];
-	acc = wave_sum(acc);
-	if (lane == 0 && act)
+	acc = block_sum<DENSE_BS>(acc);
+	if (threadIdx.x == 0 && act)
 		x[row] = (TM) acc;
 }
 
@@ -343,7 +355,7 @@ static void vcycle_level(const ggbk_level *lv, int nlev, int l, const TB *b, TM 
 	const ggbk_grid &g = L.g;
 	TM *r = (TM *) L.r;
 	if (l > 0 && L.M) {
-		unsigned nb = (unsigned) ((g.n + DENSE_BS / 64 - 1) / (DENSE_BS / 64));
+		unsigned nb = (unsigned) g.n;
 		if (sizeof(TM) == 8)
 			GGBK_LAUNCH((k_mg_dense<TM, double, TB>), nb, DENSE_BS, 0, (const double *) L.M, g.n, b, x, s);
 		else

@@ -15,15 +15,14 @@
 //   3. fill   -- the same boxes write the particle's slot into each tile's list, and
 //                each particle is copied to its home tile's range of the next storage
 //                order, which the caller adopts after the draw;
-//   4. draw   -- one workgroup per tile: load the tile, then per window of particles
-//                each one computes its samples on the tile once (cached in LDS) and
-//                appends a fragment to every texel it touches; each texel sorts its few
-//                fragments by (particle index, sample) and blends them in that order --
-//                the reference's per-texel order.  A tile with more than WIN particles
-//                is sorted by particle index and processed in windows; a window in
-//                which a texel collects more than CAPT fragments is retried with a
-//                quarter of the particles, and a single particle with more than SMAX
-//                samples on the tile is blended in order by one thread.
+//   4. draw   -- one workgroup per tile: load the tile into LDS, then per window of
+//                particles compute every (particle, sample) pixel on the tile with one
+//                thread per sample, append each on-tile sample to the texels it touches,
+//                and let each texel sort its few fragments by (particle index, sample)
+//                and blend them in that order -- the reference's per-texel order.  A
+//                tile with more than WIN particles is sorted by particle index and cut
+//                into windows; a window that overflows the LDS sample or fragment
+//                capacity is retried with a quarter of the particles.
 // The texel arithmetic is the reference's (render.c:113-154) in float with no FMA
 // contraction, so texels match the CPU path bit for bit for identical positions.
 #include <hip/hip_runtime.h>
@@ -37,9 +36,7 @@
 #define NTEX (TILE * TILE)
 #define WIN 128           // particles per window = draw workgroup size
 #define DBS WIN
-#define SMAX 12           // samples per particle cached in LDS (sample index fits 4 bits)
 #define CAPT 12           // fragments one texel can hold per window
-#define SORT_CAP 512      // LDS sort capacity per tile
 #define SCAN_BS 1024
 
 struct SplatGeo {
@@ -426,17 +423,50 @@ __device__ __forceinline__ void blend_texel(float4 &dd, int t, const TileRect &t
 	dd.w = dd.w * keep + add;
 }
 
+// Compare-exchange and odd-even transposition networks on 32-bit keys held in registers
+// (all indices static after unrolling).
+__device__ __forceinline__ void cswap(unsigned &a, unsigned &b)
+{
+	unsigned lo = min(a, b), hi = max(a, b);
+	a = lo;
+	b = hi;
+}
+
+template <int M>
+__device__ __forceinline__ void sort_net(unsigned *k)
+{
+#pragma unroll
+	for (int r = 0; r < M; r++)
+#pragma unroll
+		for (int i = r & 1; i + 1 < M; i += 2)
+			cswap(k[i], k[i + 1]);
+}
+
+// Per window of W particles (W <= WIN):
+//   A  one thread per particle: endpoints, sample count ns, index, colour;
+//   B  the particle's rank by index within the window (the window is index-sorted when the
+//      tile list was, otherwise ranks are counted); everything is stored by rank, and the
+//      exclusive prefix of ns in rank order;
+//   then per range of consecutive ranks whose samples fit SCAP (ranges are in index order):
+//   C  one thread per (particle, sample): the sample's pixel on this tile's face; samples
+//      on the tile take a slot (position + key rank<<12 | k-1) and append the slot to every
+//      texel they touch;
+//   D  one thread per texel: sort its fragments by key and blend them in that order.
+// A range in which a texel collects more than CAPT fragments is retried with half the
+// particles; a single particle that still overflows is blended in order by one thread.
+#define SCAP 512         // on-tile samples per range (slot fits 10 bits)
 __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts ps, SplatGeo sg, float invg)
 {
 	__shared__ float4 tex[NTEX];
 	__shared__ int tcnt[NTEX];
 	__shared__ unsigned short frag[CAPT][NTEX];
-	__shared__ float2 samp[SMAX][WIN];
-	__shared__ float4 pcol[WIN];
-	__shared__ int wpid[WIN];
-	__shared__ unsigned long long sbuf[SORT_CAP];
+	__shared__ float2 sxy[SCAP];
+	__shared__ unsigned skey[SCAP];
+	__shared__ double pa[3][WIN], pb[3][WIN];  // by rank
+	__shared__ int pns[WIN], pst[WIN + 1], ppid[WIN];
+	__shared__ float4 pcol[WIN];               // by rank
 	__shared__ unsigned char touched[NTEX];
-	__shared__ int overflow, ntouched;
+	__shared__ int overflow, nsamp, wtot, ntouched;
 	long long c0 = clock64();
 	int t = xblock();
 	const int lid = threadIdx.x;
@@ -463,21 +493,20 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 		int ly = i / TILE, lx = i % TILE;
 		if (ly < tr.h && lx < tr.w)
 			tex[i] = gbuf[((size_t) tr.f * r.R + tr.y0 + ly) * r.R + tr.x0 + lx];
-	}
-	for (int i = lid; i < NTEX; i += DBS)
 		touched[i] = 0;
+	}
 	if (lid == 0)
 		ntouched = 0;
 
-	// A tile with at most WIN particles is one window in list order (fragments carry the
-	// particle index); otherwise (pid, slot) pairs are sorted and cut into windows.
-	unsigned long long *sorted = sbuf;
-	bool is_sorted = false;
-	auto sort_list = [&]() {
+	// a tile with more than WIN particles is cut into index-ordered windows: sort the
+	// (index, slot) pairs in this tile's part of the scratch array
+	unsigned long long *sorted = nullptr;
+	bool is_sorted = L > WIN;
+	if (is_sorted) {
 		int P = 1;
 		while (P < L)
 			P <<= 1;
-		sorted = P <= SORT_CAP ? sbuf : r.sort_scratch + 2 * (size_t) r.tile_start[t];
+		sorted = r.sort_scratch + 2 * (size_t) r.tile_start[t];
 		for (int i = lid; i < P; i += DBS) {
 			unsigned long long key = ~0ull;
 			if (i < L) {
@@ -488,132 +517,181 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 		}
 		__syncthreads();
 		bitonic_sort64(sorted, P);
-		is_sorted = true;
-	};
-	if (L > WIN)
-		sort_list();
+	}
 	__syncthreads();
 
 	long long csamp = 0, cblend = 0;
 	int windows = 0, serial_windows = 0;
-	int W = min(L, WIN);
-	for (int base = 0; base < L;) {
+	for (int base = 0; base < L; base += WIN) {
+		int W = min(WIN, L - base);
 		long long ca = clock64();
-		windows++;
-		for (int i = lid; i < NTEX; i += DBS)
-			tcnt[i] = 0;
-		if (lid == 0)
-			overflow = 0;
-		__syncthreads();
-		// samples of this window's particles on the tile (one thread per particle);
-		// every (particle, sample, texel) fragment goes to its texel's list
+		// A: the window's particles
+		float4 mycol = make_float4(0, 0, 0, 0);
+		int myns = 0, mypid = 0;
+		dv3 a = d3(0, 0, 0), b = d3(0, 0, 0);
 		if (lid < W) {
-			int slot = is_sorted ? (int) (unsigned) sorted[base + lid] : lst[base + lid];
-			dv3 a = ld3(ps.prev + 3 * (size_t) slot), b = ld3(ps.pos + 3 * (size_t) slot);
-			int ns = arc_samples(vangle(a, b), sg.spacing);
-			int nsamp = 0;
-			bool over = false;
-			for (int k = 1; k <= ns && !over; k++) {
-				float px, py;
-				if (!face_pixel(tr.f, arc_point(a, b, k, ns), sg.R, px, py))
-					continue;
-				int xlo, xhi, ylo, yhi;
-				if (!tile_clip(tr, px, py, rad, sg.R, xlo, xhi, ylo, yhi))
-					continue;
-				if (nsamp >= SMAX) {
-					over = true;
-					break;
-				}
-				samp[nsamp][lid] = make_float2(px, py);
-				unsigned short key = (unsigned short) ((lid << 4) | nsamp);
-				splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
-					int s = atomicAdd(&tcnt[tx], 1);
-					if (s < CAPT)
-						frag[s][tx] = key;
-					else
-						over = true;
-				});
-				nsamp++;
-			}
-			pcol[lid] = *(const float4 *) (ps.col + 4 * (size_t) slot);
-			wpid[lid] = ps.pid[slot];
-			if (over)
-				overflow = 1;
+			int slot = is_sorted ? (int) (unsigned) sorted[base + lid] : lst[lid];
+			a = ld3(ps.prev + 3 * (size_t) slot);
+			b = ld3(ps.pos + 3 * (size_t) slot);
+			myns = arc_samples(vangle(a, b), sg.spacing);
+			mypid = ps.pid[slot];
+			mycol = *(const float4 *) (ps.col + 4 * (size_t) slot);
+			ppid[lid] = mypid;
 		}
 		__syncthreads();
-		long long cb = clock64();
-		csamp += cb - ca;
-		if (!overflow) {
-			// each texel applies its fragments in (particle index, sample) order
-			for (int tx = lid; tx < NTEX; tx += DBS) {
-				int n = tcnt[tx];
-				if (n == 0)
-					continue;
-				touched[tx] = 1;
-				unsigned long long keys[CAPT];
-#pragma unroll
-				for (int q = 0; q < CAPT; q++) {
-					unsigned long long kk = ~0ull;
-					if (q < n) {
-						unsigned f = frag[q][tx];
-						kk = ((unsigned long long) (unsigned) wpid[f >> 4] << 12) | f;
-					}
-					keys[q] = kk;
-				}
-#pragma unroll
-				for (int q = 1; q < CAPT; q++) {
-#pragma unroll
-					for (int p = q; p > 0; p--) {
-						unsigned long long x = keys[p - 1], y = keys[p];
-						bool sw = x > y;
-						keys[p - 1] = sw ? y : x;
-						keys[p] = sw ? x : y;
-					}
-				}
-				float4 dd = tex[tx];
-				for (int q = 0; q < n; q++) {
-					unsigned f = (unsigned) (keys[q] & 0xfffull);
-					int pr = f >> 4, k = f & 15;
-					float2 s2 = samp[k][pr];
-					float4 c = pcol[pr];
-					blend_texel(dd, tx, tr, s2.x, s2.y, invrad, r.alpha * c.w, c, invg);
-				}
-				tex[tx] = dd;
+		// B: rank by index; store by rank
+		int rank = lid;
+		if (lid < W) {
+			if (!is_sorted) {
+				rank = 0;
+				for (int q = 0; q < W; q++)
+					rank += ppid[q] < mypid;
 			}
-			base += W;
-			W = min(WIN, L - base);
-		} else if (!is_sorted) {
-			// the single unsorted window overflowed before anything was blended: sort
-			// and retry in index-ordered windows
-			__syncthreads();
-			sort_list();
-			W = max(W / 4, 1);
-		} else if (W > 1) {
-			W = max(W / 4, 1);
-		} else {
-			// one particle with more than SMAX samples on the tile, or more than CAPT
-			// samples on one texel: one thread blends it in order
-			serial_windows++;
-			if (lid == 0) {
-				int slot = (int) (unsigned) sorted[base];
-				dv3 a = ld3(ps.prev + 3 * (size_t) slot), b = ld3(ps.pos + 3 * (size_t) slot);
-				int ns = arc_samples(vangle(a, b), sg.spacing);
-				float4 c = *(const float4 *) (ps.col + 4 * (size_t) slot);
-				for (int k = 1; k <= ns; k++) {
+			pa[0][rank] = a.x; pa[1][rank] = a.y; pa[2][rank] = a.z;
+			pb[0][rank] = b.x; pb[1][rank] = b.y; pb[2][rank] = b.z;
+			pcol[rank] = mycol;
+			pns[rank] = myns;
+		}
+		__syncthreads();
+		// exclusive prefix of ns in rank order (two waves)
+		int nsr = lid < W ? pns[lid] : 0;
+		int incl = wave_incl_scan(nsr);
+		if (__lane_id() == 63 && lid < 64)
+			wtot = incl;
+		__syncthreads();
+		if (lid < W)
+			pst[lid] = incl - nsr + (lid >= 64 ? wtot : 0);
+		__syncthreads();
+		if (lid == W - 1)
+			pst[W] = pst[W - 1] + pns[W - 1];
+		__syncthreads();
+		csamp += clock64() - ca;
+
+		for (int r0 = 0; r0 < W;) {
+			long long cc = clock64();
+			windows++;
+			// the longest range from r0 whose samples fit SCAP (at least one particle)
+			int lo = r0 + 1, hi = W;
+			while (lo < hi) {
+				int mid = (lo + hi + 1) >> 1;
+				if (pst[mid] - pst[r0] <= SCAP)
+					lo = mid;
+				else
+					hi = mid - 1;
+			}
+			int r1 = lo;
+			bool retry = true;
+			while (retry) {
+				for (int i = lid; i < NTEX; i += DBS)
+					tcnt[i] = 0;
+				if (lid == 0) {
+					overflow = 0;
+					nsamp = 0;
+				}
+				__syncthreads();
+				// C: one thread per (particle, sample) of ranks [r0, r1)
+				int j0 = pst[r0], j1 = pst[r1];
+				for (int j = j0 + lid; j < j1; j += DBS) {
+					int l = r0, h = r1 - 1; // last rank whose first sample is <= j
+					while (l < h) {
+						int mid = (l + h + 1) >> 1;
+						if (pst[mid] <= j)
+							l = mid;
+						else
+							h = mid - 1;
+					}
+					int p = l, k = j - pst[p] + 1;
+					dv3 pa3 = d3(pa[0][p], pa[1][p], pa[2][p]), pb3 = d3(pb[0][p], pb[1][p], pb[2][p]);
 					float px, py;
-					if (!face_pixel(tr.f, arc_point(a, b, k, ns), sg.R, px, py))
+					if (!face_pixel(tr.f, arc_point(pa3, pb3, k, pns[p]), sg.R, px, py))
 						continue;
+					int xlo, xhi, ylo, yhi;
+					if (!tile_clip(tr, px, py, rad, sg.R, xlo, xhi, ylo, yhi))
+						continue;
+					int sl = atomicAdd(&nsamp, 1);
+					if (sl >= SCAP) {
+						overflow = 1;
+						continue;
+					}
+					sxy[sl] = make_float2(px, py);
+					skey[sl] = ((unsigned) p << 12) | (unsigned) (k - 1);
+					bool over = false;
 					splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
-						touched[tx] = 1;
-						blend_texel(tex[tx], tx, tr, px, py, invrad, r.alpha * c.w, c, invg);
+						int c = atomicAdd(&tcnt[tx], 1);
+						if (c < CAPT)
+							frag[c][tx] = (unsigned short) sl;
+						else
+							over = true;
 					});
+					if (over)
+						overflow = 1;
 				}
+				__syncthreads();
+				long long cb = clock64();
+				csamp += cb - cc;
+				if (!overflow) {
+					// D: each texel sorts its fragments (key << 10 | slot) and blends
+					for (int tx = lid; tx < NTEX; tx += DBS) {
+						int n = tcnt[tx];
+						if (n == 0)
+							continue;
+						touched[tx] = 1;
+						unsigned keys[CAPT];
+#pragma unroll
+						for (int q = 0; q < CAPT; q++) {
+							unsigned v = ~0u;
+							if (q < n) {
+								unsigned sl = frag[q][tx];
+								v = (skey[sl] << 10) | sl;
+							}
+							keys[q] = v;
+						}
+						if (n <= 4)
+							sort_net<4>(keys);
+						else if (n <= 8)
+							sort_net<8>(keys);
+						else
+							sort_net<CAPT>(keys);
+						float4 dd = tex[tx];
+						for (int q = 0; q < n; q++) {
+							unsigned sl = keys[q] & 1023u;
+							unsigned rk = keys[q] >> 22;
+							float2 s2 = sxy[sl];
+							float4 c = pcol[rk];
+							blend_texel(dd, tx, tr, s2.x, s2.y, invrad, r.alpha * c.w, c, invg);
+						}
+						tex[tx] = dd;
+					}
+					retry = false;
+					r0 = r1;
+				} else if (r1 - r0 > 1) {
+					r1 = r0 + (r1 - r0) / 2;
+				} else {
+					// one particle that alone overflows: one thread blends it in order
+					serial_windows++;
+					if (lid == 0) {
+						dv3 pa3 = d3(pa[0][r0], pa[1][r0], pa[2][r0]);
+						dv3 pb3 = d3(pb[0][r0], pb[1][r0], pb[2][r0]);
+						int ns = pns[r0];
+						float4 c = pcol[r0];
+						for (int k = 1; k <= ns; k++) {
+							float px, py;
+							if (!face_pixel(tr.f, arc_point(pa3, pb3, k, ns), sg.R, px, py))
+								continue;
+							splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
+								touched[tx] = 1;
+								blend_texel(tex[tx], tx, tr, px, py, invrad, r.alpha * c.w, c,
+									    invg);
+							});
+						}
+					}
+					retry = false;
+					r0 = r1;
+				}
+				__syncthreads();
+				cblend += clock64() - cb;
 			}
-			base += 1;
-			W = min(WIN, L - base);
 		}
-		__syncthreads();
-		cblend += clock64() - cb;
 	}
 
 	// write back the texels a splat touched (NTEX is a multiple of DBS: whole waves)
@@ -628,8 +706,8 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 	if (__lane_id() == 0)
 		atomicAdd(&ntouched, cnt);
 	__syncthreads();
-	// this tile's diagnostics (summed over tiles on request): touched texels, windows,
-	// serial windows, sorted, sample / blend / total cycles in units of 1024
+	// this tile's diagnostics (summed over tiles on request): touched texels, ranges,
+	// serial ranges, sorted, sample / blend / total cycles in units of 1024
 	if (lid == 0) {
 		diag[0] = ntouched;
 		diag[1] = windows;
