@@ -134,6 +134,22 @@ ggbk_grid ggb_grid_dev(const CubeSphere *csc)
 	g.gc = grid_buf(gd, sizeof(float) * 3 * nk, gc);
 	g.ghost = grid_buf(gd, sizeof(int) * 6 * (size_t) (N + 2) * (N + 2), cs->ghost);
 	g.dinv = grid_buf(gd, sizeof(float) * (size_t) n, dinv);
+	if (maxnb <= 6) {
+		/* interleaved per-cell (neighbour, weight) pairs for the stencil passes */
+		int *stn = malloc(sizeof(int) * 12 * (size_t) n);
+		if (!stn)
+			ggb_die("out of memory");
+		for (int i = 0; i < n; i++)
+			for (int k = 0; k < 6; k++) {
+				int in = k < cs->nnb[i];
+				size_t src = (size_t) CS_MAXNB * i + k;
+				float wv = in ? cs->w[src] : 0.0f;
+				stn[12 * (size_t) i + 2 * k] = in ? cs->nbr[src] : i;
+				memcpy(&stn[12 * (size_t) i + 2 * k + 1], &wv, sizeof(float));
+			}
+		g.stn = grid_buf(gd, sizeof(int) * 12 * (size_t) n, stn);
+		free(stn);
+	}
 	free(nbr);
 	free(w);
 	free(el);

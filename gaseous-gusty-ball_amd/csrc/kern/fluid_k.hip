@@ -24,11 +24,7 @@ __device__ __forceinline__ double applyK_at(const ggbk_grid &g, const T *x, int 
 {
 	// (K x)_i = sum_k w_ik (x_i - x_nb(k)) in stored neighbour order (fluid.c:277-286)
 	double xi = (double) x[i], s = 0;
-	int nn = g.nnb[i];
-	for (int k = 0; k < nn; k++) {
-		int j = g.nbr[(size_t) k * g.n + i];
-		s += (double) g.w[(size_t) k * g.n + i] * (xi - (double) x[j]);
-	}
+	for_stencil(g, i, [&](int j, float w) { s += (double) w * (xi - (double) x[j]); });
 	return s;
 }
 
@@ -289,12 +285,10 @@ __global__ __launch_bounds__(BS) void k_cg_Ap(ggbk_grid g, const TZ *z, const T 
 	if (i < g.n) {
 		T pi = first ? (T) z[i] : (T) ((double) z[i] + beta * (double) p_in[i]);
 		double sum = 0, xi = (double) pi;
-		int nn = g.nnb[i];
-		for (int k = 0; k < nn; k++) {
-			int j = g.nbr[(size_t) k * g.n + i];
+		for_stencil(g, i, [&](int j, float w) {
 			T pj = first ? (T) z[j] : (T) ((double) z[j] + beta * (double) p_in[j]);
-			sum += (double) g.w[(size_t) k * g.n + i] * (xi - (double) pj);
-		}
+			sum += (double) w * (xi - (double) pj);
+		});
 		p_out[i] = pi;
 		Ap[i] = (T) sum;
 		v = sum * xi;

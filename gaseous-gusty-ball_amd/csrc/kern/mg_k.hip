@@ -53,11 +53,7 @@ __global__ __launch_bounds__(BS) void k_mg_down(ggbk_grid g, const TB *b, TM *r1
 	if (i >= g.n)
 		return;
 	double xi = (double) x1_at<TM, TB>(g, b, i), sum = 0;
-	int nn = g.nnb[i];
-	for (int k = 0; k < nn; k++) {
-		int j = g.nbr[(size_t) k * g.n + i];
-		sum += (double) g.w[(size_t) k * g.n + i] * (xi - (double) x1_at<TM, TB>(g, b, j));
-	}
+	for_stencil(g, i, [&](int j, float w) { sum += (double) w * (xi - (double) x1_at<TM, TB>(g, b, j)); });
 	if (act)
 		r1[i] = (TM) ((double) b[i] - sum);
 }
@@ -80,12 +76,10 @@ __global__ __launch_bounds__(BS) void k_cg_update_down(ggbk_grid g, T *psi, cons
 		T rt = (T) ri;
 		T ps = psi[i] + (T) (alpha * (double) p[i]);
 		double xi = (double) x1_of<TM, T>(rt, g.dinv[i]), sum = 0;
-		int nn = g.nnb[i];
-		for (int k = 0; k < nn; k++) {
-			int j = g.nbr[(size_t) k * g.n + i];
+		for_stencil(g, i, [&](int j, float w) {
 			T rj = (T) ((double) r_in[j] - alpha * (double) Ap[j]);
-			sum += (double) g.w[(size_t) k * g.n + i] * (xi - (double) x1_of<TM, T>(rj, g.dinv[j]));
-		}
+			sum += (double) w * (xi - (double) x1_of<TM, T>(rj, g.dinv[j]));
+		});
 		if (act) {
 			psi[i] = ps;
 			r_out[i] = rt;
@@ -149,11 +143,9 @@ __global__ __launch_bounds__(BS) void k_mg_up(ggbk_grid g, ggbk_xfer xf, const T
 	if (i < g.n) {
 		TM xi = xcorr_at<TM, TB>(g, xf, b, xc, i);
 		double sum = 0;
-		int nn = g.nnb[i];
-		for (int k = 0; k < nn; k++) {
-			int j = g.nbr[(size_t) k * g.n + i];
-			sum += (double) g.w[(size_t) k * g.n + i] * ((double) xi - (double) xcorr_at<TM, TB>(g, xf, b, xc, j));
-		}
+		for_stencil(g, i, [&](int j, float w) {
+			sum += (double) w * ((double) xi - (double) xcorr_at<TM, TB>(g, xf, b, xc, j));
+		});
 		TM ri = (TM) ((double) b[i] - sum);
 		TM xo = xi + (TM) (MG_OMEGA * (double) g.dinv[i] * (double) ri);
 		if (act)
@@ -195,11 +187,7 @@ __global__ __launch_bounds__(BS) void k_mg_smooth(ggbk_grid g, const TB *b, cons
 	if (i < g.n) {
 		TM xi = xp[i];
 		double sum = 0;
-		int nn = g.nnb[i];
-		for (int k = 0; k < nn; k++) {
-			int j = g.nbr[(size_t) k * g.n + i];
-			sum += (double) g.w[(size_t) k * g.n + i] * ((double) xi - (double) xp[j]);
-		}
+		for_stencil(g, i, [&](int j, float w) { sum += (double) w * ((double) xi - (double) xp[j]); });
 		double bi = (double) b[i];
 		TM ri = (TM) (bi - sum);
 		TM xo = xi + (TM) (MG_OMEGA * (double) g.dinv[i] * (double) ri);
