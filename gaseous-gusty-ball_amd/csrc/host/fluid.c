@@ -286,9 +286,7 @@ static ggbk_xfer build_transfer(struct ggb_fluid_dev *D, const CubeSphere *fine,
 	int *ridx = malloc(sizeof(int) * 4 * (size_t) nf);
 	float *rw = malloc(sizeof(float) * 4 * (size_t) nf);
 	int *cur = malloc(sizeof(int) * (size_t) nc);
-	int *pt_cm = malloc(sizeof(int) * 4 * (size_t) nf);
-	float *pw_cm = malloc(sizeof(float) * 4 * (size_t) nf);
-	if (!ptap || !pw || !rstart || !ridx || !rw || !cur || !pt_cm || !pw_cm)
+	if (!ptap || !pw || !rstart || !ridx || !rw || !cur)
 		ggb_die("fluid: out of memory");
 	#pragma omp parallel for schedule(static)
 	for (int i = 0; i < nf; i++) {
@@ -298,8 +296,6 @@ static ggbk_xfer build_transfer(struct ggb_fluid_dev *D, const CubeSphere *fine,
 		for (int k = 0; k < 4; k++) {
 			ptap[4 * i + k] = idx[k];
 			pw[4 * i + k] = (float) wt[k];
-			pt_cm[(size_t) k * nf + i] = idx[k];
-			pw_cm[(size_t) k * nf + i] = (float) wt[k];
 		}
 	}
 	for (int i = 0; i < 4 * nf; i++)
@@ -321,38 +317,39 @@ static ggbk_xfer build_transfer(struct ggb_fluid_dev *D, const CubeSphere *fine,
 			rmax = rstart[c + 1] - rstart[c];
 	if (rmax > GGBK_RMAX)
 		ggb_die("fluid: restriction row of %d entries exceeds %d", rmax, GGBK_RMAX);
-	int *ri_ell = calloc((size_t) rmax * nc, sizeof(int));
-	float *rw_ell = calloc((size_t) rmax * nc, sizeof(float));
-	if (!ri_ell || !rw_ell)
+	int rpad = rmax <= 18 ? 18 : GGBK_RMAX; /* the restriction kernel is built for these widths */
+	int *rr = calloc((size_t) 2 * rpad * nc, sizeof(int)); /* padding: fine cell 0, weight 0 */
+	int *pt = malloc(sizeof(int) * 8 * (size_t) nf);
+	if (!rr || !pt)
 		ggb_die("fluid: out of memory");
 	for (int c = 0; c < nc; c++)
 		for (int k = rstart[c]; k < rstart[c + 1]; k++) {
-			ri_ell[(size_t) (k - rstart[c]) * nc + c] = ridx[k];
-			rw_ell[(size_t) (k - rstart[c]) * nc + c] = rw[k];
+			int e = k - rstart[c];
+			size_t o = ((size_t) (e / 2) * nc + c) * 4 + 2 * (e % 2);
+			rr[o] = ridx[k];
+			memcpy(&rr[o + 1], &rw[k], sizeof(float));
+		}
+	for (size_t f = 0; f < (size_t) nf; f++)
+		for (int k = 0; k < 4; k++) {
+			pt[8 * f + 2 * k] = ptap[4 * f + k];
+			memcpy(&pt[8 * f + 2 * k + 1], &pw[4 * f + k], sizeof(float));
 		}
 	ggbk_xfer x;
 	memset(&x, 0, sizeof(x));
 	x.nf = nf;
 	x.nc = nc;
 	x.rmax = rmax;
+	x.rpad = rpad;
 	void *d;
-	d = xbuf(D, sizeof(int) * 4 * (size_t) nf, "transfer");
-	upload(d, pt_cm, sizeof(int) * 4 * (size_t) nf);
-	x.ptap = d;
-	d = xbuf(D, sizeof(float) * 4 * (size_t) nf, "transfer");
-	upload(d, pw_cm, sizeof(float) * 4 * (size_t) nf);
-	x.pw = d;
-	d = xbuf(D, sizeof(int) * (size_t) rmax * nc, "transfer");
-	upload(d, ri_ell, sizeof(int) * (size_t) rmax * nc);
-	x.ridx = d;
-	d = xbuf(D, sizeof(float) * (size_t) rmax * nc, "transfer");
-	upload(d, rw_ell, sizeof(float) * (size_t) rmax * nc);
-	x.rw = d;
-	free(ri_ell);
-	free(rw_ell);
+	d = xbuf(D, sizeof(int) * 8 * (size_t) nf, "transfer");
+	upload(d, pt, sizeof(int) * 8 * (size_t) nf);
+	x.pt = d;
+	d = xbuf(D, sizeof(int) * 2 * (size_t) rpad * nc, "transfer");
+	upload(d, rr, sizeof(int) * 2 * (size_t) rpad * nc);
+	x.rr = d;
+	free(rr);
+	free(pt);
 	free(cur);
-	free(pt_cm);
-	free(pw_cm);
 	hl->ptap = ptap;
 	hl->pw = pw;
 	hl->rstart = rstart;

@@ -79,92 +79,101 @@ __device__ __forceinline__ double fold_max(const double *p, int nb)
 __global__ __launch_bounds__(FIN_BS) void k_fin(int op, ggbk_scalars *s, ggbk_red red, int nb, double tol, int maxit,
 						 double total_area)
 {
+	// the scalars an update reads are loaded together with the partials, and the loop
+	// flag is checked only after the fold, so the kernel waits for memory once
 	bool guarded = op == GGBK_FIN_PAP || op == GGBK_FIN_RR || op == GGBK_FIN_RZ_FIRST || op == GGBK_FIN_RZ;
-	if (guarded && !s->active)
-		return;
+	int act = s->active;
+	double rz = s->rz, tol2 = s->tol2;
+	int it = s->it, smaxit = s->maxit;
 	const double *P = red.part;
 	size_t st = red.stride;
+	double a = 0, b = 0;
+	switch (op) {
+	case GGBK_FIN_START:
+		a = fold_sum(P + GGBK_SL_BB * st, nb);
+		b = fold_sum(P + GGBK_SL_RR0 * st, nb);
+		break;
+	case GGBK_FIN_PAP:
+		a = fold_sum(P + GGBK_SL_PAP * st, nb);
+		break;
+	case GGBK_FIN_RR:
+		a = fold_sum(P + GGBK_SL_RR * st, nb);
+		break;
+	case GGBK_FIN_RZ_FIRST:
+	case GGBK_FIN_RZ:
+		a = fold_sum(P + GGBK_SL_RZ * st, nb);
+		break;
+	case GGBK_FIN_MEAN_F64:
+	case GGBK_FIN_MEAN_F32:
+		a = fold_sum(P + GGBK_SL_MEAN * st, nb);
+		break;
+	case GGBK_FIN_VMAX:
+		a = fold_max(P + GGBK_SL_VMAX * st, nb);
+		break;
+	}
+	if (threadIdx.x != 0 || (guarded && !act))
+		return;
 	switch (op) {
 	case GGBK_FIN_START: {
-		double bb = fold_sum(P + GGBK_SL_BB * st, nb);
-		double rr = fold_sum(P + GGBK_SL_RR0 * st, nb);
-		if (threadIdx.x == 0) {
-			s->bb = bb;
-			s->rr = rr;
-			s->it = 0;
-			s->maxit = maxit;
-			s->rz = 0;
-			if (bb == 0.0) {
-				s->status = 2;
-				s->tol2 = 0;
-				s->active = 0;
-			} else {
-				s->status = 0;
-				s->tol2 = tol * tol * bb;
-				s->active = (rr > s->tol2 && maxit > 0) ? 1 : 0;
-			}
+		double bb = a, rr = b;
+		s->bb = bb;
+		s->rr = rr;
+		s->it = 0;
+		s->maxit = maxit;
+		s->rz = 0;
+		if (bb == 0.0) {
+			s->status = 2;
+			s->tol2 = 0;
+			s->active = 0;
+		} else {
+			s->status = 0;
+			s->tol2 = tol * tol * bb;
+			s->active = (rr > s->tol2 && maxit > 0) ? 1 : 0;
 		}
 		break;
 	}
 	case GGBK_FIN_PAP: {
-		double pAp = fold_sum(P + GGBK_SL_PAP * st, nb);
-		if (threadIdx.x == 0) {
-			s->pAp = pAp;
-			if (pAp <= 0) {
-				s->status = 1;
-				s->active = 0;
-			} else {
-				s->alpha = s->rz / pAp;
-			}
+		double pAp = a;
+		s->pAp = pAp;
+		if (pAp <= 0) {
+			s->status = 1;
+			s->active = 0;
+		} else {
+			s->alpha = rz / pAp;
 		}
 		break;
 	}
 	case GGBK_FIN_RR: {
-		double rr = fold_sum(P + GGBK_SL_RR * st, nb);
-		if (threadIdx.x == 0) {
-			s->rr = rr;
-			int it = s->it + 1;
-			s->it = it;
-			if (rr <= s->tol2) {
-				s->active = 0;
-				s->status = 0;
-			} else if (it >= s->maxit) {
-				s->active = 0;
-				s->status = 3;
-			}
+		double rr = a;
+		s->rr = rr;
+		it = it + 1;
+		s->it = it;
+		if (rr <= tol2) {
+			s->active = 0;
+			s->status = 0;
+		} else if (it >= smaxit) {
+			s->active = 0;
+			s->status = 3;
 		}
 		break;
 	}
-	case GGBK_FIN_RZ_FIRST: {
-		double rz = fold_sum(P + GGBK_SL_RZ * st, nb);
-		if (threadIdx.x == 0)
-			s->rz = rz;
+	case GGBK_FIN_RZ_FIRST:
+		s->rz = a;
 		break;
-	}
-	case GGBK_FIN_RZ: {
-		double rznew = fold_sum(P + GGBK_SL_RZ * st, nb);
-		if (threadIdx.x == 0) {
-			s->beta = rznew / s->rz;
-			s->rz = rznew;
-		}
+	case GGBK_FIN_RZ:
+		s->beta = a / rz;
+		s->rz = a;
 		break;
-	}
 	case GGBK_FIN_MEAN_F64:
 	case GGBK_FIN_MEAN_F32: {
-		double sum = fold_sum(P + GGBK_SL_MEAN * st, nb);
-		if (threadIdx.x == 0) {
-			s->sum = sum;
-			double m = sum / total_area;
-			s->mean = op == GGBK_FIN_MEAN_F32 ? (double) (float) m : m;
-		}
+		s->sum = a;
+		double m = a / total_area;
+		s->mean = op == GGBK_FIN_MEAN_F32 ? (double) (float) m : m;
 		break;
 	}
-	case GGBK_FIN_VMAX: {
-		double v = fold_max(P + GGBK_SL_VMAX * st, nb);
-		if (threadIdx.x == 0)
-			s->vmax2 = v;
+	case GGBK_FIN_VMAX:
+		s->vmax2 = a;
 		break;
-	}
 	}
 }
 
@@ -273,12 +282,13 @@ __global__ __launch_bounds__(BS) void k_cg_start(ggbk_grid g, T *omega, const T 
 
 // p_out = z (first) or z + beta p_in, then Ap = K p_out with p_out recomputed at the
 // neighbours (the update of p is fused into the stencil pass; p is double-buffered).
-template <typename T, typename TZ>
-__global__ __launch_bounds__(BS) void k_cg_Ap(ggbk_grid g, const TZ *z, const T *p_in, T *p_out, T *Ap, int first,
+// FIRST is a template parameter: a run-time flag around the neighbour loads makes the
+// compiler wait for each of them in turn.
+template <typename T, typename TZ, bool first>
+__global__ __launch_bounds__(BS) void k_cg_Ap(ggbk_grid g, const TZ *z, const T *p_in, T *p_out, T *Ap,
 					       const ggbk_scalars *s, ggbk_red red)
 {
-	if (!s->active)
-		return;
+	int act = s->active; // checked before the stores only
 	double beta = first ? 0.0 : s->beta;
 	int i = xblock() * BS + threadIdx.x;
 	double v = 0;
@@ -289,11 +299,14 @@ __global__ __launch_bounds__(BS) void k_cg_Ap(ggbk_grid g, const TZ *z, const T 
 			T pj = first ? (T) z[j] : (T) ((double) z[j] + beta * (double) p_in[j]);
 			sum += (double) w * (xi - (double) pj);
 		});
-		p_out[i] = pi;
-		Ap[i] = (T) sum;
+		if (act) {
+			p_out[i] = pi;
+			Ap[i] = (T) sum;
+		}
 		v = sum * xi;
 	}
-	store_partial(red, GGBK_SL_PAP, v);
+	if (act)
+		store_partial(red, GGBK_SL_PAP, v);
 }
 
 template <typename T>
@@ -562,19 +575,28 @@ extern "C" void ggbk_cg_start(ggbk_grid g, void *omega, const void *psi, void *r
 			    sub_mean, s, red);
 }
 
+template <typename T, typename TZ>
+static void launch_cg_Ap(ggbk_grid g, const void *z, const void *p_in, void *p_out, void *Ap, int first,
+			 const ggbk_scalars *s, ggbk_red red)
+{
+	if (first)
+		GGBK_LAUNCH((k_cg_Ap<T, TZ, true>), NB(g.n), BS, 0, g, (const TZ *) z, (const T *) p_in, (T *) p_out,
+			    (T *) Ap, s, red);
+	else
+		GGBK_LAUNCH((k_cg_Ap<T, TZ, false>), NB(g.n), BS, 0, g, (const TZ *) z, (const T *) p_in, (T *) p_out,
+			    (T *) Ap, s, red);
+}
+
 extern "C" void ggbk_cg_Ap(ggbk_grid g, const void *z, int TZ, const void *p_in, void *p_out, void *Ap, int T,
 			   int first, const ggbk_scalars *s, ggbk_red red)
 {
 	ggbk_probe_begin(GGBK_PROBE_CG_AP);
 	if (T == GGBK_F64 && TZ == GGBK_F64)
-		GGBK_LAUNCH((k_cg_Ap<double, double>), NB(g.n), BS, 0, g, (const double *) z, (const double *) p_in,
-			    (double *) p_out, (double *) Ap, first, s, red);
+		launch_cg_Ap<double, double>(g, z, p_in, p_out, Ap, first, s, red);
 	else if (T == GGBK_F64)
-		GGBK_LAUNCH((k_cg_Ap<double, float>), NB(g.n), BS, 0, g, (const float *) z, (const double *) p_in,
-			    (double *) p_out, (double *) Ap, first, s, red);
+		launch_cg_Ap<double, float>(g, z, p_in, p_out, Ap, first, s, red);
 	else
-		GGBK_LAUNCH((k_cg_Ap<float, float>), NB(g.n), BS, 0, g, (const float *) z, (const float *) p_in,
-			    (float *) p_out, (float *) Ap, first, s, red);
+		launch_cg_Ap<float, float>(g, z, p_in, p_out, Ap, first, s, red);
 	ggbk_probe_end(GGBK_PROBE_CG_AP);
 }
 

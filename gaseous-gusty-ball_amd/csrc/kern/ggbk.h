@@ -38,13 +38,14 @@ typedef struct {
 	                           * (self, 0); NULL when a cell has more than 6 neighbours */
 } ggbk_grid;
 
-/* bilinear prolongation from level l+1 to l, and its transpose */
+/* bilinear prolongation from level l+1 to l, and its transpose.  Both are stored per
+ * row as interleaved (index, weight bits) pairs, so a row is read with 16-byte loads. */
 typedef struct {
-	int nf, nc, rmax, pad_;
-	const int *ptap;   /* [4][nf] coarse taps of each fine cell */
-	const float *pw;   /* [4][nf] */
-	const int *ridx;   /* [rmax][nc] rows of R = P^T, fine index ascending, zero-weight padded */
-	const float *rw;   /* [rmax][nc] */
+	int nf, nc, rmax, rpad;  /* rpad: padded row width, 18 or GGBK_RMAX */
+	const int *pt;     /* [nf][4][2] coarse tap and weight of each fine cell */
+	const int *rr;     /* [rpad/2][nc][4] rows of R = P^T, two (fine index, weight bits) pairs
+	                    * per element, column-major over rows (coalesced across a wave); fine
+	                    * index ascending, zero-weight padded */
 } ggbk_xfer;
 #define GGBK_RMAX 24
 

@@ -95,29 +95,32 @@ __global__ __launch_bounds__(BS) void k_cg_update_down(ggbk_grid g, T *psi, cons
 }
 
 // coarse b = sum over the row of R of rw * r_fine (product in storage precision,
-// accumulated in double, fine-index order: fluid.c:130-136).  Rows are stored padded to
-// rmax with zero weights, so all loads of a row are independent.
-template <typename TM>
+// accumulated in double, fine-index order: fluid.c:130-136).  Rows are padded to RP
+// entries with zero weights, so every load is unconditional and independent (a bound
+// known only at run time makes the compiler wait for each load in turn).
+template <typename TM, int RP>
 __global__ __launch_bounds__(BS) void k_mg_restrict(ggbk_xfer xf, const TM *rf, TM *bc, const ggbk_scalars *s)
 {
 	int act = s->active;
 	int c = xblock() * BS + threadIdx.x;
 	if (c >= xf.nc)
 		return;
+	const int4 *rows = (const int4 *) xf.rr;
 	size_t nc = (size_t) xf.nc;
-	int idx[GGBK_RMAX];
-	float w[GGBK_RMAX];
+	int idx[RP];
+	float w[RP];
 #pragma unroll
-	for (int k = 0; k < GGBK_RMAX; k++)
-		if (k < xf.rmax) {
-			idx[k] = xf.ridx[k * nc + c];
-			w[k] = xf.rw[k * nc + c];
-		}
+	for (int k = 0; k < RP; k += 2) {
+		int4 v = rows[(k / 2) * nc + c];
+		idx[k] = v.x;
+		w[k] = __int_as_float(v.y);
+		idx[k + 1] = v.z;
+		w[k + 1] = __int_as_float(v.w);
+	}
 	double acc = 0;
 #pragma unroll
-	for (int k = 0; k < GGBK_RMAX; k++)
-		if (k < xf.rmax)
-			acc += (double) (w[k] * rf[idx[k]]);
+	for (int k = 0; k < RP; k++)
+		acc += (double) (w[k] * rf[idx[k]]);
 	if (act)
 		bc[c] = (TM) acc;
 }
@@ -127,9 +130,10 @@ __global__ __launch_bounds__(BS) void k_mg_restrict(ggbk_xfer xf, const TM *rf, 
 template <typename TM, typename TB>
 __device__ __forceinline__ TM xcorr_at(const ggbk_grid &g, const ggbk_xfer &xf, const TB *b, const TM *xc, int j)
 {
-	size_t nf = (size_t) xf.nf;
-	TM v = xf.pw[j] * xc[xf.ptap[j]] + xf.pw[nf + j] * xc[xf.ptap[nf + j]] +
-	       xf.pw[2 * nf + j] * xc[xf.ptap[2 * nf + j]] + xf.pw[3 * nf + j] * xc[xf.ptap[3 * nf + j]];
+	const int4 *q = (const int4 *) (xf.pt + 8 * (size_t) j);
+	int4 t01 = q[0], t23 = q[1];
+	TM v = __int_as_float(t01.y) * xc[t01.x] + __int_as_float(t01.w) * xc[t01.z] +
+	       __int_as_float(t23.y) * xc[t23.x] + __int_as_float(t23.w) * xc[t23.z];
 	return x1_at<TM, TB>(g, b, j) + (TM) (1.0 * (double) v);
 }
 
@@ -370,7 +374,10 @@ static void vcycle_level(const ggbk_level *lv, int nlev, int l, const TB *b, TM 
 		if (l == 0)
 			ggbk_probe_end(GGBK_PROBE_MG_DOWN0);
 	}
-	GGBK_LAUNCH((k_mg_restrict<TM>), NB(L.xf.nc), BS, 0, L.xf, (const TM *) r, (TM *) C.b, s);
+	if (L.xf.rpad == 18)
+		GGBK_LAUNCH((k_mg_restrict<TM, 18>), NB(L.xf.nc), BS, 0, L.xf, (const TM *) r, (TM *) C.b, s);
+	else
+		GGBK_LAUNCH((k_mg_restrict<TM, GGBK_RMAX>), NB(L.xf.nc), BS, 0, L.xf, (const TM *) r, (TM *) C.b, s);
 	ggbk_red none = { nullptr, 0 };
 	vcycle_level<TM, TM>(lv, nlev, l + 1, (const TM *) C.b, (TM *) C.x, s, none);
 	if (l == 0)
