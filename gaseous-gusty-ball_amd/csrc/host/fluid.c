@@ -134,8 +134,9 @@ ggbk_grid ggb_grid_dev(const CubeSphere *csc)
 	g.gc = grid_buf(gd, sizeof(float) * 3 * nk, gc);
 	g.ghost = grid_buf(gd, sizeof(int) * 6 * (size_t) (N + 2) * (N + 2), cs->ghost);
 	g.dinv = grid_buf(gd, sizeof(float) * (size_t) n, dinv);
-	if (maxnb <= 6) {
-		/* interleaved per-cell (neighbour, weight) pairs for the stencil passes */
+	{
+		/* interleaved per-cell (neighbour, weight) pairs of slots 0-5 for the stencil
+		 * passes; slots 6-7 stay in the column-major tables */
 		int *stn = malloc(sizeof(int) * 12 * (size_t) n);
 		if (!stn)
 			ggb_die("out of memory");

@@ -194,27 +194,24 @@ __device__ __forceinline__ dv3 sample_vec(const ggbk_grid &g, const T *fld3, dv3
 __device__ __forceinline__ dv3 cell_pos(const ggbk_grid &g, int i) { return ld3(g.pos + 3 * (size_t) i); }
 
 // Visits the Laplacian stencil of cell i in the reference's slot order: fn(j, w) for each
-// neighbour j with weight w.  With the interleaved per-cell copy (every grid whose degree
-// is at most 6) the six (j, w) pairs come in three 16-byte loads; the padding slots are
-// (i, 0) and add exact zeros after the real terms.  Otherwise the column-major tables
-// are read slot by slot.
+// neighbour j with weight w.  Slots 0-5 come from the interleaved per-cell copy in three
+// 16-byte loads (cells with fewer neighbours are padded with (i, 0), which add exact
+// zeros after the real terms); the few cells with 7 or 8 neighbours read the rest from
+// the column-major tables.
 template <typename Fn>
 __device__ __forceinline__ void for_stencil(const ggbk_grid &g, int i, Fn fn)
 {
-	if (g.stn) {
-		const int4 *p = (const int4 *) (g.stn + (size_t) i * 12);
-		int4 a = p[0], b = p[1], c = p[2];
-		fn(a.x, __int_as_float(a.y));
-		fn(a.z, __int_as_float(a.w));
-		fn(b.x, __int_as_float(b.y));
-		fn(b.z, __int_as_float(b.w));
-		fn(c.x, __int_as_float(c.y));
-		fn(c.z, __int_as_float(c.w));
-	} else {
-		int nn = g.nnb[i];
-		for (int k = 0; k < nn; k++)
-			fn(g.nbr[(size_t) k * g.n + i], g.w[(size_t) k * g.n + i]);
-	}
+	const int4 *p = (const int4 *) (g.stn + (size_t) i * 12);
+	int4 a = p[0], b = p[1], c = p[2];
+	int nn = g.maxnb > 6 ? (int) g.nnb[i] : 6;
+	fn(a.x, __int_as_float(a.y));
+	fn(a.z, __int_as_float(a.w));
+	fn(b.x, __int_as_float(b.y));
+	fn(b.z, __int_as_float(b.w));
+	fn(c.x, __int_as_float(c.y));
+	fn(c.z, __int_as_float(c.w));
+	for (int k = 6; k < nn; k++)
+		fn(g.nbr[(size_t) k * g.n + i], g.w[(size_t) k * g.n + i]);
 }
 
 // ---------------- deterministic reductions ----------------

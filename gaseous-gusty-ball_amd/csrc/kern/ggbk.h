@@ -34,8 +34,8 @@ typedef struct {
 	const float *gc;          /* [maxnb][3][n] */
 	const int *ghost;         /* [6][N+2][N+2] tap index of (f, i, j), i, j in [-1, N] */
 	const float *dinv;        /* [n] 1 / sum(w), rounded to float */
-	const int *stn;           /* [n][6][2] (neighbour, weight bits) per cell, padded with
-	                           * (self, 0); NULL when a cell has more than 6 neighbours */
+	const int *stn;           /* [n][6][2] (neighbour, weight bits) of slots 0-5 per cell,
+	                           * padded with (self, 0); slots 6-7 only in nbr/w */
 } ggbk_grid;
 
 /* bilinear prolongation from level l+1 to l, and its transpose.  Both are stored per
