@@ -107,13 +107,14 @@ __device__ __forceinline__ int home_tile(dv3 p, int R, int ntx)
 // Short arcs: every sample d(t) = normalize((1-t) a + t b) has face_dot between
 // min(da, db) and max(da, db) / cos(angle / 2), so with both endpoints at dc >= 0.45 the
 // arc projects onto face f as the straight segment between the endpoint pixels (great
-// circles are gnomonic lines), and with both below 0.45 no sample reaches dc = 0.5.
-// Otherwise the samples are walked.  The box is widened by 0.01 texel for rounding.
-#define SHORT_ARC 0.05
-__device__ __forceinline__ bool face_box(dv3 a, dv3 b, double ang, int &ns, int f, const SplatGeo &sg, int &xlo,
+// circles are gnomonic lines), and with both below 0.45 no sample reaches dc = 0.5
+// (both hold for any angle below 0.9 rad; "short" is a chord under 2 sin(0.025), an
+// angle under 0.05).  Otherwise the samples are walked.  The box is widened by 0.01
+// texel for rounding.
+#define SHORT_CHORD2 (4.0 * 0.024997395914712332 * 0.024997395914712332)
+__device__ __forceinline__ bool face_box(dv3 a, dv3 b, bool shortarc, int &ns, int f, const SplatGeo &sg, int &xlo,
 					 int &xhi, int &ylo, int &yhi)
 {
-	bool shortarc = ang < SHORT_ARC;
 	double da = face_dot(f, a), db = face_dot(f, b);
 	if (shortarc && da < 0.45 && db < 0.45)
 		return false;
@@ -133,7 +134,7 @@ __device__ __forceinline__ bool face_box(dv3 a, dv3 b, double ang, int &ns, int 
 		return xlo <= xhi && ylo <= yhi;
 	}
 	if (ns == 0)
-		ns = arc_samples(ang, sg.spacing);
+		ns = arc_samples(vangle(a, b), sg.spacing);
 	xlo = ylo = INT_MAX;
 	xhi = yhi = INT_MIN;
 	for (int k = 1; k <= ns; k++) {
@@ -154,25 +155,46 @@ __device__ __forceinline__ bool face_box(dv3 a, dv3 b, double ang, int &ns, int 
 // Adds one per participating lane to cnt[key] with one atomic per distinct key in the
 // wave and returns each participating lane's slot (previous count + rank among the lanes
 // with the same key, in lane order); -1 for lanes not participating.  Called by every
-// lane of the wave.
+// lane of the wave.  The distinct keys are found with ballots first; their leaders then
+// issue all the atomics in one instruction, so the wave waits for one round trip.
 __device__ __forceinline__ int wave_atomic_inc(int *cnt, int key, bool on)
 {
 	unsigned long long todo = __ballot(on);
 	int lane = __lane_id();
-	int slot = -1;
+	int leader_of = lane, rank = 0, add = 0;
 	while (todo) {
 		int leader = __ffsll((long long) todo) - 1;
 		int lk = __shfl(key, leader);
 		unsigned long long m = __ballot(on && key == lk) & todo;
-		int base = 0;
 		if (lane == leader)
-			base = atomicAdd(&cnt[lk], __popcll(m));
-		base = __shfl(base, leader);
-		if ((m >> lane) & 1ull)
-			slot = base + __popcll(m & ((1ull << lane) - 1ull));
+			add = __popcll(m);
+		if ((m >> lane) & 1ull) {
+			leader_of = leader;
+			rank = __popcll(m & ((1ull << lane) - 1ull));
+		}
 		todo &= ~m;
 	}
-	return slot;
+	int base = 0;
+	if (add > 0)
+		base = atomicAdd(&cnt[key], add);
+	base = __shfl(base, leader_of);
+	return on ? base + rank : -1;
+}
+
+// The same without the slots: the atomics return nothing, so the wave does not wait for
+// each of them in turn.
+__device__ __forceinline__ void wave_atomic_count(int *cnt, int key, bool on)
+{
+	unsigned long long todo = __ballot(on);
+	int lane = __lane_id();
+	while (todo) {
+		int leader = __ffsll((long long) todo) - 1;
+		int lk = __shfl(key, leader);
+		unsigned long long m = __ballot(on && key == lk) & todo;
+		if (lane == leader)
+			atomicAdd(&cnt[lk], __popcll(m));
+		todo &= ~m;
+	}
 }
 
 // Visits every tile of every face box of particle i's arc: fn(has, tile) is called by
@@ -181,11 +203,24 @@ __device__ __forceinline__ int wave_atomic_inc(int *cnt, int key, bool on)
 template <typename Fn>
 __device__ __forceinline__ void for_arc_tiles(bool on, dv3 a, dv3 b, const SplatGeo &sg, Fn fn)
 {
-	double ang = on ? vangle(a, b) : 0.0;
+	dv3 dab = vsub(a, b);
+	bool shortarc = vdot(dab, dab) < SHORT_CHORD2;
 	int ns = 0;
+	// faces this lane's arc can reach (a short arc only those with an endpoint at
+	// dc >= 0.45; a long one any), united over the wave: the loop visits only those
+	unsigned fmask = 0;
+	if (on) {
+		for (int f = 0; f < 6; f++)
+			if (!shortarc || face_dot(f, a) >= 0.45 || face_dot(f, b) >= 0.45)
+				fmask |= 1u << f;
+	}
+	for (int off = 32; off > 0; off >>= 1)
+		fmask |= (unsigned) __shfl_xor((int) fmask, off);
 	for (int f = 0; f < 6; f++) {
+		if (!((fmask >> f) & 1u))
+			continue;
 		int xlo = 0, xhi = -1, ylo = 0, yhi = -1;
-		bool any = on && face_box(a, b, ang, ns, f, sg, xlo, xhi, ylo, yhi);
+		bool any = on && face_box(a, b, shortarc, ns, f, sg, xlo, xhi, ylo, yhi);
 		int tx0 = xlo / TILE, ty0 = ylo / TILE;
 		int w = any ? xhi / TILE - tx0 + 1 : 0;
 		int nt = any ? w * (yhi / TILE - ty0 + 1) : 0;
@@ -207,8 +242,8 @@ __global__ __launch_bounds__(PBS) void k_render_count(ggbk_parts ps, SplatGeo sg
 		b = ld3(ps.pos + 3 * (size_t) i);
 	}
 	int home = on ? home_tile(b, sg.R, sg.ntx) : 0;
-	wave_atomic_inc(hcounts, home, on);
-	for_arc_tiles(on, a, b, sg, [&](bool has, int t) { wave_atomic_inc(counts, t, has); });
+	wave_atomic_count(hcounts, home, on);
+	for_arc_tiles(on, a, b, sg, [&](bool has, int t) { wave_atomic_count(counts, t, has); });
 }
 
 __global__ __launch_bounds__(PBS) void k_render_fill(ggbk_parts ps, ggbk_parts out, SplatGeo sg, const int *starts,
