@@ -1,0 +1,99 @@
+"""The full-size workload (N=512, 1,572,864 cells; 4,000,000 particles; faces 2048^2;
+forcing 0.5, rotation 10, CG tol 1e-8), checked through properties that do not depend on
+the grid size: the oracle is too slow at this size, so these stand in for it.
+
+  * every step's solve reaches the FP64 true relative residual 1e-8 (plus the gap between
+    the recurrence residual the loop tests and the true one);
+  * omega keeps zero area-weighted mean; u = p x grad(psi) is tangent to the sphere;
+  * the mixed-precision solver (FP32 V-cycle) and the FP64 one agree to about the
+    solver tolerance;
+  * particles stay on the unit sphere and prev is the previous position;
+  * the render is finite, deterministic, and its alpha channel stays in [0, 1 / g].
+"""
+import numpy as np
+import pytest
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+
+N, NP, R = 512, 4_000_000, 2048
+
+
+@pytest.fixture(scope="module")
+def grid(ggb, gpu):
+    return ggb.Grid(N)
+
+
+def make(ggb, g, precision, particles=True):
+    vel0, vband = g.initial_velocity(seed=1, octaves=4)
+    F = ggb.Fluid(g, forcing=0.5, rotation=10.0, cg_tol=1e-8, cg_maxit=500, precision=precision)
+    F.set_state(vel0, vband)
+    P = Rr = None
+    if particles:
+        P = ggb.Particles(NP)
+        P.color_from_image(ggb.strip_to_float(ggb.banded_strip(1)), 64, 1024)
+        Rr = ggb.Render(R, 1.0, 0.35)
+    return F, P, Rr
+
+
+def steps(ggb, g, F, P, Rr, n, check=None):
+    for _ in range(n):
+        dt = g.h / max(F.max_speed(), 1e-6)
+        if P is not None:
+            before = P.get()[0] if check else None
+            P.advect(g, F, dt)
+        F.step(dt)
+        if Rr is not None:
+            Rr.fade(0.02)
+            Rr.deposit(P)
+        if check:
+            check(F, P, before)
+
+
+def test_fullsize_invariants(ggb, grid):
+    g = grid
+    area = g.area.astype(np.float64)
+    p = g.pos.astype(np.float64).reshape(-1, 3)
+
+    def check(F, P, before):
+        assert 0 < F.cg_iters < 500
+        assert F.true_relres() <= 1.5e-8
+        om = F.get(ggb.FIELD_OMEGA)
+        assert abs(np.sum(area * om)) <= 1e-12 * np.sum(area * np.abs(om))
+        u = F.get(ggb.FIELD_VEL).reshape(-1, 3)
+        speed = np.linalg.norm(u, axis=1)
+        assert np.all(np.isfinite(speed))
+        assert np.max(np.abs(np.sum(u * p, axis=1))) <= 1e-6 * speed.max()
+        pos, prev = P.get()
+        r = np.linalg.norm(pos.reshape(-1, 3), axis=1)
+        assert np.max(np.abs(r - 1.0)) < 1e-12
+        assert np.array_equal(prev, before)
+
+    F, P, Rr = make(ggb, g, ggb.PREC_MIXED)
+    steps(ggb, g, F, P, Rr, 3, check)
+    buf = Rr.sync_host()
+    assert np.all(np.isfinite(buf))
+    alpha = buf.reshape(-1, 4)[:, 3]
+    assert alpha.min() >= 0 and alpha.max() <= 1.0 / Rr.g + 1e-6
+    assert Rr.last_entries() >= NP
+    first = buf.copy()
+    F.free(), P.free(), Rr.free()
+    # the same run again: bit for bit
+    F, P, Rr = make(ggb, g, ggb.PREC_MIXED)
+    steps(ggb, g, F, P, Rr, 3)
+    assert np.array_equal(first.view(np.uint32), Rr.sync_host().view(np.uint32))
+
+
+def test_fullsize_mixed_matches_fp64(ggb, grid):
+    g = grid
+    res = []
+    for prec in (ggb.PREC_FP64, ggb.PREC_MIXED):
+        F, _, _ = make(ggb, g, prec, particles=False)
+        steps(ggb, g, F, None, None, 3)
+        res.append((F.get(ggb.FIELD_OMEGA), F.get(ggb.FIELD_PSI), F.get(ggb.FIELD_VEL), F.true_relres()))
+        F.free()
+    for k in range(3):
+        a, b = res[0][k], res[1][k]
+        err = np.linalg.norm(a - b) / np.linalg.norm(a)
+        print("field %d: mixed vs FP64 relative difference %.2e" % (k, err))
+        assert err < 1e-6
+    assert res[1][3] <= 1.5e-8 and res[0][3] <= 1.5e-8
