@@ -43,16 +43,19 @@ HBM_PEAK = 8.0e12
 # stored float tables.  kbar = mean Voronoi degree, n = cells, P = particles,
 # E = (particle, tile) entries, T = texels touched, Ts/Tm = state / multigrid word.
 PROBES = {
-    "k_render_draw": 1, "render_binning": 2, "mg_up(level 0)": 3, "k_mg_down(level 0)": 4,
+    "k_render_draw": 1, "render_count": 2, "mg_up(level 0)": 3, "k_mg_down(level 0)": 4,
     "k_cg_Ap": 5, "k_cg_update_down": 6, "k_particles_advect": 7, "k_departures": 8,
+    "k_render_fill": 9,
 }
 
 
 def kernel_bytes(name, n, kbar, P, E, T, Ts, Tm, Tz):
     if name == "k_render_draw":      # texels read+written, list entry, FP64 endpoints, colour, index
         return 32 * T + E * (4 + 48 + 16 + 4)
-    if name == "render_binning":     # count: endpoints; fill: particle read + reordered copy, list
-        return P * (48 + 68 + 68) + E * 4
+    if name == "render_count":       # FP64 endpoints (the tile counters stay in L2)
+        return P * 48
+    if name == "k_render_fill":      # particle read + reordered copy (68 B each), sample count, list
+        return P * (68 + 68 + 2) + E * 4
     if name == "mg_up(level 0)":     # b, dinv, nnb, stencil, taps+weights, coarse x, z out
         return n * (Ts + 4 + 1 + 8 * kbar + 32 + Tm / 4 + Tm)
     if name == "k_mg_down(level 0)":  # b, dinv, nnb, stencil, r1 out
@@ -231,10 +234,13 @@ def main():
     timers = {k: ggb.Timer() for k in "pfr"}
     acc = dict(p=0.0, f=0.0, r=0.0, its=0, entries=0)
     L = ggb.lib()
-    # warmup with every probe on: per-kernel device time per step, to pick the dominant
+    # warmup: one plain step (first launches load code objects), then the rest with every
+    # probe on: per-kernel device time per step, to pick the dominant kernel
+    nplain = 1 if args.warmup > 1 else 0
+    run_steps(g, F, P, R, nplain, timers, dict(acc))
     for pid in PROBES.values():
         L.ggbk_probe_enable(pid, 1)
-    run_steps(g, F, P, R, args.warmup, timers, dict(acc))
+    run_steps(g, F, P, R, args.warmup - nplain, timers, dict(acc))
     warm = {}
     for name, pid in PROBES.items():
         ms = C.c_double()
@@ -285,7 +291,7 @@ def main():
     for name, (cnt, ms) in warm.items():
         if cnt:
             b = kernel_bytes(name, g.n, kbar, cfg["P"], E, T, Ts, Tm, Tm)
-            kernels[name] = {"launches_per_step": round(cnt / max(args.warmup, 1), 2),
+            kernels[name] = {"launches_per_step": round(cnt / max(args.warmup - nplain, 1), 2),
                              "avg_us": round(ms * 1000.0 / cnt, 2),
                              "algo_bytes": int(b),
                              "achieved_GBps": round(b / (ms / 1000.0 / cnt) / 1e9, 1)}

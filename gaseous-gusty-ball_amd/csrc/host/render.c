@@ -73,6 +73,8 @@ int render_init(Render *r, int R, float radius, float alpha)
 	k->scan_part = ggbk_malloc(sizeof(long long) * 3 * (size_t) ggbk_render_scan_blocks(k->ntiles));
 	k->cap = 0;
 	k->list = NULL;
+	k->nsamp = NULL;
+	k->nsamp_cap = 0;
 	k->sort_scratch = NULL;
 	if (!k->buf || !k->tile_count || !k->tile_start || !k->tile_cursor || !k->home_count || !k->home_start ||
 	    !k->home_cursor || !k->total || !k->tile_diag ||
@@ -104,6 +106,7 @@ void render_free(Render *r)
 		ggbk_free(D->k.tile_diag);
 		ggbk_free(D->k.scan_part);
 		ggbk_free(D->k.list);
+		ggbk_free(D->k.nsamp);
 		ggbk_free(D->k.sort_scratch);
 		pthread_mutex_destroy(&D->mu);
 		free(D);

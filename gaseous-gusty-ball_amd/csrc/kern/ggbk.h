@@ -89,13 +89,14 @@ void *ggbk_stream_handle(void);
 /* ---- launch probes (HIP events around selected kernels, for measurement) ---- */
 enum {
 	GGBK_PROBE_RENDER_DRAW = 1,
-	GGBK_PROBE_RENDER_BIN = 2,   /* count + scan + fill */
+	GGBK_PROBE_RENDER_COUNT = 2, /* binning: count + scan */
 	GGBK_PROBE_MG_UP0 = 3,       /* level-0 prolongation + post-smooth */
 	GGBK_PROBE_MG_DOWN0 = 4,     /* level-0 pre-smooth residual */
 	GGBK_PROBE_CG_AP = 5,
 	GGBK_PROBE_CG_UPDATE = 6,
 	GGBK_PROBE_PARTICLES = 7,
 	GGBK_PROBE_DEPARTURES = 8,
+	GGBK_PROBE_RENDER_FILL = 9,  /* binning: list fill + reordered particle copy */
 };
 void ggbk_probe_enable(int id, int on);
 int ggbk_probe_collect(int id, double *total_ms);
@@ -245,6 +246,8 @@ typedef struct {
 	int *home_start;            /* [ntiles + 1] */
 	int *home_cursor;           /* [ntiles] */
 	int *list;                  /* [cap] particle slots per tile */
+	unsigned short *nsamp;      /* [nsamp_cap] sample count of each particle slot (filled by the binning) */
+	long long nsamp_cap;
 	long long cap;
 	unsigned long long *sort_scratch; /* [scratch_cap] for tiles whose list exceeds the LDS sort */
 	long long scratch_cap;

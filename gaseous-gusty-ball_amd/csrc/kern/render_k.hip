@@ -248,7 +248,7 @@ __global__ __launch_bounds__(PBS) void k_render_count(ggbk_parts ps, SplatGeo sg
 
 __global__ __launch_bounds__(PBS) void k_render_fill(ggbk_parts ps, ggbk_parts out, SplatGeo sg, const int *starts,
 						     int *cursor, const int *hstarts, int *hcursor, int *list,
-						     long long cap)
+						     long long cap, unsigned short *nsamp)
 {
 	int i = xblock() * PBS + threadIdx.x;
 	bool on = i < ps.n;
@@ -257,6 +257,10 @@ __global__ __launch_bounds__(PBS) void k_render_fill(ggbk_parts ps, ggbk_parts o
 		a = ld3(ps.prev + 3 * (size_t) i);
 		b = ld3(ps.pos + 3 * (size_t) i);
 	}
+	// the arc's sample count (render.c:179-181), for the draw: it needs the exact angle,
+	// whose FP64 atan2 would double the draw kernel's registers
+	if (on)
+		nsamp[i] = (unsigned short) arc_samples(vangle(a, b), sg.spacing);
 	// the particle's place in the next storage order
 	int home = on ? home_tile(b, sg.R, sg.ntx) : 0;
 	int hr = wave_atomic_inc(hcursor, home, on);
@@ -443,6 +447,19 @@ __device__ __forceinline__ void splat_texels(const TileRect &tr, float px, float
 	}
 }
 
+// Whether local texel t is one of splat_texels' texels for the splat at (px, py).
+__device__ __forceinline__ bool texel_in_splat(const TileRect &tr, int t, float px, float py, float rad, float invrad,
+					       int R)
+{
+	int xlo, xhi, ylo, yhi;
+	if (!tile_clip(tr, px, py, rad, R, xlo, xhi, ylo, yhi))
+		return false;
+	int x = tr.x0 + t % TILE, y = tr.y0 + t / TILE;
+	if (x < xlo || x > xhi || y < ylo || y > yhi)
+		return false;
+	return 1.0f - fabsf((float) y - py) * invrad > 0 && 1.0f - fabsf((float) x - px) * invrad > 0;
+}
+
 // One texel's blend of one splat (render.c:145-150).
 __device__ __forceinline__ void blend_texel(float4 &dd, int t, const TileRect &tr, float px, float py, float invrad,
 					    float w0, float4 c, float invg)
@@ -489,10 +506,15 @@ __device__ __forceinline__ void sort_net(unsigned *k)
 //   D  one thread per texel: sort its fragments by key and blend them in that order.
 // A range in which a texel collects more than CAPT fragments is retried with half the
 // particles; a single particle that still overflows is blended in order by one thread.
-#define SCAP 512         // on-tile samples per range (slot fits 10 bits)
+// The sample counts come from the binning (an FP64 atan2 here would double the
+// registers), each thread keeps its two texels in registers, and the LDS arrays (20.4 KB,
+// allocated in 1280-byte granules) leave room for 8 workgroups per CU, which is also the
+// register limit.
+#define SCAP 288         // on-tile samples per range (slot fits 10 bits)
 __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts ps, SplatGeo sg, float invg)
 {
-	__shared__ float4 tex[NTEX];
+	static_assert(NTEX % DBS == 0, "each thread owns NTEX / DBS texels");
+	constexpr int TPT = NTEX / DBS;
 	__shared__ int tcnt[NTEX];
 	__shared__ unsigned short frag[CAPT][NTEX];
 	__shared__ float2 sxy[SCAP];
@@ -500,7 +522,6 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 	__shared__ double pa[3][WIN], pb[3][WIN];  // by rank
 	__shared__ int pns[WIN], pst[WIN + 1], ppid[WIN];
 	__shared__ float4 pcol[WIN];               // by rank
-	__shared__ unsigned char touched[NTEX];
 	__shared__ int overflow, nsamp, wtot, ntouched;
 	long long c0 = clock64();
 	int t = xblock();
@@ -523,12 +544,17 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 	const int *lst = r.list + r.tile_start[t];
 	const float rad = sg.rad, invrad = 1.0f / rad;
 
+	// thread lid owns texels lid + h DBS and keeps them in registers
 	float4 *gbuf = (float4 *) r.buf;
-	for (int i = lid; i < NTEX; i += DBS) {
+	float4 tex[TPT];
+	unsigned touched = 0;
+#pragma unroll
+	for (int h = 0; h < TPT; h++) {
+		int i = lid + h * DBS;
 		int ly = i / TILE, lx = i % TILE;
+		tex[h] = make_float4(0, 0, 0, 0);
 		if (ly < tr.h && lx < tr.w)
-			tex[i] = gbuf[((size_t) tr.f * r.R + tr.y0 + ly) * r.R + tr.x0 + lx];
-		touched[i] = 0;
+			tex[h] = gbuf[((size_t) tr.f * r.R + tr.y0 + ly) * r.R + tr.x0 + lx];
 	}
 	if (lid == 0)
 		ntouched = 0;
@@ -555,7 +581,7 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 	}
 	__syncthreads();
 
-	long long csamp = 0, cblend = 0;
+	long long csamp = 0, cblend = 0, cpre = 0;
 	int windows = 0, serial_windows = 0;
 	for (int base = 0; base < L; base += WIN) {
 		int W = min(WIN, L - base);
@@ -568,7 +594,7 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 			int slot = is_sorted ? (int) (unsigned) sorted[base + lid] : lst[lid];
 			a = ld3(ps.prev + 3 * (size_t) slot);
 			b = ld3(ps.pos + 3 * (size_t) slot);
-			myns = arc_samples(vangle(a, b), sg.spacing);
+			myns = r.nsamp[slot];
 			mypid = ps.pid[slot];
 			mycol = *(const float4 *) (ps.col + 4 * (size_t) slot);
 			ppid[lid] = mypid;
@@ -601,6 +627,7 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 			pst[W] = pst[W - 1] + pns[W - 1];
 		__syncthreads();
 		csamp += clock64() - ca;
+		cpre += clock64() - ca;
 
 		for (int r0 = 0; r0 < W;) {
 			long long cc = clock64();
@@ -666,11 +693,13 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 				csamp += cb - cc;
 				if (!overflow) {
 					// D: each texel sorts its fragments (key << 10 | slot) and blends
-					for (int tx = lid; tx < NTEX; tx += DBS) {
+#pragma unroll
+					for (int h = 0; h < TPT; h++) {
+						int tx = lid + h * DBS;
 						int n = tcnt[tx];
 						if (n == 0)
 							continue;
-						touched[tx] = 1;
+						touched |= 1u << h;
 						unsigned keys[CAPT];
 #pragma unroll
 						for (int q = 0; q < CAPT; q++) {
@@ -683,11 +712,11 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 						}
 						if (n <= 4)
 							sort_net<4>(keys);
-						else if (n <= 8)
+						else if (CAPT > 8 && n <= 8)
 							sort_net<8>(keys);
 						else
 							sort_net<CAPT>(keys);
-						float4 dd = tex[tx];
+						float4 dd = tex[h];
 						for (int q = 0; q < n; q++) {
 							unsigned sl = keys[q] & 1023u;
 							unsigned rk = keys[q] >> 22;
@@ -695,29 +724,32 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 							float4 c = pcol[rk];
 							blend_texel(dd, tx, tr, s2.x, s2.y, invrad, r.alpha * c.w, c, invg);
 						}
-						tex[tx] = dd;
+						tex[h] = dd;
 					}
 					retry = false;
 					r0 = r1;
 				} else if (r1 - r0 > 1) {
 					r1 = r0 + (r1 - r0) / 2;
 				} else {
-					// one particle that alone overflows: one thread blends it in order
+					// one particle that alone overflows: every thread walks its samples
+					// in order and blends those that cover its own texels
 					serial_windows++;
-					if (lid == 0) {
-						dv3 pa3 = d3(pa[0][r0], pa[1][r0], pa[2][r0]);
-						dv3 pb3 = d3(pb[0][r0], pb[1][r0], pb[2][r0]);
-						int ns = pns[r0];
-						float4 c = pcol[r0];
-						for (int k = 1; k <= ns; k++) {
-							float px, py;
-							if (!face_pixel(tr.f, arc_point(pa3, pb3, k, ns), sg.R, px, py))
-								continue;
-							splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
-								touched[tx] = 1;
-								blend_texel(tex[tx], tx, tr, px, py, invrad, r.alpha * c.w, c,
+					dv3 pa3 = d3(pa[0][r0], pa[1][r0], pa[2][r0]);
+					dv3 pb3 = d3(pb[0][r0], pb[1][r0], pb[2][r0]);
+					int ns = pns[r0];
+					float4 c = pcol[r0];
+					for (int k = 1; k <= ns; k++) {
+						float px, py;
+						if (!face_pixel(tr.f, arc_point(pa3, pb3, k, ns), sg.R, px, py))
+							continue;
+#pragma unroll
+						for (int h = 0; h < TPT; h++) {
+							int tx = lid + h * DBS;
+							if (texel_in_splat(tr, tx, px, py, rad, invrad, sg.R)) {
+								touched |= 1u << h;
+								blend_texel(tex[h], tx, tr, px, py, invrad, r.alpha * c.w, c,
 									    invg);
-							});
+							}
 						}
 					}
 					retry = false;
@@ -731,18 +763,20 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 
 	// write back the texels a splat touched (NTEX is a multiple of DBS: whole waves)
 	int cnt = 0;
-	for (int i = lid; i < NTEX; i += DBS) {
+#pragma unroll
+	for (int h = 0; h < TPT; h++) {
+		int i = lid + h * DBS;
 		int ly = i / TILE, lx = i % TILE;
-		bool w = touched[i] != 0;
+		bool w = (touched >> h) & 1u;
 		if (w)
-			gbuf[((size_t) tr.f * r.R + tr.y0 + ly) * r.R + tr.x0 + lx] = tex[i];
+			gbuf[((size_t) tr.f * r.R + tr.y0 + ly) * r.R + tr.x0 + lx] = tex[h];
 		cnt += __popcll(__ballot(w));
 	}
 	if (__lane_id() == 0)
 		atomicAdd(&ntouched, cnt);
 	__syncthreads();
 	// this tile's diagnostics (summed over tiles on request): touched texels, ranges,
-	// serial ranges, sorted, sample / blend / total cycles in units of 1024
+	// serial ranges, sorted, sample / blend / total / window set-up cycles in units of 1024
 	if (lid == 0) {
 		diag[0] = ntouched;
 		diag[1] = windows;
@@ -751,17 +785,18 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 		diag[4] = (int) (csamp >> 10);
 		diag[5] = (int) (cblend >> 10);
 		diag[6] = (int) ((clock64() - c0) >> 10);
-		diag[7] = L;
+		diag[7] = (int) (cpre >> 10);
 	}
 }
 
-// Sums the per-tile diagnostics into stats[2..8] (one workgroup):
-// [2] windows, [3] serial windows, [4] sorted tiles, [5] sample cycles, [6] blend cycles,
-// [7] total cycles, [8] touched texels.
+// Sums the per-tile diagnostics into stats[2..9] (one workgroup):
+// [2] ranges, [3] serial ranges, [4] sorted tiles, [5] sample cycles (window set-up and
+// samples), [6] blend cycles, [7] total cycles, [8] touched texels, [9] window set-up
+// cycles (particle loads, ranks, prefix).
 __global__ __launch_bounds__(SCAN_BS) void k_render_diag_sum(int ntiles, const int *tile_diag, long long *stats)
 {
-	__shared__ long long part[7][SCAN_BS / 64];
-	long long acc[7] = { 0, 0, 0, 0, 0, 0, 0 };
+	__shared__ long long part[8][SCAN_BS / 64];
+	long long acc[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
 	for (int t = threadIdx.x; t < ntiles; t += SCAN_BS) {
 		const int *d = tile_diag + 8 * (size_t) t;
 		acc[0] += d[1];
@@ -771,9 +806,10 @@ __global__ __launch_bounds__(SCAN_BS) void k_render_diag_sum(int ntiles, const i
 		acc[4] += (long long) d[5] << 10;
 		acc[5] += (long long) d[6] << 10;
 		acc[6] += d[0];
+		acc[7] += (long long) d[7] << 10;
 	}
 	int w = threadIdx.x / 64;
-	for (int k = 0; k < 7; k++) {
+	for (int k = 0; k < 8; k++) {
 		long long v = acc[k];
 		for (int off = 32; off > 0; off >>= 1)
 			v += __shfl_down(v, off);
@@ -781,7 +817,7 @@ __global__ __launch_bounds__(SCAN_BS) void k_render_diag_sum(int ntiles, const i
 			part[k][w] = v;
 	}
 	__syncthreads();
-	if (threadIdx.x < 7) {
+	if (threadIdx.x < 8) {
 		long long v = 0;
 		for (int i = 0; i < SCAN_BS / 64; i++)
 			v += part[threadIdx.x][i];
@@ -831,7 +867,7 @@ extern "C" long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_par
 	sg.rad = r->radius;
 	sg.spacing = 0.7 / r->R;
 	unsigned pb = ggbk_blocks(in.n, PBS);
-	int pr_bin = ggbk_probe_begin(GGBK_PROBE_RENDER_BIN);
+	ggbk_probe_begin(GGBK_PROBE_RENDER_COUNT);
 	if (ggbk_zero(r->tile_count, sizeof(int) * (size_t) r->ntiles) ||
 	    ggbk_zero(r->home_count, sizeof(int) * (size_t) r->ntiles))
 		return -1;
@@ -841,7 +877,7 @@ extern "C" long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_par
 		    r->tile_cursor, (const int *) r->home_count, r->home_start, r->home_cursor, r->scan_part);
 	GGBK_LAUNCH(k_render_scan_fix, sb, SCAN_BS, 0, r->ntiles, r->tile_start, r->home_start,
 		    (const long long *) r->scan_part, r->total);
-	ggbk_probe_end(GGBK_PROBE_RENDER_BIN);
+	ggbk_probe_end(GGBK_PROBE_RENDER_COUNT);
 	long long stats[2] = { 0, 0 };
 	if (ggbk_d2h(stats, r->total, sizeof(stats)))
 		return -1;
@@ -858,11 +894,17 @@ extern "C" long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_par
 		r->cap = ncap;
 		r->scratch_cap = 2 * ncap;
 	}
-	if (pr_bin)
-		ggbk_probe_begin(GGBK_PROBE_RENDER_BIN);
+	if (in.n > r->nsamp_cap) {
+		ggbk_free(r->nsamp);
+		r->nsamp = (unsigned short *) ggbk_malloc(sizeof(unsigned short) * (size_t) in.n);
+		if (!r->nsamp)
+			return -1;
+		r->nsamp_cap = in.n;
+	}
+	ggbk_probe_begin(GGBK_PROBE_RENDER_FILL);
 	GGBK_LAUNCH(k_render_fill, pb, PBS, 0, in, out, sg, (const int *) r->tile_start, r->tile_cursor,
-		    (const int *) r->home_start, r->home_cursor, r->list, r->cap);
-	ggbk_probe_end(GGBK_PROBE_RENDER_BIN);
+		    (const int *) r->home_start, r->home_cursor, r->list, r->cap, r->nsamp);
+	ggbk_probe_end(GGBK_PROBE_RENDER_FILL);
 	ggbk_probe_begin(GGBK_PROBE_RENDER_DRAW);
 	GGBK_LAUNCH(k_render_draw, (unsigned) r->ntiles, DBS, 0, *r, in, sg, invg);
 	ggbk_probe_end(GGBK_PROBE_RENDER_DRAW);

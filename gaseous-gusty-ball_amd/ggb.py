@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libggb.so")
+# GGB_LIB selects another build of the library (for comparing build variants)
+LIB_PATH = os.environ.get("GGB_LIB") or os.path.join(HERE, "lib", "libggb.so")
 
 PREC_FP64, PREC_MIXED, PREC_REF32 = 0, 1, 2
 FIELD_OMEGA, FIELD_OMEGA_BAND, FIELD_PSI, FIELD_VEL, FIELD_DEP_B, FIELD_DEP_F = range(6)

@@ -63,8 +63,9 @@ void particles_set_round32(Particles *P, int on);
 void render_sync_host(Render *r);             /* device buffer -> r->buf mirror */
 long long render_last_entries(const Render *r); /* (particle, tile) entries of the last deposit */
 /* diagnostics of the last deposit (16 values): [0] (particle, tile) entries, [1] longest
- * tile list, [2] windows, [3] serial windows, [4] tiles sorted by particle index, [5..7]
- * draw-pass cycle sums (samples, blend, total), [8] texels touched */
+ * tile list, [2] sample ranges, [3] serial ranges, [4] tiles sorted by particle index,
+ * [5..7] draw-pass cycle sums (samples, blend, total), [8] texels touched, [9] cycles of
+ * the window set-up (part of [5]) */
 int render_last_stats(const Render *r, long long *out16);
 /* 8-bit faces [6][R][R][4] (bg NULL: un-premultiplied RGBA) computed on the host mirror. */
 void render_faces8(Render *r, const float *bg, unsigned char *out);

@@ -209,3 +209,42 @@ def test_ref32_matches_reference(ggb, orc, ref, gpu):
     F.sync_host()
     assert rel(F.omega, RF.omega) < 1e-5
     assert rel(F.psi, RF.psi) < 1e-5
+
+
+def test_render_crowded_and_antipodal(ggb, orc, gpu):
+    """Tiles holding several windows of particles (index-sorted windows, sample ranges
+    split when a texel overflows) and particles whose endpoints are antipodal (every
+    sample of the arc falls on one of two points, so one particle alone overflows a texel
+    and is blended by the in-order fallback), against the oracle's serial blend."""
+    R, n, ncl, nanti = 128, 3000, 2000, 24
+    rng = np.random.default_rng(11)
+    c = np.array([0.3, 0.2, 0.93])
+    pos = rng.standard_normal((n, 3))
+    pos[:ncl] = c / np.linalg.norm(c) + 0.01 * rng.standard_normal((ncl, 3))
+    pos /= np.linalg.norm(pos, axis=1, keepdims=True)
+    prev = pos + 0.02 * rng.standard_normal((n, 3))
+    prev /= np.linalg.norm(prev, axis=1, keepdims=True)
+    prev[ncl:ncl + nanti] = -pos[ncl:ncl + nanti]
+    prev[:4] = -pos[:4]
+    P = ggb.Particles(n)
+    P.set(pos, prev)
+    OP = orc.OracleParticles("f64", n)
+    OP.pos[:] = pos.ravel()
+    OP.prev[:] = prev.ravel()
+    img = ggb.strip_to_float(ggb.banded_strip(5))
+    P.color_from_image(img, 64, 1024)
+    OP.color_from_image(img, 64, 1024)
+    Rr = ggb.Render(R)
+    OR = orc.OracleRender("f64", R)
+    for _ in range(2):
+        Rr.fade(0.02)
+        OR.fade(0.02)
+        Rr.deposit(P)
+        OR.deposit(OP)
+        st = Rr.last_stats()
+        assert st[3] > 0, "no range took the in-order fallback"
+        assert st[4] > 0, "no tile was cut into index-sorted windows"
+    buf = Rr.sync_host()
+    assert np.max(np.abs(buf - OR.buf)) < 1e-5
+    a, b = Rr.faces8().astype(int), OR.faces8().astype(int)
+    assert np.max(np.abs(a - b)) <= 1
