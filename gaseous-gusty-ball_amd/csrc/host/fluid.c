@@ -11,6 +11,7 @@
 #include <math.h>
 #include <pthread.h>
 #include <stdarg.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -77,6 +78,111 @@ static void *grid_buf(struct ggb_grid_dev *gd, size_t bytes, const void *host)
 	return p;
 }
 
+/* Stencil of the Laplacian passes in compressed form.  The Voronoi weights repeat under
+ * the cube's symmetries (cells whose weight tuples, in slot order, are bitwise equal: about
+ * 47 cells per tuple), so each distinct tuple is stored once as a class row of 8 floats
+ * (zero-padded) that stays in cache.  Per cell: the class, and the neighbours as 4-bit
+ * codes (dy + 1) << 2 | (dx + 1) of their index offsets dy N + dx, dy, dx in {-1, 0, 1}
+ * (self, code 5, is the padding); a cell with any other offset (across a face seam) has
+ * its class tagged in bit 31 and an explicit row of 8 indices (self-padded) instead. */
+static int offset_code(int off, int N)
+{
+	for (int dy = -1; dy <= 1; dy++)
+		for (int dx = -1; dx <= 1; dx++)
+			if (off == dy * N + dx)
+				return (dy + 1) << 2 | (dx + 1);
+	return -1;
+}
+
+static uint64_t tuple_hash(const uint32_t *v, int m)
+{
+	uint64_t h = 0x9e3779b97f4a7c15ull ^ (uint64_t) m;
+	for (int k = 0; k < m; k++) {
+		h ^= v[k];
+		h *= 0xff51afd7ed558ccdull;
+		h ^= h >> 32;
+	}
+	return h;
+}
+
+int ggb_stencil_compress(const CubeSphere *cs, int *rec, float *cw, int *xn, int *ncls_out, int *nirr_out)
+{
+	int n = cs->ncells, N = cs->N;
+	size_t hcap = 16;
+	while (hcap < 2 * (size_t) n)
+		hcap <<= 1;
+	int *htab = malloc(sizeof(int) * hcap);
+	if (!htab)
+		return -1;
+	memset(htab, 0xff, sizeof(int) * hcap);
+	int ncls = 0, nirr = 0;
+	for (int i = 0; i < n; i++) {
+		uint32_t key[9];
+		key[0] = cs->nnb[i];
+		for (int k = 0; k < 8; k++) {
+			float wv = k < cs->nnb[i] ? cs->w[(size_t) CS_MAXNB * i + k] : 0.0f;
+			memcpy(&key[1 + k], &wv, sizeof(float));
+		}
+		size_t hpos = tuple_hash(key, 9) & (hcap - 1);
+		int cls;
+		for (;;) {
+			cls = htab[hpos];
+			if (cls < 0) {
+				cls = ncls++;
+				memcpy(&cw[8 * (size_t) cls], &key[1], 8 * sizeof(float));
+				htab[hpos] = cls;
+				break;
+			}
+			if (!memcmp(&cw[8 * (size_t) cls], &key[1], 8 * sizeof(float)))
+				break;
+			hpos = (hpos + 1) & (hcap - 1);
+		}
+		unsigned code = 0;
+		int regular = 1;
+		for (int k = cs->nnb[i]; k < 8; k++)
+			code |= 5u << (4 * k);
+		for (int k = 0; k < cs->nnb[i] && regular; k++) {
+			int c = offset_code(cs->nbr[(size_t) CS_MAXNB * i + k] - i, N);
+			if (c < 0)
+				regular = 0;
+			else
+				code |= (unsigned) c << (4 * k);
+		}
+		if (regular) {
+			rec[2 * (size_t) i] = cls;
+			rec[2 * (size_t) i + 1] = (int) code;
+		} else {
+			for (int k = 0; k < 8; k++)
+				xn[8 * (size_t) nirr + k] = k < cs->nnb[i] ? cs->nbr[(size_t) CS_MAXNB * i + k] : i;
+			rec[2 * (size_t) i] = (int) (0x80000000u | (unsigned) cls);
+			rec[2 * (size_t) i + 1] = nirr++;
+		}
+	}
+	free(htab);
+	*ncls_out = ncls;
+	*nirr_out = nirr;
+	return 0;
+}
+
+static void build_stencil_classes(struct ggb_grid_dev *gd, ggbk_grid *g, const CubeSphere *cs)
+{
+	int n = cs->ncells;
+	float *cw = malloc(sizeof(float) * 8 * (size_t) n); /* class rows (at most n) */
+	int *rec = malloc(sizeof(int) * 2 * (size_t) n);
+	int *xn = malloc(sizeof(int) * 8 * (size_t) n);
+	int ncls, nirr;
+	if (!cw || !rec || !xn || ggb_stencil_compress(cs, rec, cw, xn, &ncls, &nirr))
+		ggb_die("out of memory");
+	g->srec = grid_buf(gd, sizeof(int) * 2 * (size_t) n, rec);
+	g->scw = grid_buf(gd, sizeof(float) * 8 * (size_t) ncls, cw);
+	g->sxn = grid_buf(gd, sizeof(int) * 8 * (size_t) (nirr > 0 ? nirr : 1), nirr > 0 ? xn : NULL);
+	g->ncls = ncls;
+	g->nirr = nirr;
+	free(cw);
+	free(rec);
+	free(xn);
+}
+
 ggbk_grid ggb_grid_dev(const CubeSphere *csc)
 {
 	CubeSphere *cs = (CubeSphere *) csc; /* the device cache is library-managed */
@@ -134,23 +240,7 @@ ggbk_grid ggb_grid_dev(const CubeSphere *csc)
 	g.gc = grid_buf(gd, sizeof(float) * 3 * nk, gc);
 	g.ghost = grid_buf(gd, sizeof(int) * 6 * (size_t) (N + 2) * (N + 2), cs->ghost);
 	g.dinv = grid_buf(gd, sizeof(float) * (size_t) n, dinv);
-	{
-		/* interleaved per-cell (neighbour, weight) pairs of slots 0-5 for the stencil
-		 * passes; slots 6-7 stay in the column-major tables */
-		int *stn = malloc(sizeof(int) * 12 * (size_t) n);
-		if (!stn)
-			ggb_die("out of memory");
-		for (int i = 0; i < n; i++)
-			for (int k = 0; k < 6; k++) {
-				int in = k < cs->nnb[i];
-				size_t src = (size_t) CS_MAXNB * i + k;
-				float wv = in ? cs->w[src] : 0.0f;
-				stn[12 * (size_t) i + 2 * k] = in ? cs->nbr[src] : i;
-				memcpy(&stn[12 * (size_t) i + 2 * k + 1], &wv, sizeof(float));
-			}
-		g.stn = grid_buf(gd, sizeof(int) * 12 * (size_t) n, stn);
-		free(stn);
-	}
+	build_stencil_classes(gd, &g, cs);
 	free(nbr);
 	free(w);
 	free(el);

@@ -13,7 +13,7 @@
  * released by cs_free. */
 struct ggb_grid_dev {
 	ggbk_grid g;
-	void *bufs[11];
+	void *bufs[13];
 	int nbufs;
 };
 

@@ -194,24 +194,44 @@ __device__ __forceinline__ dv3 sample_vec(const ggbk_grid &g, const T *fld3, dv3
 __device__ __forceinline__ dv3 cell_pos(const ggbk_grid &g, int i) { return ld3(g.pos + 3 * (size_t) i); }
 
 // Visits the Laplacian stencil of cell i in the reference's slot order: fn(j, w) for each
-// neighbour j with weight w.  Slots 0-5 come from the interleaved per-cell copy in three
-// 16-byte loads (cells with fewer neighbours are padded with (i, 0), which add exact
-// zeros after the real terms); the few cells with 7 or 8 neighbours read the rest from
-// the column-major tables.
+// neighbour j with weight w.  Slots 0-5 are always visited (cells with fewer neighbours
+// are padded with (i, 0), which add exact zeros after the real terms), slots 6-7 only
+// where the cell has them.  The cell's record (8 bytes) gives its weight class, whose
+// row of 8 weights is shared by the ~47 cells related by the cube's symmetries and is
+// read from cache, and its neighbours as 4-bit offset codes (or, for cells on face seams,
+// the index of an explicit row).
 template <typename Fn>
 __device__ __forceinline__ void for_stencil(const ggbk_grid &g, int i, Fn fn)
 {
-	const int4 *p = (const int4 *) (g.stn + (size_t) i * 12);
-	int4 a = p[0], b = p[1], c = p[2];
-	int nn = g.maxnb > 6 ? (int) g.nnb[i] : 6;
-	fn(a.x, __int_as_float(a.y));
-	fn(a.z, __int_as_float(a.w));
-	fn(b.x, __int_as_float(b.y));
-	fn(b.z, __int_as_float(b.w));
-	fn(c.x, __int_as_float(c.y));
-	fn(c.z, __int_as_float(c.w));
-	for (int k = 6; k < nn; k++)
-		fn(g.nbr[(size_t) k * g.n + i], g.w[(size_t) k * g.n + i]);
+	int2 rc = ((const int2 *) g.srec)[i];
+	const float4 *wp = (const float4 *) (g.scw + 8 * (size_t) (rc.x & 0x7fffffff));
+	float4 wa = wp[0], wb = wp[1];
+	int nb[8];
+	if (rc.x < 0) {
+		const int4 *xp = (const int4 *) (g.sxn + 8 * (size_t) rc.y);
+		int4 a = xp[0], b = xp[1];
+		nb[0] = a.x; nb[1] = a.y; nb[2] = a.z; nb[3] = a.w;
+		nb[4] = b.x; nb[5] = b.y; nb[6] = b.z; nb[7] = b.w;
+	} else {
+		// slot k: code (dy + 1) << 2 | (dx + 1), neighbour i + dy N + dx
+		unsigned code = (unsigned) rc.y;
+		int base = i - g.N - 1;
+#pragma unroll
+		for (int k = 0; k < 8; k++)
+			nb[k] = base + (int) ((code >> (4 * k + 2)) & 3u) * g.N + (int) ((code >> (4 * k)) & 3u);
+	}
+	fn(nb[0], wa.x);
+	fn(nb[1], wa.y);
+	fn(nb[2], wa.z);
+	fn(nb[3], wa.w);
+	fn(nb[4], wb.x);
+	fn(nb[5], wb.y);
+	if (g.maxnb > 6) {
+		if (nb[6] != i)
+			fn(nb[6], wb.z);
+		if (nb[7] != i)
+			fn(nb[7], wb.w);
+	}
 }
 
 // ---------------- deterministic reductions ----------------

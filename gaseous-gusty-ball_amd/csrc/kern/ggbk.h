@@ -34,8 +34,11 @@ typedef struct {
 	const float *gc;          /* [maxnb][3][n] */
 	const int *ghost;         /* [6][N+2][N+2] tap index of (f, i, j), i, j in [-1, N] */
 	const float *dinv;        /* [n] 1 / sum(w), rounded to float */
-	const int *stn;           /* [n][6][2] (neighbour, weight bits) of slots 0-5 per cell,
-	                           * padded with (self, 0); slots 6-7 only in nbr/w */
+	/* compressed stencil: per cell (class | irregular << 31, offset codes or explicit row) */
+	const int *srec;          /* [n][2] */
+	const float *scw;         /* [ncls][8] weights of each class, slot order, zero-padded */
+	const int *sxn;           /* [nirr][8] neighbours of the irregular cells, self-padded */
+	int ncls, nirr;
 } ggbk_grid;
 
 /* bilinear prolongation from level l+1 to l, and its transpose.  Both are stored per

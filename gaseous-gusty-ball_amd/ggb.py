@@ -90,7 +90,7 @@ PUBLIC_SYMBOLS = [
     "ggb_device_init", "ggb_version", "ggb_sync", "fluid_sync_host", "fluid_sync_device",
     "fluid_get_f64", "fluid_set_f64", "fluid_solve_psi_device", "fluid_true_relres",
     "fluid_vcycle_f64", "ggb_laplacian_f64", "ggb_applyK_f64", "ggb_curl_f64",
-    "ggb_divergence_f64", "particles_sync_host", "particles_get_f64", "particles_set_f64",
+    "ggb_divergence_f64", "ggb_stencil_compress", "particles_sync_host", "particles_get_f64", "particles_set_f64",
     "particles_set_round32", "render_sync_host", "render_last_entries", "render_last_stats", "render_faces8",
     "fluid_adjoint_new", "fluid_adjoint_free", "fluid_adjoint_forward", "fluid_adjoint_backward",
     # ggb_inputs.h
@@ -137,6 +137,7 @@ def lib():
             "ggb_applyK_f64": (C.c_int, [P(CCubeSphere), dp, dp]),
             "ggb_curl_f64": (C.c_int, [P(CCubeSphere), dp, dp]),
             "ggb_divergence_f64": (C.c_int, [P(CCubeSphere), dp, dp]),
+            "ggb_stencil_compress": (C.c_int, [P(CCubeSphere), ip, fp, ip, ip, ip]),
             "particles_init": (C.c_int, [P(CParticles), C.c_int]),
             "particles_free": (None, [P(CParticles)]),
             "particles_color_from_image": (None, [P(CParticles), fp, C.c_int, C.c_int]),
@@ -189,6 +190,11 @@ def lib():
 def fptr(a):
     assert a.dtype == np.float32 and a.flags.c_contiguous
     return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def iptr(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
 def dptr(a):
@@ -271,6 +277,19 @@ class Grid:
         f = np.ascontiguousarray(f, np.float32).copy()
         lib().fluid_remove_mean(C.byref(self.c), fptr(f))
         return f
+
+    def stencil_compressed(self):
+        """(rec [n,2], class weights [ncls,8], explicit rows [nirr,8]) of the compressed
+        stencil the device passes read."""
+        n = self.n
+        rec = np.zeros(2 * n, np.int32)
+        cw = np.zeros(8 * n, np.float32)
+        xn = np.zeros(8 * n, np.int32)
+        ncls, nirr = C.c_int(), C.c_int()
+        if lib().ggb_stencil_compress(C.byref(self.c), iptr(rec), fptr(cw), iptr(xn), C.byref(ncls),
+                                      C.byref(nirr)):
+            raise MemoryError("ggb_stencil_compress")
+        return rec.reshape(n, 2), cw[:8 * ncls.value].reshape(-1, 8), xn[:8 * nirr.value].reshape(-1, 8)
 
     def laplacian_f64(self, x):
         y = np.zeros(self.n)

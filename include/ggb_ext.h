@@ -51,6 +51,15 @@ int ggb_laplacian_f64(const CubeSphere *cs, const double *x, double *y);
 int ggb_applyK_f64(const CubeSphere *cs, const double *x, double *y);
 int ggb_curl_f64(const CubeSphere *cs, const double *vel, double *out);
 int ggb_divergence_f64(const CubeSphere *cs, const double *vel, double *out);
+/* The compressed stencil the device passes read (host only, no device needed): per cell
+ * rec[2i] = weight class (| 1 << 31 for a cell with an explicit neighbour row) and
+ * rec[2i+1] = eight 4-bit neighbour codes, slot k in bits 4k..4k+3, code (dy + 1) << 2 |
+ * (dx + 1) for neighbour i + dy N + dx (self = 5 pads), or the explicit row;
+ * cw[8 * class + k] = weight of slot k
+ * (zero-padded); xn[8 * row + k] = neighbour of slot k (self-padded).  Capacities: rec 2n,
+ * cw 8n, xn 8n.  Has no counterpart in the reference, whose stencil is the ELL table
+ * itself (cubesphere.h nbr/w). */
+int ggb_stencil_compress(const CubeSphere *cs, int *rec, float *cw, int *xn, int *ncls, int *nirr);
 
 /* ---- particles ---- */
 void particles_sync_host(Particles *P); /* device -> pos/prev mirrors */

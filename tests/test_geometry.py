@@ -127,3 +127,36 @@ def test_against_textbook_spherical_voronoi(ggb, N):
         assert set(mine) == set(edges), c
         for j, ln in edges.items():
             assert abs(mine[j] - ln) < 1e-5 * max(ln, g.h)
+
+
+def decode_stencil(N, n, rec, cw, xn):
+    """Neighbour and weight tables [n, 8] from the compressed stencil."""
+    mags = np.array([(c >> 2) * N + (c & 3) - N - 1 for c in range(16)], np.int64)
+    idx = np.arange(n, dtype=np.int64)
+    cls = (rec[:, 0].astype(np.int64) & 0x7FFFFFFF)
+    irr = rec[:, 0] < 0
+    code = np.where(irr, 0, rec[:, 1].astype(np.int64) & 0xFFFFFFFF)
+    nib = (code[:, None] >> (4 * np.arange(8))) & 15
+    nbr = idx[:, None] + mags[nib]
+    nbr[irr] = xn[rec[irr, 1]]
+    return nbr, cw[cls]
+
+
+@pytest.mark.parametrize("N", [8, 16, 33, 64, 128])
+def test_stencil_compression_round_trip(ggb, N):
+    """The compressed stencil (weight classes + offset codes) decodes to the ELL table
+    exactly, slot for slot, with (self, 0) padding; the classes really are shared."""
+    g = ggb.Grid(N)
+    rec, cw, xn = g.stencil_compressed()
+    nbr, w = decode_stencil(N, g.n, rec, cw, xn)
+    nnb = g.nnb.astype(int)
+    slot = np.arange(8)[None, :]
+    real = slot < nnb[:, None]
+    assert np.array_equal(nbr[real], g.nbr[real])
+    assert np.array_equal(w[real].view(np.uint32), g.w[real].view(np.uint32))
+    assert np.all(nbr[~real] == np.broadcast_to(np.arange(g.n)[:, None], nbr.shape)[~real])
+    assert np.all(w[~real] == 0)
+    # explicit rows only along the face seams (about 4 per face row), and every class
+    # shared by many cells
+    assert len(xn) <= 5 * g.n / N
+    assert len(cw) <= g.n / 10 if N >= 16 else len(cw) < g.n
