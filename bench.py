@@ -39,9 +39,10 @@ HBM_PEAK = 8.0e12
 
 # Probed kernels (ids of the library's launch probes) and their algorithmic bytes per
 # launch under the per-pass model: every pass reads its inputs and writes its outputs
-# once, gathered neighbours are cache hits, stencil and transfer coefficients are the
-# stored float tables.  kbar = mean Voronoi degree, n = cells, P = particles,
-# E = (particle, tile) entries, T = texels touched, Ts/Tm = state / multigrid word.
+# once, gathered neighbours are cache hits, the Laplacian stencil is one 8-byte record
+# per cell (its class weights are cache-resident), transfer coefficients are the stored
+# float tables.  n = cells, P = particles, E = (particle, tile) entries, T = texels
+# touched, Ts/Tm = state / multigrid word.
 PROBES = {
     "k_render_draw": 1, "render_count": 2, "mg_up(level 0)": 3, "k_mg_down(level 0)": 4,
     "k_cg_Ap": 5, "k_cg_update_down": 6, "k_particles_advect": 7, "k_departures": 8,
@@ -49,21 +50,21 @@ PROBES = {
 }
 
 
-def kernel_bytes(name, n, kbar, P, E, T, Ts, Tm, Tz):
+def kernel_bytes(name, n, P, E, T, Ts, Tm, Tz):
     if name == "k_render_draw":      # texels read+written, list entry, FP64 endpoints, colour, index
         return 32 * T + E * (4 + 48 + 16 + 4)
     if name == "render_count":       # FP64 endpoints (the tile counters stay in L2)
         return P * 48
     if name == "k_render_fill":      # particle read + reordered copy (68 B each), sample count, list
         return P * (68 + 68 + 2) + E * 4
-    if name == "mg_up(level 0)":     # b, dinv, nnb, stencil, taps+weights, coarse x, z out
-        return n * (Ts + 4 + 1 + 8 * kbar + 32 + Tm / 4 + Tm)
-    if name == "k_mg_down(level 0)":  # b, dinv, nnb, stencil, r1 out
-        return n * (Ts + 4 + 1 + 8 * kbar + Tm)
-    if name == "k_cg_Ap":            # z, p in, nnb, stencil, p out, Ap out
-        return n * (Tz + Ts + 1 + 8 * kbar + 2 * Ts)
-    if name == "k_cg_update_down":   # psi, r, p, Ap, dinv, nnb, stencil in; psi, r, r1 out
-        return n * (6 * Ts + 5 + 8 * kbar + Tm)
+    if name == "mg_up(level 0)":     # b, dinv, stencil record, tap weights, coarse x, z out
+        return n * (Ts + 4 + 8 + 16 + Tm / 4 + Tm)
+    if name == "k_mg_down(level 0)":  # b, dinv, stencil record, r1 out
+        return n * (Ts + 4 + 8 + Tm)
+    if name == "k_cg_Ap":            # z, p in, stencil record, p out, Ap out
+        return n * (Tz + Ts + 8 + 2 * Ts)
+    if name == "k_cg_update_down":   # psi, r, p, Ap, dinv, stencil record in; psi, r, r1 out
+        return n * (6 * Ts + 12 + Tm)
     if name == "k_particles_advect":  # pos in, pos+prev out (FP64), velocity field once
         return P * 72 + n * 3 * Ts
     if name == "k_departures":       # vel, omega, centre in; q0, dep_b, dep_f out
@@ -270,13 +271,11 @@ def main():
         % (K, wall, ms_fluid, acc["its"] / K, ms_part, ms_render))
 
     # roofline of the dominant kernel
-    nn = g.nnb.astype(np.int64)
-    kbar = float(nn.mean())
     Ts = 4 if cfg["precision"] == ggb.PREC_REF32 else 8
     Tm = 8 if cfg["precision"] == ggb.PREC_FP64 else 4
     E = rstats[0]
     T = rstats[8]
-    nbytes = kernel_bytes(dominant, g.n, kbar, cfg["P"], E, T, Ts, Tm, Tm)
+    nbytes = kernel_bytes(dominant, g.n, cfg["P"], E, T, Ts, Tm, Tm)
     avg_s = dms.value / 1000.0 / max(dcount, 1)
     achieved = nbytes / avg_s / 1e9
     # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated),
@@ -290,7 +289,7 @@ def main():
     kernels = {}
     for name, (cnt, ms) in warm.items():
         if cnt:
-            b = kernel_bytes(name, g.n, kbar, cfg["P"], E, T, Ts, Tm, Tm)
+            b = kernel_bytes(name, g.n, cfg["P"], E, T, Ts, Tm, Tm)
             kernels[name] = {"launches_per_step": round(cnt / max(args.warmup - nplain, 1), 2),
                              "avg_us": round(ms * 1000.0 / cnt, 2),
                              "algo_bytes": int(b),

@@ -364,6 +364,45 @@ static int mg_levels(int N)
 	return nlev;
 }
 
+/* Whether the transfer's indices follow the face-interior pattern the device passes
+ * compute instead of reading: with Nf = 2 Nc, fine cell (f, x, y) with 1 <= x, y <= Nf - 2
+ * has its taps at coarse (f, (x - 1) >> 1, (y - 1) >> 1) and the three cells right, up and
+ * up-right of it; coarse cell (f, I, J) with 1 <= I, J <= Nc - 2 has exactly the 16 fine
+ * cells x in [2I - 1, 2I + 2], y in [2J - 1, 2J + 2] in its row of R, in fine-index order,
+ * fine cell (x, y) with the weight of its tap 2 [y <= 2J] + [x <= 2I].  Checked cell by
+ * cell on the tables built above; the weights stay tabulated. */
+static int transfer_is_analytic(const CubeSphere *fine, const CubeSphere *coarse, const int *ptap,
+				const int *rstart, const int *ridx, const float *rw, const float *pw)
+{
+	int Nf = fine->N, Nc = coarse->N;
+	if (Nf != 2 * Nc || Nc < 3)
+		return 0;
+	for (int f = 0; f < 6; f++)
+		for (int y = 1; y <= Nf - 2; y++)
+			for (int x = 1; x <= Nf - 2; x++) {
+				size_t i = ((size_t) f * Nf + y) * Nf + x;
+				int b = (f * Nc + ((y - 1) >> 1)) * Nc + ((x - 1) >> 1);
+				const int *t = ptap + 4 * i;
+				if (t[0] != b || t[1] != b + 1 || t[2] != b + Nc || t[3] != b + Nc + 1)
+					return 0;
+			}
+	for (int f = 0; f < 6; f++)
+		for (int J = 1; J <= Nc - 2; J++)
+			for (int I = 1; I <= Nc - 2; I++) {
+				int c = (f * Nc + J) * Nc + I;
+				if (rstart[c + 1] - rstart[c] != 16)
+					return 0;
+				for (int e = 0; e < 16; e++) {
+					int y = 2 * J - 1 + e / 4, x = 2 * I - 1 + e % 4;
+					int i = (f * Nf + y) * Nf + x;
+					int k = 2 * (y <= 2 * J) + (x <= 2 * I);
+					if (ridx[rstart[c] + e] != i || memcmp(&rw[rstart[c] + e], &pw[4 * (size_t) i + k], 4))
+						return 0;
+				}
+			}
+	return 1;
+}
+
 /* Bilinear transfer from the coarse grid to the fine grid: taps of every fine centre on
  * the coarse grid, and R = P^T as CSR with fine indices ascending (fluid.c:64-96).  The
  * host tables are left in *hl. */
@@ -431,10 +470,15 @@ static ggbk_xfer build_transfer(struct ggb_fluid_dev *D, const CubeSphere *fine,
 	x.nc = nc;
 	x.rmax = rmax;
 	x.rpad = rpad;
+	x.Nc = coarse->N;
+	x.analytic = transfer_is_analytic(fine, coarse, ptap, rstart, ridx, rw, pw);
 	void *d;
 	d = xbuf(D, sizeof(int) * 8 * (size_t) nf, "transfer");
 	upload(d, pt, sizeof(int) * 8 * (size_t) nf);
 	x.pt = d;
+	d = xbuf(D, sizeof(float) * 4 * (size_t) nf, "transfer");
+	upload(d, pw, sizeof(float) * 4 * (size_t) nf);
+	x.pw = d;
 	d = xbuf(D, sizeof(int) * 2 * (size_t) rpad * nc, "transfer");
 	upload(d, rr, sizeof(int) * 2 * (size_t) rpad * nc);
 	x.rr = d;

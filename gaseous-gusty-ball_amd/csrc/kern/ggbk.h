@@ -45,6 +45,9 @@ typedef struct {
  * row as interleaved (index, weight bits) pairs, so a row is read with 16-byte loads. */
 typedef struct {
 	int nf, nc, rmax, rpad;  /* rpad: padded row width, 18 or GGBK_RMAX */
+	int Nc, analytic;  /* coarse face size; indices of face-interior rows follow the
+	                    * bilinear pattern (only the weights are read for those) */
+	const float *pw;   /* [nf][4] weights of each fine cell's taps */
 	const int *pt;     /* [nf][4][2] coarse tap and weight of each fine cell */
 	const int *rr;     /* [rpad/2][nc][4] rows of R = P^T, two (fine index, weight bits) pairs
 	                    * per element, column-major over rows (coalesced across a wave); fine

@@ -127,13 +127,30 @@ __global__ __launch_bounds__(BS) void k_mg_restrict(ggbk_xfer xf, const TM *rf, 
 
 // x after prolongation at cell j: x1 + (TM)(1.0 * sum_k pw_k xc[tap_k]) with the tap
 // products and sum in storage precision (fluid.c:138-144)
-template <typename TM, typename TB>
+template <typename TM, typename TB, bool ANALYTIC = false>
 __device__ __forceinline__ TM xcorr_at(const ggbk_grid &g, const ggbk_xfer &xf, const TB *b, const TM *xc, int j)
 {
-	const int4 *q = (const int4 *) (xf.pt + 8 * (size_t) j);
-	int4 t01 = q[0], t23 = q[1];
-	TM v = __int_as_float(t01.y) * xc[t01.x] + __int_as_float(t01.w) * xc[t01.z] +
-	       __int_as_float(t23.y) * xc[t23.x] + __int_as_float(t23.w) * xc[t23.z];
+	TM v;
+	int Nf = 2 * xf.Nc, f = 0, x = 0, y = 0;
+	if (ANALYTIC) {
+		f = j / (Nf * Nf);
+		int rem = j - f * Nf * Nf;
+		y = rem / Nf;
+		x = rem - y * Nf;
+	}
+	if (ANALYTIC && xf.analytic && x >= 1 && x <= Nf - 2 && y >= 1 && y <= Nf - 2) {
+		// face-interior fine cell: taps at (x - 1) >> 1, (y - 1) >> 1 and its right, up
+		// and up-right neighbours; weights from the table
+		int Nc = xf.Nc;
+		int t = (f * Nc + ((y - 1) >> 1)) * Nc + ((x - 1) >> 1);
+		float4 w = ((const float4 *) xf.pw)[j];
+		v = w.x * xc[t] + w.y * xc[t + 1] + w.z * xc[t + Nc] + w.w * xc[t + Nc + 1];
+	} else {
+		const int4 *q = (const int4 *) (xf.pt + 8 * (size_t) j);
+		int4 t01 = q[0], t23 = q[1];
+		v = __int_as_float(t01.y) * xc[t01.x] + __int_as_float(t01.w) * xc[t01.z] +
+		    __int_as_float(t23.y) * xc[t23.x] + __int_as_float(t23.w) * xc[t23.z];
+	}
 	return x1_at<TM, TB>(g, b, j) + (TM) (1.0 * (double) v);
 }
 
@@ -175,7 +192,7 @@ __global__ __launch_bounds__(BS) void k_mg_prolong(ggbk_grid g, ggbk_xfer xf, co
 	int act = s->active;
 	int i = xblock() * BS + threadIdx.x;
 	if (i < g.n) {
-		TM v = xcorr_at<TM, TB>(g, xf, b, xc, i);
+		TM v = xcorr_at<TM, TB, true>(g, xf, b, xc, i);
 		if (act)
 			xp[i] = v;
 	}
