@@ -88,15 +88,36 @@ __device__ __forceinline__ bool footprint(float px, float py, float rad, int R, 
 	return xlo <= xhi && ylo <= yhi;
 }
 
+// Pixel coordinates of direction d on face f in single precision, through one hardware
+// reciprocal (about 1 ulp): for the binning only, whose boxes carry a margin far above
+// this error (at dc >= 0.45 and R <= 4096, under 0.002 texel), and for the home tile,
+// which only orders storage.  count and fill evaluate it identically.
+__device__ __forceinline__ void pixel_fast(int f, dv3 d, int R, float &px, float &py)
+{
+	double dc, cr, cu;
+	switch (f) {
+	case 0: dc = d.z; cr = d.x; cu = d.y; break;
+	case 1: dc = d.x; cr = -d.z; cu = d.y; break;
+	case 2: dc = -d.z; cr = -d.x; cu = d.y; break;
+	case 3: dc = -d.x; cr = d.z; cu = d.y; break;
+	case 4: dc = d.y; cr = d.x; cu = -d.z; break;
+	default: dc = -d.y; cr = d.x; cu = d.z; break;
+	}
+	float inv = __builtin_amdgcn_rcpf((float) dc);
+	float u = (float) cr * inv, v = (float) cu * inv;
+	float hR = 0.5f * (float) R;
+	px = (u + 1.0f) * hR - 0.5f;
+	py = (1.0f - v) * hR - 0.5f;
+}
+
 // Tile of the texel nearest the particle's current position, on the face it lies on.
 __device__ __forceinline__ int home_tile(dv3 p, int R, int ntx)
 {
 	int f = face_of(p);
-	double dc, u, v;
-	gnomonic(f, p, dc, u, v);
-	double px = (u + 1.0) * 0.5 * R - 0.5, py = (1.0 - v) * 0.5 * R - 0.5;
-	int ix = (int) fmin(fmax(floor(px + 0.5), 0.0), (double) (R - 1));
-	int iy = (int) fmin(fmax(floor(py + 0.5), 0.0), (double) (R - 1));
+	float px, py;
+	pixel_fast(f, p, R, px, py);
+	int ix = (int) fminf(fmaxf(floorf(px + 0.5f), 0.0f), (float) (R - 1));
+	int iy = (int) fminf(fmaxf(floorf(py + 0.5f), 0.0f), (float) (R - 1));
 	return (f * ntx + iy / TILE) * ntx + ix / TILE;
 }
 
@@ -119,18 +140,16 @@ __device__ __forceinline__ bool face_box(dv3 a, dv3 b, bool shortarc, int &ns, i
 	if (shortarc && da < 0.45 && db < 0.45)
 		return false;
 	if (shortarc && da >= 0.45 && db >= 0.45) {
-		double dc, u, v;
-		gnomonic(f, a, dc, u, v);
-		double pxa = (u + 1.0) * 0.5 * sg.R - 0.5, pya = (1.0 - v) * 0.5 * sg.R - 0.5;
-		gnomonic(f, b, dc, u, v);
-		double pxb = (u + 1.0) * 0.5 * sg.R - 0.5, pyb = (1.0 - v) * 0.5 * sg.R - 0.5;
-		double x0 = fmin(pxa, pxb) - 0.01 - sg.rad, x1 = fmax(pxa, pxb) + 0.01 + sg.rad;
-		double y0 = fmin(pya, pyb) - 0.01 - sg.rad, y1 = fmax(pya, pyb) + 0.01 + sg.rad;
-		// clamp in double before converting (far-off projections of a grazing endpoint)
-		xlo = (int) fmax(ceil(x0), 0.0);
-		xhi = (int) fmin(floor(x1), (double) (sg.R - 1));
-		ylo = (int) fmax(ceil(y0), 0.0);
-		yhi = (int) fmin(floor(y1), (double) (sg.R - 1));
+		float pxa, pya, pxb, pyb;
+		pixel_fast(f, a, sg.R, pxa, pya);
+		pixel_fast(f, b, sg.R, pxb, pyb);
+		float x0 = fminf(pxa, pxb) - 0.01f - sg.rad, x1 = fmaxf(pxa, pxb) + 0.01f + sg.rad;
+		float y0 = fminf(pya, pyb) - 0.01f - sg.rad, y1 = fmaxf(pya, pyb) + 0.01f + sg.rad;
+		// clamp before converting (dc >= 0.45 keeps |u|, |v| below 2.1)
+		xlo = (int) fmaxf(ceilf(x0), 0.0f);
+		xhi = (int) fminf(floorf(x1), (float) (sg.R - 1));
+		ylo = (int) fmaxf(ceilf(y0), 0.0f);
+		yhi = (int) fminf(floorf(y1), (float) (sg.R - 1));
 		return xlo <= xhi && ylo <= yhi;
 	}
 	if (ns == 0)
@@ -164,7 +183,7 @@ __device__ __forceinline__ int wave_atomic_inc(int *cnt, int key, bool on)
 	int leader_of = lane, rank = 0, add = 0;
 	while (todo) {
 		int leader = __ffsll((long long) todo) - 1;
-		int lk = __shfl(key, leader);
+		int lk = __builtin_amdgcn_readlane(key, leader);
 		unsigned long long m = __ballot(on && key == lk) & todo;
 		if (lane == leader)
 			add = __popcll(m);
@@ -189,7 +208,7 @@ __device__ __forceinline__ void wave_atomic_count(int *cnt, int key, bool on)
 	int lane = __lane_id();
 	while (todo) {
 		int leader = __ffsll((long long) todo) - 1;
-		int lk = __shfl(key, leader);
+		int lk = __builtin_amdgcn_readlane(key, leader);
 		unsigned long long m = __ballot(on && key == lk) & todo;
 		if (lane == leader)
 			atomicAdd(&cnt[lk], __popcll(m));
@@ -214,8 +233,11 @@ __device__ __forceinline__ void for_arc_tiles(bool on, dv3 a, dv3 b, const Splat
 			if (!shortarc || face_dot(f, a) >= 0.45 || face_dot(f, b) >= 0.45)
 				fmask |= 1u << f;
 	}
-	for (int off = 32; off > 0; off >>= 1)
-		fmask |= (unsigned) __shfl_xor((int) fmask, off);
+	unsigned wmask = 0;
+	for (int f = 0; f < 6; f++)
+		if (__ballot((fmask >> f) & 1u))
+			wmask |= 1u << f;
+	fmask = wmask;
 	for (int f = 0; f < 6; f++) {
 		if (!((fmask >> f) & 1u))
 			continue;
