@@ -522,12 +522,13 @@ __device__ __forceinline__ void sort_net(unsigned *k)
 //      tile list was, otherwise ranks are counted); everything is stored by rank, and the
 //      exclusive prefix of ns in rank order;
 //   then per range of consecutive ranks whose samples fit SCAP (ranges are in index order):
-//   C  one thread per (particle, sample): the sample's pixel on this tile's face; samples
-//      on the tile take a slot (position + key rank<<12 | k-1) and append the slot to every
-//      texel they touch;
+//   C  one thread per (particle, sample), its rank read from a per-range sample map: the
+//      sample's pixel on this tile's face; samples on the tile take a slot (position + key
+//      rank<<12 | k-1) and append the slot to every texel they touch;
 //   D  one thread per texel: sort its fragments by key and blend them in that order.
 // A range in which a texel collects more than CAPT fragments is retried with half the
-// particles; a single particle that still overflows is blended in order by one thread.
+// particles; a single particle that still overflows (or alone has more than SCAP
+// samples) is blended in sample order by every thread for its own texels.
 // The sample counts come from the binning (an FP64 atan2 here would double the
 // registers), each thread keeps its two texels in registers, and the LDS arrays (20.4 KB,
 // allocated in 1280-byte granules) leave room for 8 workgroups per CU, which is also the
@@ -542,7 +543,9 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 	__shared__ float2 sxy[SCAP];
 	__shared__ unsigned skey[SCAP];
 	__shared__ double pa[3][WIN], pb[3][WIN];  // by rank
-	__shared__ int pns[WIN], pst[WIN + 1], ppid[WIN];
+	__shared__ int pst[WIN + 1], ppid[WIN];
+	__shared__ unsigned short pns[WIN];        // sample count by rank (<= 4096)
+	__shared__ unsigned char smap[SCAP];       // rank - r0 of each sample of a range
 	__shared__ float4 pcol[WIN];               // by rank
 	__shared__ int overflow, nsamp, wtot, ntouched;
 	long long c0 = clock64();
@@ -633,7 +636,7 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 			pa[0][rank] = a.x; pa[1][rank] = a.y; pa[2][rank] = a.z;
 			pb[0][rank] = b.x; pb[1][rank] = b.y; pb[2][rank] = b.z;
 			pcol[rank] = mycol;
-			pns[rank] = myns;
+			pns[rank] = (unsigned short) myns;
 		}
 		__syncthreads();
 		// exclusive prefix of ns in rank order (two waves)
@@ -666,25 +669,26 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 			int r1 = lo;
 			bool retry = true;
 			while (retry) {
+				int j0 = pst[r0], j1 = pst[r1];
+				// a single particle with more samples than a range holds goes to the
+				// in-order fallback directly
+				bool big = j1 - j0 > SCAP;
 				for (int i = lid; i < NTEX; i += DBS)
 					tcnt[i] = 0;
 				if (lid == 0) {
-					overflow = 0;
+					overflow = big ? 1 : 0;
 					nsamp = 0;
+				}
+				// the rank of every sample of the range (r1 - r0 <= WIN = DBS)
+				if (!big && r0 + lid < r1) {
+					int p = r0 + lid;
+					for (int k = pst[p] - j0; k < pst[p + 1] - j0; k++)
+						smap[k] = (unsigned char) lid;
 				}
 				__syncthreads();
 				// C: one thread per (particle, sample) of ranks [r0, r1)
-				int j0 = pst[r0], j1 = pst[r1];
-				for (int j = j0 + lid; j < j1; j += DBS) {
-					int l = r0, h = r1 - 1; // last rank whose first sample is <= j
-					while (l < h) {
-						int mid = (l + h + 1) >> 1;
-						if (pst[mid] <= j)
-							l = mid;
-						else
-							h = mid - 1;
-					}
-					int p = l, k = j - pst[p] + 1;
+				for (int j = j0 + lid; !big && j < j1; j += DBS) {
+					int p = r0 + smap[j - j0], k = j - pst[p] + 1;
 					dv3 pa3 = d3(pa[0][p], pa[1][p], pa[2][p]), pb3 = d3(pb[0][p], pb[1][p], pb[2][p]);
 					float px, py;
 					if (!face_pixel(tr.f, arc_point(pa3, pb3, k, pns[p]), sg.R, px, py))
