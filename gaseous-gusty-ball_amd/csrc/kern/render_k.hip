@@ -543,7 +543,8 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 	__shared__ float2 sxy[SCAP];
 	__shared__ unsigned skey[SCAP];
 	__shared__ double pa[3][WIN], pb[3][WIN];  // by rank
-	__shared__ int pst[WIN + 1], ppid[WIN];
+	__shared__ int pst[WIN + 1];
+	__shared__ __align__(16) int ppid[WIN];    // window order; padded to 4 with INT_MAX
 	__shared__ unsigned short pns[WIN];        // sample count by rank (<= 4096)
 	__shared__ unsigned char smap[SCAP];       // rank - r0 of each sample of a range
 	__shared__ float4 pcol[WIN];               // by rank
@@ -623,15 +624,21 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 			mypid = ps.pid[slot];
 			mycol = *(const float4 *) (ps.col + 4 * (size_t) slot);
 			ppid[lid] = mypid;
+		} else if (lid < ((W + 3) & ~3)) {
+			ppid[lid] = INT_MAX;
 		}
 		__syncthreads();
 		// B: rank by index; store by rank
 		int rank = lid;
 		if (lid < W) {
 			if (!is_sorted) {
+				// count the smaller indices, four per (broadcast) LDS read
 				rank = 0;
-				for (int q = 0; q < W; q++)
-					rank += ppid[q] < mypid;
+				const int4 *p4 = (const int4 *) ppid;
+				for (int q = 0; q < (W + 3) / 4; q++) {
+					int4 v = p4[q];
+					rank += (v.x < mypid) + (v.y < mypid) + (v.z < mypid) + (v.w < mypid);
+				}
 			}
 			pa[0][rank] = a.x; pa[1][rank] = a.y; pa[2][rank] = a.z;
 			pb[0][rank] = b.x; pb[1][rank] = b.y; pb[2][rank] = b.z;
