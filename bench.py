@@ -53,8 +53,8 @@ PROBES = {
 def kernel_bytes(name, n, P, E, T, Ts, Tm, Tz):
     if name == "k_render_draw":      # texels read+written, list entry, FP64 endpoints, colour, index
         return 32 * T + E * (4 + 48 + 16 + 4)
-    if name == "render_count":       # FP64 endpoints (the tile counters stay in L2)
-        return P * 48
+    if name == "render_count":       # current position (FP64) for the home tile
+        return P * 24
     if name == "k_render_fill":      # particle read + reordered copy (68 B each), sample count, list
         return P * (68 + 68 + 2) + E * 4
     if name == "mg_up(level 0)":     # b, dinv, stencil record, tap weights, coarse x, z out

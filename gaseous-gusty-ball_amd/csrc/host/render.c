@@ -71,6 +71,8 @@ int render_init(Render *r, int R, float radius, float alpha)
 	k->total = ggbk_malloc(sizeof(long long) * 16);
 	k->tile_diag = ggbk_malloc(sizeof(int) * 8 * (size_t) k->ntiles);
 	k->scan_part = ggbk_malloc(sizeof(long long) * 3 * (size_t) ggbk_render_scan_blocks(k->ntiles));
+	k->bucket = ggbk_malloc(sizeof(int) * (size_t) ggbk_render_bucket() * (size_t) k->ntiles);
+	k->bucket_mode = 0;
 	k->cap = 0;
 	k->list = NULL;
 	k->nsamp = NULL;
@@ -78,7 +80,7 @@ int render_init(Render *r, int R, float radius, float alpha)
 	k->sort_scratch = NULL;
 	if (!k->buf || !k->tile_count || !k->tile_start || !k->tile_cursor || !k->home_count || !k->home_start ||
 	    !k->home_cursor || !k->total || !k->tile_diag ||
-	    !k->scan_part) {
+	    !k->scan_part || !k->bucket) {
 		fprintf(stderr, "render: out of device memory\n");
 		return -1;
 	}
@@ -105,6 +107,7 @@ void render_free(Render *r)
 		ggbk_free(D->k.total);
 		ggbk_free(D->k.tile_diag);
 		ggbk_free(D->k.scan_part);
+		ggbk_free(D->k.bucket);
 		ggbk_free(D->k.list);
 		ggbk_free(D->k.nsamp);
 		ggbk_free(D->k.sort_scratch);

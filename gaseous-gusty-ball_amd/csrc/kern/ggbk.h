@@ -252,6 +252,8 @@ typedef struct {
 	int *home_start;            /* [ntiles + 1] */
 	int *home_cursor;           /* [ntiles] */
 	int *list;                  /* [cap] particle slots per tile */
+	int *bucket;                /* [ntiles][tile_bucket()] particle slots, while no tile has more */
+	int bucket_mode;            /* the draw reads the buckets (1) or the list (0) */
 	unsigned short *nsamp;      /* [nsamp_cap] sample count of each particle slot (filled by the binning) */
 	long long nsamp_cap;
 	long long cap;
@@ -267,6 +269,7 @@ void ggbk_render_scale(float *buf, size_t count, float g);
 /* sums the per-tile diagnostics of the last deposit and copies the 16 values
  * (synchronises) */
 int ggbk_render_stats(const ggbk_render *r, long long *out16);
+int ggbk_render_bucket(void);  /* slots per tile bucket */
 /* bin + ordered blend of the particles `in`; writes them to `out` in the next storage
  * (tile) order.  Returns the number of (particle, tile) entries, or -1 on error. */
 long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_parts out, float invg);

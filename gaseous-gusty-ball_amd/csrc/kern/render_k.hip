@@ -37,6 +37,7 @@
 #define WIN 128           // particles per window = draw workgroup size
 #define DBS WIN
 #define CAPT 12           // fragments one texel can hold per window
+#define BUCKET WIN        // slots per tile bucket: a full bucket is one draw window
 #define SCAN_BS 1024
 
 struct SplatGeo {
@@ -254,23 +255,26 @@ __device__ __forceinline__ void for_arc_tiles(bool on, dv3 a, dv3 b, const Splat
 	}
 }
 
-__global__ __launch_bounds__(PBS) void k_render_count(ggbk_parts ps, SplatGeo sg, int *counts, int *hcounts)
+// Particles per home tile (the next storage order).  The tile lists are counted by the
+// fill itself, into fixed-size buckets.
+__global__ __launch_bounds__(PBS) void k_render_count(ggbk_parts ps, SplatGeo sg, int *hcounts)
 {
 	int i = xblock() * PBS + threadIdx.x;
 	bool on = i < ps.n;
-	dv3 a = d3(0, 0, 1), b = d3(0, 0, 1);
-	if (on) {
-		a = ld3(ps.prev + 3 * (size_t) i);
+	dv3 b = d3(0, 0, 1);
+	if (on)
 		b = ld3(ps.pos + 3 * (size_t) i);
-	}
 	int home = on ? home_tile(b, sg.R, sg.ntx) : 0;
 	wave_atomic_count(hcounts, home, on);
-	for_arc_tiles(on, a, b, sg, [&](bool has, int t) { wave_atomic_count(counts, t, has); });
 }
 
+// Writes each particle's (particle, tile) entries, and its copy in the next storage order.
+// With bucket != NULL the entries go to fixed-size tile buckets and cursor ends as the
+// exact count of every tile (entries past a full bucket are counted, not stored);
+// otherwise to the list at the scanned starts.
 __global__ __launch_bounds__(PBS) void k_render_fill(ggbk_parts ps, ggbk_parts out, SplatGeo sg, const int *starts,
 						     int *cursor, const int *hstarts, int *hcursor, int *list,
-						     long long cap, unsigned short *nsamp)
+						     long long cap, int *bucket, unsigned short *nsamp)
 {
 	int i = xblock() * PBS + threadIdx.x;
 	bool on = i < ps.n;
@@ -296,9 +300,14 @@ __global__ __launch_bounds__(PBS) void k_render_fill(ggbk_parts ps, ggbk_parts o
 	for_arc_tiles(on, a, b, sg, [&](bool has, int t) {
 		int r = wave_atomic_inc(cursor, t, has);
 		if (has) {
-			long long slot = (long long) starts[t] + r;
-			if (slot < cap)
-				list[slot] = i;
+			if (bucket) {
+				if (r < BUCKET)
+					bucket[(size_t) t * BUCKET + r] = i;
+			} else {
+				long long slot = (long long) starts[t] + r;
+				if (slot < cap)
+					list[slot] = i;
+			}
 		}
 	});
 }
@@ -407,6 +416,36 @@ __global__ __launch_bounds__(SCAN_BS) void k_render_scan_fix(int ntiles, int *st
 		stats[1] = m;
 		for (int k = 2; k < 16; k++)
 			stats[k] = 0;
+	}
+}
+
+// stats[0] += total of the bucket counts, stats[1] = max(stats[1], longest) (integer
+// atomics: order-free).
+__global__ __launch_bounds__(SCAN_BS) void k_render_bucket_stats(int ntiles, const int *counts, long long *stats)
+{
+	__shared__ long long ws[SCAN_BS / 64];
+	__shared__ int wm[SCAN_BS / 64];
+	int t = blockIdx.x * SCAN_BS + threadIdx.x;
+	long long c = t < ntiles ? counts[t] : 0;
+	int m = (int) c;
+	for (int off = 32; off > 0; off >>= 1) {
+		c += __shfl_xor(c, off);
+		m = max(m, __shfl_xor(m, off));
+	}
+	if (__lane_id() == 0) {
+		ws[threadIdx.x / 64] = c;
+		wm[threadIdx.x / 64] = m;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		long long sum = 0;
+		int mm = 0;
+		for (int k = 0; k < SCAN_BS / 64; k++) {
+			sum += ws[k];
+			mm = max(mm, wm[k]);
+		}
+		atomicAdd((unsigned long long *) &stats[0], (unsigned long long) sum);
+		atomicMax(&stats[1], (long long) mm);
 	}
 }
 
@@ -553,8 +592,8 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 	int t = xblock();
 	const int lid = threadIdx.x;
 	int *diag = r.tile_diag + 8 * (size_t) t;
-	int L = r.tile_start[t + 1] - r.tile_start[t];
-	if (L == 0 || r.total[0] > r.cap) {
+	int L = r.bucket_mode ? r.tile_cursor[t] : r.tile_start[t + 1] - r.tile_start[t];
+	if (L == 0 || (!r.bucket_mode && r.total[0] > r.cap)) {
 		// nothing to draw (or the list did not fit: the host grows it before launching)
 		if (lid < 8)
 			diag[lid] = 0;
@@ -567,7 +606,7 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 	tr.x0 = (rem % r.ntx) * TILE;
 	tr.w = min(TILE, r.R - tr.x0);
 	tr.h = min(TILE, r.R - tr.y0);
-	const int *lst = r.list + r.tile_start[t];
+	const int *lst = r.bucket_mode ? r.bucket + (size_t) t * BUCKET : r.list + r.tile_start[t];
 	const float rad = sg.rad, invrad = 1.0f / rad;
 
 	// thread lid owns texels lid + h DBS and keeps them in registers
@@ -876,6 +915,8 @@ extern "C" void ggbk_render_scale(float *buf, size_t count, float g)
 
 extern "C" int ggbk_render_tile(void) { return TILE; }
 
+extern "C" int ggbk_render_bucket(void) { return BUCKET; }
+
 extern "C" int ggbk_render_scan_blocks(int ntiles) { return (ntiles + SCAN_BS - 1) / SCAN_BS; }
 
 extern "C" int ggbk_render_stats(const ggbk_render *r, long long *out16)
@@ -885,9 +926,11 @@ extern "C" int ggbk_render_stats(const ggbk_render *r, long long *out16)
 }
 
 // Host side of one deposit.  Returns the number of (particle, tile) entries, or -1.
-// The only synchronisation is the read-back of the entry total after the scan (the list
-// grows when the total exceeds its capacity).  `out` receives the particles in the next
-// storage order; the caller adopts it once this returns.
+// The fill writes the tile lists into fixed-size buckets and counts every tile exactly;
+// the one synchronisation reads the entry total and the longest list.  If a tile
+// overflowed its bucket, the counts are scanned into starts and the fill runs again into
+// one packed list (grown when the total exceeds its capacity).  `out` receives the
+// particles in the next storage order; the caller adopts it once this returns.
 extern "C" long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_parts out, float invg)
 {
 	if (in.n <= 0)
@@ -900,33 +943,7 @@ extern "C" long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_par
 	sg.rad = r->radius;
 	sg.spacing = 0.7 / r->R;
 	unsigned pb = ggbk_blocks(in.n, PBS);
-	ggbk_probe_begin(GGBK_PROBE_RENDER_COUNT);
-	if (ggbk_zero(r->tile_count, sizeof(int) * (size_t) r->ntiles) ||
-	    ggbk_zero(r->home_count, sizeof(int) * (size_t) r->ntiles))
-		return -1;
-	GGBK_LAUNCH(k_render_count, pb, PBS, 0, in, sg, r->tile_count, r->home_count);
 	unsigned sb = ggbk_blocks(r->ntiles, SCAN_BS);
-	GGBK_LAUNCH(k_render_scan_local, sb, SCAN_BS, 0, r->ntiles, (const int *) r->tile_count, r->tile_start,
-		    r->tile_cursor, (const int *) r->home_count, r->home_start, r->home_cursor, r->scan_part);
-	GGBK_LAUNCH(k_render_scan_fix, sb, SCAN_BS, 0, r->ntiles, r->tile_start, r->home_start,
-		    (const long long *) r->scan_part, r->total);
-	ggbk_probe_end(GGBK_PROBE_RENDER_COUNT);
-	long long stats[2] = { 0, 0 };
-	if (ggbk_d2h(stats, r->total, sizeof(stats)))
-		return -1;
-	if (stats[0] > INT_MAX / 2)
-		return -1;
-	if (stats[0] > r->cap) {
-		long long ncap = stats[0] + stats[0] / 4 + 1024;
-		ggbk_free(r->list);
-		ggbk_free(r->sort_scratch);
-		r->list = (int *) ggbk_malloc(sizeof(int) * (size_t) ncap);
-		r->sort_scratch = (unsigned long long *) ggbk_malloc(sizeof(unsigned long long) * 2 * (size_t) ncap);
-		if (!r->list || !r->sort_scratch)
-			return -1;
-		r->cap = ncap;
-		r->scratch_cap = 2 * ncap;
-	}
 	if (in.n > r->nsamp_cap) {
 		ggbk_free(r->nsamp);
 		r->nsamp = (unsigned short *) ggbk_malloc(sizeof(unsigned short) * (size_t) in.n);
@@ -934,10 +951,53 @@ extern "C" long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_par
 			return -1;
 		r->nsamp_cap = in.n;
 	}
+	// home counts, and their scan (the tile counts are zero here: the scan clears the
+	// tile cursors the fill counts into, and the stats)
+	ggbk_probe_begin(GGBK_PROBE_RENDER_COUNT);
+	if (ggbk_zero(r->tile_count, sizeof(int) * (size_t) r->ntiles) ||
+	    ggbk_zero(r->home_count, sizeof(int) * (size_t) r->ntiles))
+		return -1;
+	GGBK_LAUNCH(k_render_count, pb, PBS, 0, in, sg, r->home_count);
+	GGBK_LAUNCH(k_render_scan_local, sb, SCAN_BS, 0, r->ntiles, (const int *) r->tile_count, r->tile_start,
+		    r->tile_cursor, (const int *) r->home_count, r->home_start, r->home_cursor, r->scan_part);
+	GGBK_LAUNCH(k_render_scan_fix, sb, SCAN_BS, 0, r->ntiles, r->tile_start, r->home_start,
+		    (const long long *) r->scan_part, r->total);
+	ggbk_probe_end(GGBK_PROBE_RENDER_COUNT);
 	ggbk_probe_begin(GGBK_PROBE_RENDER_FILL);
 	GGBK_LAUNCH(k_render_fill, pb, PBS, 0, in, out, sg, (const int *) r->tile_start, r->tile_cursor,
-		    (const int *) r->home_start, r->home_cursor, r->list, r->cap, r->nsamp);
+		    (const int *) r->home_start, r->home_cursor, r->list, r->cap, r->bucket, r->nsamp);
+	GGBK_LAUNCH(k_render_bucket_stats, sb, SCAN_BS, 0, r->ntiles, (const int *) r->tile_cursor, r->total);
 	ggbk_probe_end(GGBK_PROBE_RENDER_FILL);
+	long long stats[2] = { 0, 0 };
+	if (ggbk_d2h(stats, r->total, sizeof(stats)))
+		return -1;
+	if (stats[0] > INT_MAX / 2)
+		return -1;
+	r->bucket_mode = stats[1] <= BUCKET;
+	if (!r->bucket_mode) {
+		// a tile holds more than a bucket: pack the exact counts into one list
+		if (stats[0] > r->cap) {
+			long long ncap = stats[0] + stats[0] / 4 + 1024;
+			ggbk_free(r->list);
+			ggbk_free(r->sort_scratch);
+			r->list = (int *) ggbk_malloc(sizeof(int) * (size_t) ncap);
+			r->sort_scratch =
+				(unsigned long long *) ggbk_malloc(sizeof(unsigned long long) * 2 * (size_t) ncap);
+			if (!r->list || !r->sort_scratch)
+				return -1;
+			r->cap = ncap;
+			r->scratch_cap = 2 * ncap;
+		}
+		if (ggbk_d2d(r->tile_count, r->tile_cursor, sizeof(int) * (size_t) r->ntiles))
+			return -1;
+		GGBK_LAUNCH(k_render_scan_local, sb, SCAN_BS, 0, r->ntiles, (const int *) r->tile_count,
+			    r->tile_start, r->tile_cursor, (const int *) r->home_count, r->home_start, r->home_cursor,
+			    r->scan_part);
+		GGBK_LAUNCH(k_render_scan_fix, sb, SCAN_BS, 0, r->ntiles, r->tile_start, r->home_start,
+			    (const long long *) r->scan_part, r->total);
+		GGBK_LAUNCH(k_render_fill, pb, PBS, 0, in, out, sg, (const int *) r->tile_start, r->tile_cursor,
+			    (const int *) r->home_start, r->home_cursor, r->list, r->cap, (int *) nullptr, r->nsamp);
+	}
 	ggbk_probe_begin(GGBK_PROBE_RENDER_DRAW);
 	GGBK_LAUNCH(k_render_draw, (unsigned) r->ntiles, DBS, 0, *r, in, sg, invg);
 	ggbk_probe_end(GGBK_PROBE_RENDER_DRAW);
