@@ -479,6 +479,24 @@ static ggbk_xfer build_transfer(struct ggb_fluid_dev *D, const CubeSphere *fine,
 	d = xbuf(D, sizeof(float) * 4 * (size_t) nf, "transfer");
 	upload(d, pw, sizeof(float) * 4 * (size_t) nf);
 	x.pw = d;
+	if (x.analytic) {
+		/* the weights of the face-interior rows alone, coalesced per entry group */
+		float *w16 = calloc(16 * (size_t) nc, sizeof(float));
+		if (!w16)
+			ggb_die("fluid: out of memory");
+		int Nc = coarse->N;
+		for (int c = 0; c < nc; c++) {
+			int J = (c / Nc) % Nc, I = c % Nc;
+			if (I < 1 || I > Nc - 2 || J < 1 || J > Nc - 2)
+				continue;
+			for (int e = 0; e < 16; e++)
+				w16[((size_t) (e / 4) * nc + c) * 4 + e % 4] = rw[rstart[c] + e];
+		}
+		d = xbuf(D, sizeof(float) * 16 * (size_t) nc, "transfer");
+		upload(d, w16, sizeof(float) * 16 * (size_t) nc);
+		x.rw16 = d;
+		free(w16);
+	}
 	d = xbuf(D, sizeof(int) * 2 * (size_t) rpad * nc, "transfer");
 	upload(d, rr, sizeof(int) * 2 * (size_t) rpad * nc);
 	x.rr = d;

@@ -48,6 +48,8 @@ typedef struct {
 	int Nc, analytic;  /* coarse face size; indices of face-interior rows follow the
 	                    * bilinear pattern (only the weights are read for those) */
 	const float *pw;   /* [nf][4] weights of each fine cell's taps */
+	const float *rw16; /* [4][nc][4] weights of the 16 entries of face-interior rows of R, in
+	                    * row order (entry e in element e / 4, lane e % 4) */
 	const int *pt;     /* [nf][4][2] coarse tap and weight of each fine cell */
 	const int *rr;     /* [rpad/2][nc][4] rows of R = P^T, two (fine index, weight bits) pairs
 	                    * per element, column-major over rows (coalesced across a wave); fine

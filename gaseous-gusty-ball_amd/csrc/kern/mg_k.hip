@@ -105,6 +105,40 @@ __global__ __launch_bounds__(BS) void k_mg_restrict(ggbk_xfer xf, const TM *rf, 
 	int c = xblock() * BS + threadIdx.x;
 	if (c >= xf.nc)
 		return;
+	if (xf.analytic) {
+		// face-interior row: the 4x4 fine block around the coarse cell, in fine-index
+		// order, with the row's weights read as four coalesced 16-byte groups
+		int Nc = xf.Nc, Nf = 2 * Nc;
+		int f = c / (Nc * Nc), rem = c - f * Nc * Nc;
+		int J = rem / Nc, I = rem - J * Nc;
+		if (I >= 1 && I <= Nc - 2 && J >= 1 && J <= Nc - 2) {
+			const float4 *wq = (const float4 *) xf.rw16;
+			size_t nc = (size_t) xf.nc;
+			float4 wv[4];
+			TM v[16];
+			int i0 = (f * Nf + 2 * J - 1) * Nf + 2 * I - 1;
+#pragma unroll
+			for (int q = 0; q < 4; q++) {
+				wv[q] = wq[q * nc + c];
+				const TM *row = rf + i0 + q * Nf;
+				v[4 * q] = row[0];
+				v[4 * q + 1] = row[1];
+				v[4 * q + 2] = row[2];
+				v[4 * q + 3] = row[3];
+			}
+			double acc = 0;
+#pragma unroll
+			for (int q = 0; q < 4; q++) {
+				acc += (double) (wv[q].x * v[4 * q]);
+				acc += (double) (wv[q].y * v[4 * q + 1]);
+				acc += (double) (wv[q].z * v[4 * q + 2]);
+				acc += (double) (wv[q].w * v[4 * q + 3]);
+			}
+			if (act)
+				bc[c] = (TM) acc;
+			return;
+		}
+	}
 	const int4 *rows = (const int4 *) xf.rr;
 	size_t nc = (size_t) xf.nc;
 	int idx[RP];
