@@ -45,36 +45,30 @@ __device__ __forceinline__ void store_partial_max(ggbk_red red, int slot, double
 // ---------------- fold kernel ----------------
 
 // Fold nb partials: thread t sums p[t], p[t + FIN_BS], ... in order, then the block
-// tree; fixed order for a given nb.  Loads are issued four at a time so a fold costs
-// about one round trip to L2 per four partials per thread.
-__device__ __forceinline__ double fold_sum(const double *p, int nb)
+// tree; fixed order for a given nb.  Loads are issued eight at a time (the last batch
+// from clamped indices, masked), so a fold of up to 8 FIN_BS partials costs one round
+// trip to L2.
+#define FOLD_BATCH 8
+template <bool MAX>
+__device__ __forceinline__ double fold(const double *p, int nb)
 {
 	double acc = 0;
 	int i = threadIdx.x;
-	for (; i + 3 * FIN_BS < nb; i += 4 * FIN_BS) {
-		double a0 = p[i], a1 = p[i + FIN_BS], a2 = p[i + 2 * FIN_BS], a3 = p[i + 3 * FIN_BS];
-		acc += a0;
-		acc += a1;
-		acc += a2;
-		acc += a3;
+	for (; i < nb; i += FOLD_BATCH * FIN_BS) {
+		double v[FOLD_BATCH];
+#pragma unroll
+		for (int q = 0; q < FOLD_BATCH; q++)
+			v[q] = p[min(i + q * FIN_BS, nb - 1)];
+#pragma unroll
+		for (int q = 0; q < FOLD_BATCH; q++)
+			if (i + q * FIN_BS < nb)
+				acc = MAX ? fmax(acc, v[q]) : acc + v[q];
 	}
-	for (; i < nb; i += FIN_BS)
-		acc += p[i];
-	return block_sum<FIN_BS>(acc);
+	return MAX ? block_max<FIN_BS>(acc) : block_sum<FIN_BS>(acc);
 }
 
-__device__ __forceinline__ double fold_max(const double *p, int nb)
-{
-	double acc = 0;
-	int i = threadIdx.x;
-	for (; i + 3 * FIN_BS < nb; i += 4 * FIN_BS) {
-		double a0 = p[i], a1 = p[i + FIN_BS], a2 = p[i + 2 * FIN_BS], a3 = p[i + 3 * FIN_BS];
-		acc = fmax(fmax(acc, a0), fmax(a1, fmax(a2, a3)));
-	}
-	for (; i < nb; i += FIN_BS)
-		acc = fmax(acc, p[i]);
-	return block_max<FIN_BS>(acc);
-}
+__device__ __forceinline__ double fold_sum(const double *p, int nb) { return fold<false>(p, nb); }
+__device__ __forceinline__ double fold_max(const double *p, int nb) { return fold<true>(p, nb); }
 
 __global__ __launch_bounds__(FIN_BS) void k_fin(int op, ggbk_scalars *s, ggbk_red red, int nb, double tol, int maxit,
 						 double total_area)
