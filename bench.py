@@ -34,6 +34,8 @@ CONFIGS = {
                precision=ggb.PREC_FP64, octaves=4),
     "c0": dict(N=64, P=100_000, R=256, forcing=0.0, rotation=0.0, cg_tol=1e-4,
                precision=ggb.PREC_FP64, octaves=4),
+    "c4": dict(N=1024, P=16_000_000, R=4096, forcing=0.2, rotation=5.0, cg_tol=1e-8,
+               precision=ggb.PREC_MIXED, octaves=6),
 }
 HBM_PEAK = 8.0e12
 
@@ -160,6 +162,76 @@ def cpu_baseline(cfg, steps):
     return res
 
 
+def hbm_used_bytes():
+    """Device memory in use (total - free, hipMemGetInfo), or None."""
+    try:
+        hip = C.CDLL("libamdhip64.so")
+        free, total = C.c_size_t(), C.c_size_t()
+        if hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+            return None
+        return total.value - free.value
+    except OSError:
+        return None
+
+
+def precision_leg(cfg, checkpoints=(10, 100, 2000), seed=1):
+    """Mixed (FP32 V-cycle in FP64 CG) against FP64 on the same workload: both fluids and
+    their particles stepped in lockstep with the FP64 run's dt; relative L2 errors of
+    omega, psi and u and the largest particle position error (rad) at the checkpoints."""
+    g = ggb.Grid(cfg["N"])
+    vel0, vband = g.initial_velocity(seed=seed, octaves=cfg["octaves"])
+    runs = []
+    for prec in (ggb.PREC_FP64, ggb.PREC_MIXED):
+        F = ggb.Fluid(g, forcing=cfg["forcing"], rotation=cfg["rotation"], cg_tol=cfg["cg_tol"],
+                      cg_maxit=500, precision=prec)
+        F.set_state(vel0, vband if cfg["forcing"] > 0 else None)
+        runs.append((F, ggb.Particles(cfg["P"])))
+    rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    out = {}
+    for step in range(1, max(checkpoints) + 1):
+        dt = g.h / max(runs[0][0].max_speed(), 1e-6)
+        for F, P in runs:
+            P.advect(g, F, dt)
+            F.step(dt)
+        if step in checkpoints:
+            (F64, P64), (Fm, Pm) = runs
+            e = {}
+            for name, fld in (("omega", ggb.FIELD_OMEGA), ("psi", ggb.FIELD_PSI), ("u", ggb.FIELD_VEL)):
+                e[name] = rel(Fm.get(fld), F64.get(fld))
+            e["particles_max_rad"] = float(np.max(np.abs(Pm.get()[0] - P64.get()[0])))
+            out[str(step)] = e
+            log("mixed vs fp64 at step %d: %s" % (step, e))
+    for F, P in runs:
+        P.free()
+        F.free()
+    return out
+
+
+def adjoint_window(cfg, steps=100, K=25, seed=1):
+    """Adjoint window of the large workload: forward and backward over `steps` steps of
+    the mixed-precision fluid with checkpoints every K, timed."""
+    g = ggb.Grid(cfg["N"])
+    vel0, vband = g.initial_velocity(seed=seed, octaves=cfg["octaves"])
+    F = ggb.Fluid(g, forcing=cfg["forcing"], rotation=cfg["rotation"], cg_tol=1e-10, cg_maxit=2000,
+                  precision=cfg["precision"])
+    F.set_state(vel0, vband)
+    om0 = F.get(ggb.FIELD_OMEGA)
+    dts = np.full(steps, g.h / max(F.max_speed(), 1e-6))
+    A = ggb.FluidAdjoint(F, K)
+    area = g.area.astype(np.float64)
+    t0 = time.perf_counter()
+    wT = A.forward(om0, dts)
+    t1 = time.perf_counter()
+    g0, gk = A.backward(area * (wT - om0))
+    t2 = time.perf_counter()
+    res = {"workload": "N=%d, %d steps, checkpoint every %d, CG tol 1e-10" % (cfg["N"], steps, K),
+           "forward_ms": round((t1 - t0) * 1000, 1), "backward_ms": round((t2 - t1) * 1000, 1),
+           "grad_finite": bool(np.all(np.isfinite(g0))), "dL_dforcing": gk}
+    A.free()
+    F.free()
+    return res
+
+
 def adjoint_leg(N=128, steps=50, K=10, seed=1, fd_steps=10):
     """Discrete adjoint at N=128 over 50 FP64 steps with checkpoints every 10: timed
     forward and backward; and a central-difference check of the gradient along one random
@@ -218,6 +290,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the mixed-vs-FP64 error run (steps 10, 100, 2000)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain copy kernel of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -269,6 +343,7 @@ def main():
     dms = C.c_double()
     dcount = L.ggbk_probe_collect(PROBES[dominant], C.byref(dms))
     L.ggbk_probe_enable(PROBES[dominant], 0)
+    hbm_peak = hbm_used_bytes()
     rstats = R.last_stats()
     K = args.steps
     ms_fluid = acc["f"] / K
@@ -337,6 +412,8 @@ def main():
                      "algo_bytes_per_launch": int(nbytes)},
         "kernels_warmup": kernels,
     }
+    if hbm_peak is not None:
+        out["hbm_used_GB"] = round(hbm_peak / 1e9, 2)
     F.free()
     P.free()
     R.free()
@@ -353,9 +430,17 @@ def main():
         out["mixed_over_fp64"] = round(fluid_sps / fp64_sps, 3)
         F2.free()
 
+    if not args.no_precision and cfg["precision"] == ggb.PREC_MIXED and args.config == "c2":
+        t0 = time.perf_counter()
+        out["mixed_vs_fp64_error"] = precision_leg(cfg)
+        log("precision leg %.1fs" % (time.perf_counter() - t0))
+
     if not args.no_adjoint:
         t0 = time.perf_counter()
-        out["adjoint"] = adjoint_leg()
+        if args.config == "c4":
+            out["adjoint_window"] = adjoint_window(cfg)
+        else:
+            out["adjoint"] = adjoint_leg()
         log("adjoint leg %.1fs" % (time.perf_counter() - t0))
 
     if not args.no_cpu:
