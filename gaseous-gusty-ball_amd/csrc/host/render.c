@@ -20,6 +20,8 @@ struct ggb_render_dev {
 	unsigned long gen_dev;  /* bumped by every change of the device buffer */
 	unsigned long gen_host; /* generation the mirror holds */
 	pthread_mutex_t mu;
+	unsigned char *faces8;  /* device 8-bit faces (render_faces8) */
+	size_t faces8_bytes;
 };
 
 int render_init(Render *r, int R, float radius, float alpha)
@@ -108,6 +110,7 @@ void render_free(Render *r)
 		ggbk_free(D->k.tile_diag);
 		ggbk_free(D->k.scan_part);
 		ggbk_free(D->k.bucket);
+		ggbk_free(D->faces8);
 		ggbk_free(D->k.list);
 		ggbk_free(D->k.nsamp);
 		ggbk_free(D->k.sort_scratch);
@@ -235,16 +238,19 @@ void render_pixel8(const float c[4], const float *bg, unsigned char *out)
 	}
 }
 
+/* The whole buffer through render_texel + render_pixel8, converted on the device: the
+ * copy back is the 8-bit faces (a quarter or less of the float buffer). */
 void render_faces8(Render *r, const float *bg, unsigned char *out)
 {
-	refresh(r);
-	int R = r->R, ch = bg ? 3 : 4;
-	#pragma omp parallel for schedule(static)
-	for (int f = 0; f < 6; f++)
-		for (int y = 0; y < R; y++)
-			for (int x = 0; x < R; x++) {
-				float c[4];
-				render_texel(r, f, x, y, c);
-				render_pixel8(c, bg, out + (((size_t) f * R + y) * R + x) * ch);
-			}
+	struct ggb_render_dev *D = r->dev;
+	int ch = bg ? 3 : 4;
+	size_t ntex = 6 * (size_t) r->R * r->R, bytes = ntex * ch;
+	if (D->faces8_bytes < ntex * 4) {
+		ggbk_free(D->faces8);
+		D->faces8 = ggb_dmalloc(ntex * 4, "render faces8");
+		D->faces8_bytes = ntex * 4;
+	}
+	ggbk_render_faces8(D->k.buf, ntex, (float) r->g, bg, D->faces8);
+	if (ggbk_d2h(out, D->faces8, bytes))
+		ggb_die("render: %s", ggbk_last_error());
 }

@@ -913,6 +913,56 @@ extern "C" void ggbk_render_scale(float *buf, size_t count, float g)
 	GGBK_LAUNCH(k_render_scale, (unsigned) blocks, 256, 0, buf, count, g);
 }
 
+// reference render.c:253-258
+__device__ __forceinline__ unsigned char to8(float v)
+{
+	if (v <= 0)
+		return 0;
+	if (v >= 1)
+		return 255;
+	return (unsigned char) (v * 255.0f + 0.5f);
+}
+
+// One texel per thread: c = texel * g (render.c:220-226), then render_pixel8
+// (render.c:260-279) in the same float operations.
+__global__ void k_render_faces8(const float4 *buf, size_t ntex, float g, int has_bg, float bg0, float bg1, float bg2,
+				unsigned char *out)
+{
+	size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= ntex)
+		return;
+	float4 d = buf[i];
+	float c0 = d.x * g, c1 = d.y * g, c2 = d.z * g, c3 = d.w * g;
+	float a = c3;
+	if (a > 1)
+		a = 1;
+	if (a < 0)
+		a = 0;
+	if (!has_bg) {
+		uchar4 o;
+		if (a > 1e-6f) {
+			o.x = to8(c0 / a);
+			o.y = to8(c1 / a);
+			o.z = to8(c2 / a);
+		} else {
+			o.x = o.y = o.z = 0;
+		}
+		o.w = to8(a);
+		((uchar4 *) out)[i] = o;
+	} else {
+		out[3 * i] = to8(c0 + bg0 * (1 - a));
+		out[3 * i + 1] = to8(c1 + bg1 * (1 - a));
+		out[3 * i + 2] = to8(c2 + bg2 * (1 - a));
+	}
+}
+
+extern "C" void ggbk_render_faces8(const float *buf, size_t ntexels, float g, const float *bg, unsigned char *out)
+{
+	unsigned nb = (unsigned) ((ntexels + 255) / 256);
+	GGBK_LAUNCH(k_render_faces8, nb, 256, 0, (const float4 *) buf, ntexels, g, bg ? 1 : 0, bg ? bg[0] : 0.0f,
+		    bg ? bg[1] : 0.0f, bg ? bg[2] : 0.0f, out);
+}
+
 extern "C" int ggbk_render_tile(void) { return TILE; }
 
 extern "C" int ggbk_render_bucket(void) { return BUCKET; }

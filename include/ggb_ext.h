@@ -76,7 +76,9 @@ long long render_last_entries(const Render *r); /* (particle, tile) entries of t
  * [5..7] draw-pass cycle sums (samples, blend, total), [8] texels touched, [9] cycles of
  * the window set-up (part of [5]) */
 int render_last_stats(const Render *r, long long *out16);
-/* 8-bit faces [6][R][R][4] (bg NULL: un-premultiplied RGBA) computed on the host mirror. */
+/* 8-bit faces [6][R][R][4] (bg NULL: un-premultiplied RGBA) or [6][R][R][3] (composited
+ * over bg), texel by texel as render_texel then render_pixel8 (reference render.c:220-226,
+ * 253-279), converted on the device; only the 8-bit result is copied back. */
 void render_faces8(Render *r, const float *bg, unsigned char *out);
 
 /* ---- discrete adjoint of fluid_step ----

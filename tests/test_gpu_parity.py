@@ -248,3 +248,44 @@ def test_render_crowded_and_antipodal(ggb, orc, gpu):
     assert np.max(np.abs(buf - OR.buf)) < 1e-5
     a, b = Rr.faces8().astype(int), OR.faces8().astype(int)
     assert np.max(np.abs(a - b)) <= 1
+
+
+def _faces8_host(buf, g, bg=None):
+    """render_texel then render_pixel8 (reference render.c:220-226, 253-279) in float32."""
+    c = buf * np.float32(g)
+    a = np.clip(c[..., 3], np.float32(0), np.float32(1))
+
+    def to8(v):
+        out = (v * np.float32(255.0) + np.float32(0.5)).astype(np.float32)
+        out = np.where(v >= 1, np.float32(255), out)
+        out = np.where(v <= 0, np.float32(0), out)
+        return out.astype(np.uint8)
+
+    if bg is None:
+        ok = a > np.float32(1e-6)
+        safe = np.where(ok, a, np.float32(1))
+        rgb = [np.where(ok, to8(c[..., k] / safe), 0).astype(np.uint8) for k in range(3)]
+        return np.stack(rgb + [to8(a)], axis=-1)
+    return np.stack([to8(c[..., k] + np.float32(bg[k]) * (np.float32(1) - a)) for k in range(3)], axis=-1)
+
+
+def test_render_faces8_device(ggb, gpu):
+    """The device 8-bit conversion equals render_texel + render_pixel8 texel by texel."""
+    g = ggb.Grid(32)
+    vel0, _ = g.initial_velocity(seed=6, octaves=2)
+    F = ggb.Fluid(g)
+    F.set_state(vel0)
+    P = ggb.Particles(20000)
+    P.color_from_image(ggb.strip_to_float(ggb.banded_strip(6)), 64, 1024)
+    Rr = ggb.Render(128)
+    for _ in range(4):
+        dt = g.h / max(F.max_speed(), 1e-6)
+        P.advect(g, F, dt)
+        F.step(dt)
+        Rr.fade(0.3)
+        Rr.deposit(P)
+    buf = Rr.sync_host().copy()
+    assert Rr.g < 1.0
+    assert np.array_equal(Rr.faces8(), _faces8_host(buf, Rr.g))
+    bg = [0.1, 0.2, 0.7]
+    assert np.array_equal(Rr.faces8(bg), _faces8_host(buf, Rr.g, bg))
