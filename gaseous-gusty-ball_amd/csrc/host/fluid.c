@@ -471,7 +471,9 @@ static ggbk_xfer build_transfer(struct ggb_fluid_dev *D, const CubeSphere *fine,
 	x.rmax = rmax;
 	x.rpad = rpad;
 	x.Nc = coarse->N;
-	x.analytic = transfer_is_analytic(fine, coarse, ptap, rstart, ridx, rw, pw);
+	/* computed indices pay off where the tables stream from HBM; the small levels' tables
+	 * stay in L2, and there the index arithmetic costs more than the loads it saves */
+	x.analytic = nc >= 65536 && transfer_is_analytic(fine, coarse, ptap, rstart, ridx, rw, pw);
 	void *d;
 	d = xbuf(D, sizeof(int) * 8 * (size_t) nf, "transfer");
 	upload(d, pt, sizeof(int) * 8 * (size_t) nf);
