@@ -22,7 +22,11 @@ struct ggb_render_dev {
 	pthread_mutex_t mu;
 	unsigned char *faces8;  /* device 8-bit faces (render_faces8) */
 	size_t faces8_bytes;
+	unsigned long ndeposits;
 };
+#ifndef GGB_REORDER_EVERY
+#define GGB_REORDER_EVERY 4
+#endif
 
 int render_init(Render *r, int R, float radius, float alpha)
 {
@@ -140,10 +144,13 @@ void render_deposit(Render *r, const Particles *P)
 		return;
 	ggb_particles_dev_parts(P, &cur, &nxt);
 	float invg = (float) (1.0 / r->g);
-	long long e = ggbk_render_deposit(&r->dev->k, cur, nxt, invg);
+	/* particles move a few texels per step: tile order is restored every few deposits */
+	int reorder = r->dev->ndeposits++ % GGB_REORDER_EVERY == 0;
+	long long e = ggbk_render_deposit(&r->dev->k, cur, nxt, invg, reorder);
 	if (e < 0)
 		ggb_die("render: deposit failed (%s)", ggbk_last_error());
-	ggb_particles_dev_swap(P);
+	if (reorder)
+		ggb_particles_dev_swap(P);
 	r->dev->last_entries = e;
 	r->dev->gen_dev++;
 }

@@ -275,9 +275,10 @@ void ggbk_render_faces8(const float *buf, size_t ntexels, float g, const float *
  * (synchronises) */
 int ggbk_render_stats(const ggbk_render *r, long long *out16);
 int ggbk_render_bucket(void);  /* slots per tile bucket */
-/* bin + ordered blend of the particles `in`; writes them to `out` in the next storage
- * (tile) order.  Returns the number of (particle, tile) entries, or -1 on error. */
-long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_parts out, float invg);
+/* bin + ordered blend of the particles `in`; with reorder != 0 also writes them to `out`
+ * in the next storage (tile) order.  Returns the number of (particle, tile) entries, or
+ * -1 on error. */
+long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_parts out, float invg, int reorder);
 
 #ifdef __cplusplus
 }

@@ -287,15 +287,17 @@ __global__ __launch_bounds__(PBS) void k_render_fill(ggbk_parts ps, ggbk_parts o
 	// whose FP64 atan2 would double the draw kernel's registers
 	if (on)
 		nsamp[i] = (unsigned short) arc_samples(vangle(a, b), sg.spacing);
-	// the particle's place in the next storage order
-	int home = on ? home_tile(b, sg.R, sg.ntx) : 0;
-	int hr = wave_atomic_inc(hcursor, home, on);
-	if (on) {
-		size_t o = (size_t) hstarts[home] + hr;
-		st3(out.pos + 3 * o, b);
-		st3(out.prev + 3 * o, a);
-		*(float4 *) (out.col + 4 * o) = *(const float4 *) (ps.col + 4 * (size_t) i);
-		out.pid[o] = ps.pid[i];
+	// the particle's place in the next storage order (hcursor NULL: no reordering)
+	if (hcursor) {
+		int home = on ? home_tile(b, sg.R, sg.ntx) : 0;
+		int hr = wave_atomic_inc(hcursor, home, on);
+		if (on) {
+			size_t o = (size_t) hstarts[home] + hr;
+			st3(out.pos + 3 * o, b);
+			st3(out.prev + 3 * o, a);
+			*(float4 *) (out.col + 4 * o) = *(const float4 *) (ps.col + 4 * (size_t) i);
+			out.pid[o] = ps.pid[i];
+		}
 	}
 	for_arc_tiles(on, a, b, sg, [&](bool has, int t) {
 		int r = wave_atomic_inc(cursor, t, has);
@@ -976,12 +978,14 @@ extern "C" int ggbk_render_stats(const ggbk_render *r, long long *out16)
 }
 
 // Host side of one deposit.  Returns the number of (particle, tile) entries, or -1.
+// With reorder, the particles are counted per home tile first and the fill also copies
+// them to `out` in that order (the caller adopts it once this returns); without, they
+// stay where they are (the storage order does not change the result, only locality).
 // The fill writes the tile lists into fixed-size buckets and counts every tile exactly;
 // the one synchronisation reads the entry total and the longest list.  If a tile
 // overflowed its bucket, the counts are scanned into starts and the fill runs again into
-// one packed list (grown when the total exceeds its capacity).  `out` receives the
-// particles in the next storage order; the caller adopts it once this returns.
-extern "C" long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_parts out, float invg)
+// one packed list (grown when the total exceeds its capacity).
+extern "C" long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_parts out, float invg, int reorder)
 {
 	if (in.n <= 0)
 		return 0;
@@ -1001,21 +1005,28 @@ extern "C" long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_par
 			return -1;
 		r->nsamp_cap = in.n;
 	}
-	// home counts, and their scan (the tile counts are zero here: the scan clears the
-	// tile cursors the fill counts into, and the stats)
-	ggbk_probe_begin(GGBK_PROBE_RENDER_COUNT);
-	if (ggbk_zero(r->tile_count, sizeof(int) * (size_t) r->ntiles) ||
-	    ggbk_zero(r->home_count, sizeof(int) * (size_t) r->ntiles))
+	int *hcursor = reorder ? r->home_cursor : nullptr;
+	if (reorder) {
+		// home counts, and their scan (the tile counts are zero here: the scan clears
+		// the tile cursors the fill counts into, and the stats)
+		ggbk_probe_begin(GGBK_PROBE_RENDER_COUNT);
+		if (ggbk_zero(r->tile_count, sizeof(int) * (size_t) r->ntiles) ||
+		    ggbk_zero(r->home_count, sizeof(int) * (size_t) r->ntiles))
+			return -1;
+		GGBK_LAUNCH(k_render_count, pb, PBS, 0, in, sg, r->home_count);
+		GGBK_LAUNCH(k_render_scan_local, sb, SCAN_BS, 0, r->ntiles, (const int *) r->tile_count,
+			    r->tile_start, r->tile_cursor, (const int *) r->home_count, r->home_start, r->home_cursor,
+			    r->scan_part);
+		GGBK_LAUNCH(k_render_scan_fix, sb, SCAN_BS, 0, r->ntiles, r->tile_start, r->home_start,
+			    (const long long *) r->scan_part, r->total);
+		ggbk_probe_end(GGBK_PROBE_RENDER_COUNT);
+	} else if (ggbk_zero(r->tile_cursor, sizeof(int) * (size_t) r->ntiles) ||
+		   ggbk_zero(r->total, sizeof(long long) * 16)) {
 		return -1;
-	GGBK_LAUNCH(k_render_count, pb, PBS, 0, in, sg, r->home_count);
-	GGBK_LAUNCH(k_render_scan_local, sb, SCAN_BS, 0, r->ntiles, (const int *) r->tile_count, r->tile_start,
-		    r->tile_cursor, (const int *) r->home_count, r->home_start, r->home_cursor, r->scan_part);
-	GGBK_LAUNCH(k_render_scan_fix, sb, SCAN_BS, 0, r->ntiles, r->tile_start, r->home_start,
-		    (const long long *) r->scan_part, r->total);
-	ggbk_probe_end(GGBK_PROBE_RENDER_COUNT);
+	}
 	ggbk_probe_begin(GGBK_PROBE_RENDER_FILL);
 	GGBK_LAUNCH(k_render_fill, pb, PBS, 0, in, out, sg, (const int *) r->tile_start, r->tile_cursor,
-		    (const int *) r->home_start, r->home_cursor, r->list, r->cap, r->bucket, r->nsamp);
+		    (const int *) r->home_start, hcursor, r->list, r->cap, r->bucket, r->nsamp);
 	GGBK_LAUNCH(k_render_bucket_stats, sb, SCAN_BS, 0, r->ntiles, (const int *) r->tile_cursor, r->total);
 	ggbk_probe_end(GGBK_PROBE_RENDER_FILL);
 	long long stats[2] = { 0, 0 };
@@ -1046,7 +1057,7 @@ extern "C" long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_par
 		GGBK_LAUNCH(k_render_scan_fix, sb, SCAN_BS, 0, r->ntiles, r->tile_start, r->home_start,
 			    (const long long *) r->scan_part, r->total);
 		GGBK_LAUNCH(k_render_fill, pb, PBS, 0, in, out, sg, (const int *) r->tile_start, r->tile_cursor,
-			    (const int *) r->home_start, r->home_cursor, r->list, r->cap, (int *) nullptr, r->nsamp);
+			    (const int *) r->home_start, hcursor, r->list, r->cap, (int *) nullptr, r->nsamp);
 	}
 	ggbk_probe_begin(GGBK_PROBE_RENDER_DRAW);
 	GGBK_LAUNCH(k_render_draw, (unsigned) r->ntiles, DBS, 0, *r, in, sg, invg);
