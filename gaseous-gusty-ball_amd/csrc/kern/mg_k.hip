@@ -26,6 +26,9 @@
 
 #define BS 256
 #define MG_OMEGA 0.8
+#ifndef TILED_UPDATE
+#define TILED_UPDATE 1
+#endif
 #define MG_COARSE_SWEEPS 60
 #define COARSE_LDS_CELLS 3072
 #define COARSE_BS 1024
@@ -56,6 +59,115 @@ __global__ __launch_bounds__(BS) void k_mg_down(ggbk_grid g, const TB *b, TM *r1
 	for_stencil(g, i, [&](int j, float w) { sum += (double) w * (xi - (double) x1_at<TM, TB>(g, b, j)); });
 	if (act)
 		r1[i] = (TM) ((double) b[i] - sum);
+}
+
+// ---------------- face tiles staged in LDS ----------------
+// A workgroup of BS = 256 threads takes a 16x16 tile of one face (cell (f, J, I) with J, I
+// in the tile) and stages a per-cell value of the tile and its one-cell ring (18x18, the
+// ring only where it lies on the face) in LDS.  A cell whose stencil record holds offset
+// codes (all neighbours on its own face, offsets dy N + dx) then reads its neighbours'
+// values from LDS at (ty + dy, tx + dx); a cell on a face seam (explicit neighbour row)
+// computes them from global memory.  Needs N divisible by 16.
+#define TT 16
+#define TH (TT + 2)
+
+struct FaceTile {
+	int f, J0, I0, ty, tx, c;
+};
+
+__device__ __forceinline__ FaceTile face_tile(int N)
+{
+	FaceTile t;
+	int tpf = (N / TT) * (N / TT);
+	int b = xblock();
+	t.f = b / tpf;
+	int r = b - t.f * tpf;
+	t.J0 = (r / (N / TT)) * TT;
+	t.I0 = (r % (N / TT)) * TT;
+	t.ty = threadIdx.x / TT;
+	t.tx = threadIdx.x % TT;
+	t.c = (t.f * N + t.J0 + t.ty) * N + t.I0 + t.tx;
+	return t;
+}
+
+// Calls val(cell) for the tile's ring cells on the face and stores the results at their
+// LDS places; interior values are stored by the caller at (ty + 1, tx + 1).
+template <typename V, typename Val>
+__device__ __forceinline__ void stage_ring(const FaceTile &t, int N, V *lds, Val val)
+{
+	int h = threadIdx.x;
+	if (h < 4 * TH - 4) {
+		int hy, hx;
+		if (h < TH) {
+			hy = -1;
+			hx = h - 1;
+		} else if (h < 2 * TH) {
+			hy = TT;
+			hx = h - TH - 1;
+		} else if (h < 2 * TH + TT) {
+			hy = h - 2 * TH;
+			hx = -1;
+		} else {
+			hy = h - 2 * TH - TT;
+			hx = TT;
+		}
+		int J = t.J0 + hy, I = t.I0 + hx;
+		if (J >= 0 && J < N && I >= 0 && I < N)
+			lds[(hy + 1) * TH + hx + 1] = val((t.f * N + J) * N + I);
+	}
+}
+
+// The PCG update and level-0 pre-smooth residual of k_cg_update_down on face tiles: each
+// cell's x1 (from r_out = r_in - alpha Ap) is computed once per tile and ring cell.
+template <typename T, typename TM>
+__global__ __launch_bounds__(BS) void k_cg_update_down_tile(ggbk_grid g, T *psi, const T *r_in, T *r_out, const T *p,
+							     const T *Ap, TM *r1, const ggbk_scalars *s, ggbk_red red)
+{
+	__shared__ TM x1s[TH * TH];
+	int act = s->active;
+	double alpha = s->alpha;
+	int N = g.N;
+	FaceTile t = face_tile(N);
+	int i = t.c;
+	double ri = (double) r_in[i] - alpha * (double) Ap[i];
+	T rt = (T) ri;
+	T ps = psi[i] + (T) (alpha * (double) p[i]);
+	TM xi = x1_of<TM, T>(rt, g.dinv[i]);
+	x1s[(t.ty + 1) * TH + t.tx + 1] = xi;
+	auto x1_cell = [&](int j) {
+		T rj = (T) ((double) r_in[j] - alpha * (double) Ap[j]);
+		return x1_of<TM, T>(rj, g.dinv[j]);
+	};
+	stage_ring<TM>(t, N, x1s, x1_cell);
+	__syncthreads();
+	double sum = 0, xid = (double) xi;
+	int2 rc = ((const int2 *) g.srec)[i];
+	if (rc.x >= 0) {
+		const float4 *wp = (const float4 *) (g.scw + 8 * (size_t) rc.x);
+		float4 wa = wp[0], wb = wp[1];
+		float w[8] = { wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w };
+		unsigned code = (unsigned) rc.y;
+		int l0 = (t.ty + 1) * TH + t.tx + 1;
+#pragma unroll
+		for (int k = 0; k < 8; k++) {
+			int dy = (int) ((code >> (4 * k + 2)) & 3u) - 1, dx = (int) ((code >> (4 * k)) & 3u) - 1;
+			if (k < 6 || (dy | dx) != 0)
+				sum += (double) w[k] * (xid - (double) x1s[l0 + dy * TH + dx]);
+		}
+	} else {
+		for_stencil(g, i, [&](int j, float w) { sum += (double) w * (xid - (double) x1_cell(j)); });
+	}
+	if (act) {
+		psi[i] = ps;
+		r_out[i] = rt;
+		r1[i] = (TM) ((double) rt - sum);
+	}
+	double v = ri * ri;
+	if (act) {
+		double bsum = block_sum<BS>(v);
+		if (threadIdx.x == 0)
+			red.part[(size_t) GGBK_SL_RR * red.stride + xblock()] = bsum;
+	}
 }
 
 // PCG update fused with the level-0 pre-smooth residual (fluid.c:336-345 then the first
@@ -414,8 +526,12 @@ static void vcycle_level(const ggbk_level *lv, int nlev, int l, const TB *b, TM 
 	const ggbk_level &C = lv[l + 1];
 	if (u) {
 		ggbk_probe_begin(GGBK_PROBE_CG_UPDATE);
-		GGBK_LAUNCH((k_cg_update_down<TB, TM>), NB(g.n), BS, 0, g, (TB *) u->psi, (const TB *) u->r_in,
-			    (TB *) b, (const TB *) u->p, (const TB *) u->Ap, r, s, red);
+		if (TILED_UPDATE && g.N % TT == 0)
+			GGBK_LAUNCH((k_cg_update_down_tile<TB, TM>), NB(g.n), BS, 0, g, (TB *) u->psi,
+				    (const TB *) u->r_in, (TB *) b, (const TB *) u->p, (const TB *) u->Ap, r, s, red);
+		else
+			GGBK_LAUNCH((k_cg_update_down<TB, TM>), NB(g.n), BS, 0, g, (TB *) u->psi, (const TB *) u->r_in,
+				    (TB *) b, (const TB *) u->p, (const TB *) u->Ap, r, s, red);
 		ggbk_probe_end(GGBK_PROBE_CG_UPDATE);
 		ggbk_fin(GGBK_FIN_RR, s, red, (int) NB(g.n), 0, 0, 0);
 	} else {
