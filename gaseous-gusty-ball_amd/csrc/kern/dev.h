@@ -305,6 +305,89 @@ __device__ __forceinline__ int xblock(void)
 	return x * q + min(x, r) + idx;
 }
 
+// ---------------- face tiles staged in LDS ----------------
+// A workgroup of BS = 256 threads takes a 16x16 tile of one face (cell (f, J, I) with J, I
+// in the tile) and stages a per-cell value of the tile and its one-cell ring (18x18, the
+// ring only where it lies on the face) in LDS.  A cell whose stencil record holds offset
+// codes (all neighbours on its own face, offsets dy N + dx) then reads its neighbours'
+// values from LDS at (ty + dy, tx + dx); a cell on a face seam (explicit neighbour row)
+// computes them from global memory.  Needs N divisible by 16 and workgroups of 256 threads.
+#define TT 16
+#define TH (TT + 2)
+
+struct FaceTile {
+	int f, J0, I0, ty, tx, c;
+};
+
+__device__ __forceinline__ FaceTile face_tile(int N)
+{
+	FaceTile t;
+	int tpf = (N / TT) * (N / TT);
+	int b = xblock();
+	t.f = b / tpf;
+	int r = b - t.f * tpf;
+	t.J0 = (r / (N / TT)) * TT;
+	t.I0 = (r % (N / TT)) * TT;
+	t.ty = threadIdx.x / TT;
+	t.tx = threadIdx.x % TT;
+	t.c = (t.f * N + t.J0 + t.ty) * N + t.I0 + t.tx;
+	return t;
+}
+
+// Calls val(cell) for the tile's ring cells on the face and stores the results at their
+// LDS places; interior values are stored by the caller at (ty + 1, tx + 1).
+template <typename V, typename Val>
+__device__ __forceinline__ void stage_ring(const FaceTile &t, int N, V *lds, Val val)
+{
+	int h = threadIdx.x;
+	if (h < 4 * TH - 4) {
+		int hy, hx;
+		if (h < TH) {
+			hy = -1;
+			hx = h - 1;
+		} else if (h < 2 * TH) {
+			hy = TT;
+			hx = h - TH - 1;
+		} else if (h < 2 * TH + TT) {
+			hy = h - 2 * TH;
+			hx = -1;
+		} else {
+			hy = h - 2 * TH - TT;
+			hx = TT;
+		}
+		int J = t.J0 + hy, I = t.I0 + hx;
+		if (J >= 0 && J < N && I >= 0 && I < N)
+			lds[(hy + 1) * TH + hx + 1] = val((t.f * N + J) * N + I);
+	}
+}
+
+// sum_k w_k (xi - v_k) over cell t.c's stencil in slot order, the neighbour values v from
+// the staged tile (cells with offset codes) or from gval(j) (seam cells); the same terms,
+// in the same order, as for_stencil with v = gval(j).  Call after the LDS is staged.
+template <typename V, typename GVal>
+__device__ __forceinline__ double tile_stencil_sum(const ggbk_grid &g, const FaceTile &t, const V *lds, double xi,
+						   GVal gval)
+{
+	double sum = 0;
+	int2 rc = ((const int2 *) g.srec)[t.c];
+	if (rc.x >= 0) {
+		const float4 *wp = (const float4 *) (g.scw + 8 * (size_t) rc.x);
+		float4 wa = wp[0], wb = wp[1];
+		float w[8] = { wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w };
+		unsigned code = (unsigned) rc.y;
+		int l0 = (t.ty + 1) * TH + t.tx + 1;
+#pragma unroll
+		for (int k = 0; k < 8; k++) {
+			int dy = (int) ((code >> (4 * k + 2)) & 3u) - 1, dx = (int) ((code >> (4 * k)) & 3u) - 1;
+			if (k < 6 || (dy | dx) != 0)
+				sum += (double) w[k] * (xi - (double) lds[l0 + dy * TH + dx]);
+		}
+	} else {
+		for_stencil(g, t.c, [&](int j, float w) { sum += (double) w * (xi - (double) gval(j)); });
+	}
+	return sum;
+}
+
 // Kernel launch error check helper for host launchers.
 #define GGBK_LAUNCH(kern, grid, block, shm, ...)                                           \
 	hipLaunchKernelGGL(kern, dim3(grid), dim3(block), shm, ggbk_stream(), __VA_ARGS__)

@@ -18,6 +18,9 @@
 
 #define BS 256
 #define FIN_BS 1024
+#ifndef TILED_AP
+#define TILED_AP 1
+#endif
 
 template <typename T>
 __device__ __forceinline__ double applyK_at(const ggbk_grid &g, const T *x, int i)
@@ -303,6 +306,31 @@ __global__ __launch_bounds__(BS) void k_cg_Ap(ggbk_grid g, const TZ *z, const T 
 		store_partial(red, GGBK_SL_PAP, v);
 }
 
+// k_cg_Ap after the first iteration on face tiles: p_out = z + beta p_in is computed once
+// per tile and ring cell and staged in LDS, where the stencil reads it.
+template <typename T, typename TZ>
+__global__ __launch_bounds__(BS) void k_cg_Ap_tile(ggbk_grid g, const TZ *z, const T *p_in, T *p_out, T *Ap,
+						    const ggbk_scalars *s, ggbk_red red)
+{
+	__shared__ T ps[TH * TH];
+	int act = s->active;
+	double beta = s->beta;
+	FaceTile t = face_tile(g.N);
+	auto p_cell = [&](int j) { return (T) ((double) z[j] + beta * (double) p_in[j]); };
+	T pi = p_cell(t.c);
+	ps[(t.ty + 1) * TH + t.tx + 1] = pi;
+	stage_ring<T>(t, g.N, ps, p_cell);
+	__syncthreads();
+	double xi = (double) pi;
+	double sum = tile_stencil_sum<T>(g, t, ps, xi, p_cell);
+	if (act) {
+		p_out[t.c] = pi;
+		Ap[t.c] = (T) sum;
+	}
+	if (act)
+		store_partial(red, GGBK_SL_PAP, sum * xi);
+}
+
 template <typename T>
 __global__ __launch_bounds__(BS) void k_cg_update(int n, T *psi, T *r, const T *p, const T *Ap, const ggbk_scalars *s,
 						   ggbk_red red)
@@ -575,6 +603,9 @@ static void launch_cg_Ap(ggbk_grid g, const void *z, const void *p_in, void *p_o
 {
 	if (first)
 		GGBK_LAUNCH((k_cg_Ap<T, TZ, true>), NB(g.n), BS, 0, g, (const TZ *) z, (const T *) p_in, (T *) p_out,
+			    (T *) Ap, s, red);
+	else if (TILED_AP && g.N % TT == 0)
+		GGBK_LAUNCH((k_cg_Ap_tile<T, TZ>), NB(g.n), BS, 0, g, (const TZ *) z, (const T *) p_in, (T *) p_out,
 			    (T *) Ap, s, red);
 	else
 		GGBK_LAUNCH((k_cg_Ap<T, TZ, false>), NB(g.n), BS, 0, g, (const TZ *) z, (const T *) p_in, (T *) p_out,
