@@ -564,8 +564,9 @@ __device__ __forceinline__ void sort_net(unsigned *k)
 //      exclusive prefix of ns in rank order;
 //   then per range of consecutive ranks whose samples fit SCAP (ranges are in index order):
 //   C  one thread per (particle, sample), its rank read from a per-range sample map: the
-//      sample's pixel on this tile's face; samples on the tile take a slot (position + key
-//      rank<<12 | k-1) and append the slot to every texel they touch;
+//      sample's pixel on this tile's face; a sample on the tile fills the slot of its
+//      index in the range (position + key rank<<12 | k-1) and appends the slot to every
+//      texel it touches;
 //   D  one thread per texel: sort its fragments by key and blend them in that order.
 // A range in which a texel collects more than CAPT fragments is retried with half the
 // particles; a single particle that still overflows (or alone has more than SCAP
@@ -589,7 +590,7 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 	__shared__ unsigned short pns[WIN];        // sample count by rank (<= 4096)
 	__shared__ unsigned char smap[SCAP];       // rank - r0 of each sample of a range
 	__shared__ float4 pcol[WIN];               // by rank
-	__shared__ int overflow, nsamp, wtot, ntouched;
+	__shared__ int overflow, wtot, ntouched;
 	long long c0 = clock64();
 	int t = xblock();
 	const int lid = threadIdx.x;
@@ -723,10 +724,8 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 				bool big = j1 - j0 > SCAP;
 				for (int i = lid; i < NTEX; i += DBS)
 					tcnt[i] = 0;
-				if (lid == 0) {
+				if (lid == 0)
 					overflow = big ? 1 : 0;
-					nsamp = 0;
-				}
 				// the rank of every sample of the range (r1 - r0 <= WIN = DBS)
 				if (!big && r0 + lid < r1) {
 					int p = r0 + lid;
@@ -744,11 +743,7 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 					int xlo, xhi, ylo, yhi;
 					if (!tile_clip(tr, px, py, rad, sg.R, xlo, xhi, ylo, yhi))
 						continue;
-					int sl = atomicAdd(&nsamp, 1);
-					if (sl >= SCAP) {
-						overflow = 1;
-						continue;
-					}
+					int sl = j - j0; // the range's samples fit SCAP slots
 					sxy[sl] = make_float2(px, py);
 					skey[sl] = ((unsigned) p << 12) | (unsigned) (k - 1);
 					bool over = false;
