@@ -99,6 +99,24 @@ __global__ __launch_bounds__(BS) void k_cg_update_down_tile(ggbk_grid g, T *psi,
 	}
 }
 
+// k_mg_down on face tiles (x1 of each tile and ring cell staged in LDS), for level 0 where
+// b is the FP64 CG residual
+template <typename TM, typename TB>
+__global__ __launch_bounds__(BS) void k_mg_down_tile(ggbk_grid g, const TB *b, TM *r1, const ggbk_scalars *s)
+{
+	__shared__ TM x1s[TH * TH];
+	int act = s->active;
+	FaceTile t = face_tile(g.N);
+	TM xi = x1_at<TM, TB>(g, b, t.c);
+	x1s[(t.ty + 1) * TH + t.tx + 1] = xi;
+	auto x1_cell = [&](int j) { return x1_at<TM, TB>(g, b, j); };
+	stage_ring<TM>(t, g.N, x1s, x1_cell);
+	__syncthreads();
+	double sum = tile_stencil_sum<TM>(g, t, x1s, (double) xi, x1_cell);
+	if (act)
+		r1[t.c] = (TM) ((double) b[t.c] - sum);
+}
+
 // PCG update fused with the level-0 pre-smooth residual (fluid.c:336-345 then the first
 // pass of the V-cycle): psi += alpha p, r_out = r_in - alpha Ap, and r1 = r_out - K x1
 // with x1 from r_out, where the neighbours' r_out is recomputed from r_in and Ap (r_out
@@ -466,7 +484,10 @@ static void vcycle_level(const ggbk_level *lv, int nlev, int l, const TB *b, TM 
 	} else {
 		if (l == 0)
 			ggbk_probe_begin(GGBK_PROBE_MG_DOWN0);
-		GGBK_LAUNCH((k_mg_down<TM, TB>), NB(g.n), BS, 0, g, b, r, s);
+		if (TILED_UPDATE && l == 0 && g.N % TT == 0)
+			GGBK_LAUNCH((k_mg_down_tile<TM, TB>), NB(g.n), BS, 0, g, b, r, s);
+		else
+			GGBK_LAUNCH((k_mg_down<TM, TB>), NB(g.n), BS, 0, g, b, r, s);
 		if (l == 0)
 			ggbk_probe_end(GGBK_PROBE_MG_DOWN0);
 	}
