@@ -297,7 +297,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
-              "(the path would shard by cube face; see the design notes)", file=sys.stderr)
+              "(the path would shard by cube face, exchanging one ring of seam cells per stencil "
+              "pass and two scalars per CG iteration)", file=sys.stderr)
         sys.exit(2)
     cfg = CONFIGS[args.config]
     if ggb.device_init() != 0:
