@@ -56,10 +56,25 @@ __device__ __forceinline__ int arc_samples(double ang, double spacing)
 	return ns;
 }
 
+__device__ __forceinline__ dv3 arc_point_t(dv3 a, dv3 b, double tt)
+{
+	return vnorm(vadd(vscl(a, 1 - tt), vscl(b, tt)));
+}
+
 __device__ __forceinline__ dv3 arc_point(dv3 a, dv3 b, int k, int ns)
 {
-	double tt = (double) k / ns;
-	return vnorm(vadd(vscl(a, 1 - tt), vscl(b, tt)));
+	return arc_point_t(a, b, (double) k / ns);
+}
+
+// k / ns for 1 <= k <= ns <= 4096 from inv = 1.0 / ns: the rounded product, corrected
+// once by its exact remainder (Markstein's step).  This gives the correctly rounded
+// quotient, the division's own result, for every such pair (all 8,390,656 are checked on
+// the host by the tests).
+__device__ __forceinline__ double ratio_k_ns(int k, int ns, double inv)
+{
+	double q = (double) k * inv;
+	double r = fma(-q, (double) ns, (double) k);
+	return fma(r, inv, q);
 }
 
 // Pixel position of direction d on face f (render.c:91-98); false when the face is
@@ -564,10 +579,11 @@ __device__ __forceinline__ void sort_net(unsigned *k)
 //      exclusive prefix of ns in rank order;
 //   then per range of consecutive ranks whose samples fit SCAP (ranges are in index order):
 //   C  one thread per (particle, sample), its rank read from a per-range sample map: the
-//      sample's pixel on this tile's face; a sample on the tile fills the slot of its
-//      index in the range (position + key rank<<12 | k-1) and appends the slot to every
-//      texel it touches;
-//   D  one thread per texel: sort its fragments by key and blend them in that order.
+//      sample's pixel on this tile's face; a sample on the tile stores its position in
+//      the slot of its index in the range and appends (slot << 7 | rank - r0) to every
+//      texel it touches.  Slots are numbered in (rank, sample) order, so sorting these
+//      16-bit fragments sorts them by (particle index, sample);
+//   D  one thread per texel: sort its fragments and blend them in that order.
 // A range in which a texel collects more than CAPT fragments is retried with half the
 // particles; a single particle that still overflows (or alone has more than SCAP
 // samples) is blended in sample order by every thread for its own texels.
@@ -575,16 +591,17 @@ __device__ __forceinline__ void sort_net(unsigned *k)
 // registers), each thread keeps its two texels in registers, and the LDS arrays (20.4 KB,
 // allocated in 1280-byte granules) leave room for 8 workgroups per CU, which is also the
 // register limit.
-#define SCAP 288         // on-tile samples per range (slot fits 10 bits)
+#define SCAP 288         // on-tile samples per range (a slot fits 9 bits, a rank in the range 7)
+static_assert(SCAP <= 512 && WIN <= 128, "a fragment packs slot << 7 | rank - r0 into 16 bits");
 __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts ps, SplatGeo sg, float invg)
 {
 	static_assert(NTEX % DBS == 0, "each thread owns NTEX / DBS texels");
 	constexpr int TPT = NTEX / DBS;
 	__shared__ int tcnt[NTEX];
-	__shared__ unsigned short frag[CAPT][NTEX];
+	__shared__ unsigned short frag[CAPT][NTEX];  // slot << 7 | rank - r0
 	__shared__ float2 sxy[SCAP];
-	__shared__ unsigned skey[SCAP];
 	__shared__ double pa[3][WIN], pb[3][WIN];  // by rank
+	__shared__ double pinv[WIN];               // 1 / sample count, by rank
 	__shared__ int pst[WIN + 1];
 	__shared__ __align__(16) int ppid[WIN];    // window order; padded to 4 with INT_MAX
 	__shared__ unsigned short pns[WIN];        // sample count by rank (<= 4096)
@@ -686,6 +703,7 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 			pb[0][rank] = b.x; pb[1][rank] = b.y; pb[2][rank] = b.z;
 			pcol[rank] = mycol;
 			pns[rank] = (unsigned short) myns;
+			pinv[rank] = 1.0 / myns;
 		}
 		__syncthreads();
 		// exclusive prefix of ns in rank order (two waves)
@@ -738,19 +756,20 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 					int p = r0 + smap[j - j0], k = j - pst[p] + 1;
 					dv3 pa3 = d3(pa[0][p], pa[1][p], pa[2][p]), pb3 = d3(pb[0][p], pb[1][p], pb[2][p]);
 					float px, py;
-					if (!face_pixel(tr.f, arc_point(pa3, pb3, k, pns[p]), sg.R, px, py))
+					dv3 d = arc_point_t(pa3, pb3, ratio_k_ns(k, pns[p], pinv[p]));
+					if (!face_pixel(tr.f, d, sg.R, px, py))
 						continue;
 					int xlo, xhi, ylo, yhi;
 					if (!tile_clip(tr, px, py, rad, sg.R, xlo, xhi, ylo, yhi))
 						continue;
 					int sl = j - j0; // the range's samples fit SCAP slots
 					sxy[sl] = make_float2(px, py);
-					skey[sl] = ((unsigned) p << 12) | (unsigned) (k - 1);
+					const unsigned short fv = (unsigned short) ((sl << 7) | (p - r0));
 					bool over = false;
 					splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
 						int c = atomicAdd(&tcnt[tx], 1);
 						if (c < CAPT)
-							frag[c][tx] = (unsigned short) sl;
+							frag[c][tx] = fv;
 						else
 							over = true;
 					});
@@ -761,7 +780,9 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 				long long cb = clock64();
 				csamp += cb - cc;
 				if (!overflow) {
-					// D: each texel sorts its fragments (key << 10 | slot) and blends
+					// D: each texel sorts its fragments and blends them in that order.  The
+					// fragment reads are unconditional (entries past n hold stale, unused
+					// values, replaced by 0xffff, above any fragment) so they issue together.
 #pragma unroll
 					for (int h = 0; h < TPT; h++) {
 						int tx = lid + h * DBS;
@@ -771,13 +792,16 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 						touched |= 1u << h;
 						unsigned keys[CAPT];
 #pragma unroll
-						for (int q = 0; q < CAPT; q++) {
-							unsigned v = ~0u;
-							if (q < n) {
-								unsigned sl = frag[q][tx];
-								v = (skey[sl] << 10) | sl;
+						for (int q = 0; q < 4; q++) {
+							unsigned v = frag[q][tx];
+							keys[q] = q < n ? v : 0xffffu;
+						}
+						if (__any(n > 4)) {
+#pragma unroll
+							for (int q = 4; q < CAPT; q++) {
+								unsigned v = frag[q][tx];
+								keys[q] = q < n ? v : 0xffffu;
 							}
-							keys[q] = v;
 						}
 						if (n <= 4)
 							sort_net<4>(keys);
@@ -786,12 +810,14 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 						else
 							sort_net<CAPT>(keys);
 						float4 dd = tex[h];
-						for (int q = 0; q < n; q++) {
-							unsigned sl = keys[q] & 1023u;
-							unsigned rk = keys[q] >> 22;
-							float2 s2 = sxy[sl];
-							float4 c = pcol[rk];
-							blend_texel(dd, tx, tr, s2.x, s2.y, invrad, r.alpha * c.w, c, invg);
+#pragma unroll
+						for (int q = 0; q < CAPT; q++) {
+							if (q < n) {
+								float2 s2 = sxy[keys[q] >> 7];
+								float4 c = pcol[r0 + (keys[q] & 127u)];
+								blend_texel(dd, tx, tr, s2.x, s2.y, invrad, r.alpha * c.w, c,
+									    invg);
+							}
 						}
 						tex[h] = dd;
 					}
