@@ -525,6 +525,31 @@ __device__ __forceinline__ void splat_texels(const TileRect &tr, float px, float
 	}
 }
 
+// splat_texels for radius <= 1, from the splat's clipped box [xlo, xhi] x [ylo, yhi]:
+// the texels with positive weight (|x - px| * invrad < 1) are at most two consecutive
+// columns and two consecutive rows, starting within two of the box's first column and
+// row.  The (up to) four are returned at once, on[q] / tx[q], so that their LDS atomics
+// can be issued together; the order of a splat's own texels does not matter.
+__device__ __forceinline__ void splat_texels_2x2(const TileRect &tr, float px, float py, float invrad, int xlo,
+						 int xhi, int ylo, int yhi, bool on[4], int tx[4])
+{
+	auto pos = [&](int v, float c, int hi) { return v <= hi && 1.0f - fabsf((float) v - c) * invrad > 0; };
+	bool x0 = pos(xlo, px, xhi), x1 = pos(xlo + 1, px, xhi), x2 = pos(xlo + 2, px, xhi);
+	bool y0 = pos(ylo, py, yhi), y1 = pos(ylo + 1, py, yhi), y2 = pos(ylo + 2, py, yhi);
+	int xa = x0 ? xlo : (x1 ? xlo + 1 : xlo + 2), ya = y0 ? ylo : (y1 ? ylo + 1 : ylo + 2);
+	bool ca = x0 || x1 || x2, cb = x0 ? x1 : (x1 && x2);
+	bool ra = y0 || y1 || y2, rb = y0 ? y1 : (y1 && y2);
+	int base = (ya - tr.y0) * TILE + (xa - tr.x0);
+	on[0] = ra && ca;
+	tx[0] = base;
+	on[1] = ra && cb;
+	tx[1] = base + 1;
+	on[2] = rb && ca;
+	tx[2] = base + TILE;
+	on[3] = rb && cb;
+	tx[3] = base + TILE + 1;
+}
+
 // Whether local texel t is one of splat_texels' texels for the splat at (px, py).
 __device__ __forceinline__ bool texel_in_splat(const TileRect &tr, int t, float px, float py, float rad, float invrad,
 					       int R)
@@ -724,8 +749,11 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 		for (int r0 = 0; r0 < W;) {
 			long long cc = clock64();
 			windows++;
-			// the longest range from r0 whose samples fit SCAP (at least one particle)
+			// the longest range from r0 whose samples fit SCAP (at least one particle);
+			// usually the rest of the window
 			int lo = r0 + 1, hi = W;
+			if (pst[W] - pst[r0] <= SCAP)
+				lo = W;
 			while (lo < hi) {
 				int mid = (lo + hi + 1) >> 1;
 				if (pst[mid] - pst[r0] <= SCAP)
@@ -766,13 +794,31 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 					sxy[sl] = make_float2(px, py);
 					const unsigned short fv = (unsigned short) ((sl << 7) | (p - r0));
 					bool over = false;
-					splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
-						int c = atomicAdd(&tcnt[tx], 1);
-						if (c < CAPT)
-							frag[c][tx] = fv;
-						else
-							over = true;
-					});
+					if (rad <= 1.0f) {
+						bool on[4];
+						int tx[4], c[4];
+						splat_texels_2x2(tr, px, py, invrad, xlo, xhi, ylo, yhi, on, tx);
+#pragma unroll
+						for (int q = 0; q < 4; q++)
+							c[q] = on[q] ? atomicAdd(&tcnt[tx[q]], 1) : 0;
+#pragma unroll
+						for (int q = 0; q < 4; q++) {
+							if (on[q]) {
+								if (c[q] < CAPT)
+									frag[c[q]][tx[q]] = fv;
+								else
+									over = true;
+							}
+						}
+					} else {
+						splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
+							int c = atomicAdd(&tcnt[tx], 1);
+							if (c < CAPT)
+								frag[c][tx] = fv;
+							else
+								over = true;
+						});
+					}
 					if (over)
 						overflow = 1;
 				}
