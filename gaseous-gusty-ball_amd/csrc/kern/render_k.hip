@@ -637,13 +637,6 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 	int t = xblock();
 	const int lid = threadIdx.x;
 	int *diag = r.tile_diag + 8 * (size_t) t;
-	int L = r.bucket_mode ? r.tile_cursor[t] : r.tile_start[t + 1] - r.tile_start[t];
-	if (L == 0 || (!r.bucket_mode && r.total[0] > r.cap)) {
-		// nothing to draw (or the list did not fit: the host grows it before launching)
-		if (lid < 8)
-			diag[lid] = 0;
-		return;
-	}
 	TileRect tr;
 	tr.f = t / (r.ntx * r.ntx);
 	int rem = t % (r.ntx * r.ntx);
@@ -651,10 +644,13 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 	tr.x0 = (rem % r.ntx) * TILE;
 	tr.w = min(TILE, r.R - tr.x0);
 	tr.h = min(TILE, r.R - tr.y0);
-	const int *lst = r.bucket_mode ? r.bucket + (size_t) t * BUCKET : r.list + r.tile_start[t];
 	const float rad = sg.rad, invrad = 1.0f / rad;
 
-	// thread lid owns texels lid + h DBS and keeps them in registers
+	// The tile's count, its texels (thread lid owns texels lid + h DBS and keeps them in
+	// registers) and, with buckets, its entries (at most BUCKET = DBS, one per thread) are
+	// loaded together: one round trip before the particles' data.  Entries past the count
+	// are stale and never used.
+	int L = r.bucket_mode ? r.tile_cursor[t] : r.tile_start[t + 1] - r.tile_start[t];
 	float4 *gbuf = (float4 *) r.buf;
 	float4 tex[TPT];
 	unsigned touched = 0;
@@ -666,6 +662,15 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 		if (ly < tr.h && lx < tr.w)
 			tex[h] = gbuf[((size_t) tr.f * r.R + tr.y0 + ly) * r.R + tr.x0 + lx];
 	}
+	static_assert(BUCKET == DBS, "one bucket entry per thread");
+	int bslot = r.bucket_mode ? r.bucket[(size_t) t * BUCKET + lid] : 0;
+	if (L == 0 || (!r.bucket_mode && r.total[0] > r.cap)) {
+		// nothing to draw (or the list did not fit: the host grows it before launching)
+		if (lid < 8)
+			diag[lid] = 0;
+		return;
+	}
+	const int *lst = r.bucket_mode ? nullptr : r.list + r.tile_start[t];
 	if (lid == 0)
 		ntouched = 0;
 
@@ -701,7 +706,7 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 		int myns = 0, mypid = 0;
 		dv3 a = d3(0, 0, 0), b = d3(0, 0, 0);
 		if (lid < W) {
-			int slot = is_sorted ? (int) (unsigned) sorted[base + lid] : lst[lid];
+			int slot = is_sorted ? (int) (unsigned) sorted[base + lid] : (lst ? lst[lid] : bslot);
 			a = ld3(ps.prev + 3 * (size_t) slot);
 			b = ld3(ps.pos + 3 * (size_t) slot);
 			myns = r.nsamp[slot];
