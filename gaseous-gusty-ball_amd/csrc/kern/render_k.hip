@@ -333,14 +333,18 @@ __global__ __launch_bounds__(PBS) void k_render_fill(ggbk_parts ps, ggbk_parts o
 // blocks of SCAN_BS tiles: local scans plus per-block totals, then each block adds the
 // totals of the blocks before it.  stats[0] = total list entries, stats[1] = longest
 // tile list; stats[2..15] are cleared.
+// Inclusive scan over the 64 lanes of a wave with DPP moves (no LDS): Hillis-Steele
+// within each row of 16 lanes (row_shr 1, 2, 4, 8; lanes shifted in from outside the row
+// add 0), then row 1 and row 3 add the last lane of the row before (row_bcast:15), and
+// rows 2 and 3 add lane 31 (row_bcast:31).
 __device__ __forceinline__ int wave_incl_scan(int v)
 {
-	int lane = __lane_id();
-	for (int off = 1; off < 64; off <<= 1) {
-		int u = __shfl_up(v, off);
-		if (lane >= off)
-			v += u;
-	}
+	v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+	v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+	v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+	v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+	v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+	v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
 	return v;
 }
 
@@ -744,9 +748,8 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 		__syncthreads();
 		if (lid < W)
 			pst[lid] = incl - nsr + (lid >= 64 ? wtot : 0);
-		__syncthreads();
 		if (lid == W - 1)
-			pst[W] = pst[W - 1] + pns[W - 1];
+			pst[W] = incl + (lid >= 64 ? wtot : 0);
 		__syncthreads();
 		csamp += clock64() - ca;
 		cpre += clock64() - ca;
