@@ -84,10 +84,18 @@ int render_init(Render *r, int R, float radius, float alpha)
 	k->nsamp = NULL;
 	k->nsamp_cap = 0;
 	k->sort_scratch = NULL;
+	k->hstats = NULL;
+	k->stats_ev = NULL;
+	k->fill_ev = NULL;
+	k->copy_stream = NULL;
 	if (!k->buf || !k->tile_count || !k->tile_start || !k->tile_cursor || !k->home_count || !k->home_start ||
 	    !k->home_cursor || !k->total || !k->tile_diag ||
 	    !k->scan_part || !k->bucket) {
 		fprintf(stderr, "render: out of device memory\n");
+		return -1;
+	}
+	if (ggbk_render_host_init(k)) {
+		fprintf(stderr, "render: out of pinned host memory\n");
 		return -1;
 	}
 	if (ggbk_zero(k->buf, sizeof(float) * count) || ggbk_zero(k->tile_diag, sizeof(int) * 8 * (size_t) k->ntiles) ||
@@ -118,6 +126,7 @@ void render_free(Render *r)
 		ggbk_free(D->k.list);
 		ggbk_free(D->k.nsamp);
 		ggbk_free(D->k.sort_scratch);
+		ggbk_render_host_free(&D->k);
 		pthread_mutex_destroy(&D->mu);
 		free(D);
 	}

@@ -264,7 +264,13 @@ typedef struct {
 	long long *total;           /* [16] entries, longest list, draw diagnostics (device) */
 	int *tile_diag;             /* [8 * ntiles] per-tile draw diagnostics */
 	long long *scan_part;       /* [3 * ggbk_render_scan_blocks(ntiles)] */
+	long long *hstats;          /* [2] pinned host copy of total[0..1] */
+	void *stats_ev;             /* event recorded after that copy */
+	void *fill_ev;              /* event recorded after the binning */
+	void *copy_stream;          /* stream of that copy (default priority) */
 } ggbk_render;
+int ggbk_render_host_init(ggbk_render *r); /* hstats, the events, the copy stream */
+void ggbk_render_host_free(ggbk_render *r);
 int ggbk_render_tile(void);     /* tile edge the kernels are built for */
 int ggbk_render_scan_blocks(int ntiles);
 void ggbk_render_scale(float *buf, size_t count, float g);

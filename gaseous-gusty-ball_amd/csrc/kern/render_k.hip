@@ -650,11 +650,15 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 	tr.h = min(TILE, r.R - tr.y0);
 	const float rad = sg.rad, invrad = 1.0f / rad;
 
-	// The tile's count, its texels (thread lid owns texels lid + h DBS and keeps them in
-	// registers) and, with buckets, its entries (at most BUCKET = DBS, one per thread) are
-	// loaded together: one round trip before the particles' data.  Entries past the count
-	// are stale and never used.
-	int L = r.bucket_mode ? r.tile_cursor[t] : r.tile_start[t + 1] - r.tile_start[t];
+	// The tile's count, the entry total and longest list, the tile's texels (thread lid
+	// owns texels lid + h DBS and keeps them in registers) and its bucket entries (at most
+	// BUCKET = DBS, one per thread) are loaded together, unconditionally and in bounds (the
+	// buckets exist in both modes): one round trip before the particles' data.  Entries
+	// past the count are stale and never used.
+	const int *cnt_hi = r.bucket_mode ? r.tile_cursor + t : r.tile_start + t + 1;
+	const int *cnt_lo = r.bucket_mode ? r.tile_cursor + t : r.tile_start + t;
+	int c_hi = *cnt_hi, c_lo = *cnt_lo;
+	const longlong2 tot = *(const longlong2 *) r.total;
 	float4 *gbuf = (float4 *) r.buf;
 	float4 tex[TPT];
 	unsigned touched = 0;
@@ -667,9 +671,16 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 			tex[h] = gbuf[((size_t) tr.f * r.R + tr.y0 + ly) * r.R + tr.x0 + lx];
 	}
 	static_assert(BUCKET == DBS, "one bucket entry per thread");
-	int bslot = r.bucket_mode ? r.bucket[(size_t) t * BUCKET + lid] : 0;
-	if (L == 0 || (!r.bucket_mode && r.total[0] > r.cap)) {
-		// nothing to draw (or the list did not fit: the host grows it before launching)
+	int bslot = r.bucket[(size_t) t * BUCKET + lid];
+	// keeps the entry's load above the early return below (the compiler would sink it
+	// there, into a round trip of its own)
+	asm volatile("" ::"v"(bslot));
+	int L = r.bucket_mode ? c_hi : c_hi - c_lo;
+	// A bucket-mode launch is made before the host has seen the longest list: when a list
+	// overflowed its bucket it draws nothing, and the packed list is drawn after it.  A
+	// packed list that did not fit is not drawn either (the host grows it and draws again).
+	bool skip = r.bucket_mode ? tot.y > BUCKET : tot.x > r.cap;
+	if (L == 0 || skip) {
 		if (lid < 8)
 			diag[lid] = 0;
 		return;
@@ -1044,6 +1055,38 @@ extern "C" int ggbk_render_tile(void) { return TILE; }
 
 extern "C" int ggbk_render_bucket(void) { return BUCKET; }
 
+extern "C" int ggbk_render_host_init(ggbk_render *r)
+{
+	hipEvent_t e1, e2;
+	hipStream_t st;
+	r->hstats = (long long *) ggbk_host_alloc(2 * sizeof(long long));
+	if (!r->hstats)
+		return -1;
+	if (hipEventCreateWithFlags(&e1, hipEventDisableTiming) != hipSuccess)
+		return -1;
+	r->stats_ev = (void *) e1;
+	if (hipEventCreateWithFlags(&e2, hipEventDisableTiming) != hipSuccess)
+		return -1;
+	r->fill_ev = (void *) e2;
+	if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+		return -1;
+	r->copy_stream = (void *) st;
+	return 0;
+}
+
+extern "C" void ggbk_render_host_free(ggbk_render *r)
+{
+	if (r->copy_stream)
+		(void) hipStreamDestroy((hipStream_t) r->copy_stream);
+	if (r->stats_ev)
+		(void) hipEventDestroy((hipEvent_t) r->stats_ev);
+	if (r->fill_ev)
+		(void) hipEventDestroy((hipEvent_t) r->fill_ev);
+	ggbk_host_free(r->hstats);
+	r->copy_stream = r->stats_ev = r->fill_ev = nullptr;
+	r->hstats = nullptr;
+}
+
 extern "C" int ggbk_render_scan_blocks(int ntiles) { return (ntiles + SCAN_BS - 1) / SCAN_BS; }
 
 extern "C" int ggbk_render_stats(const ggbk_render *r, long long *out16)
@@ -1056,10 +1099,12 @@ extern "C" int ggbk_render_stats(const ggbk_render *r, long long *out16)
 // With reorder, the particles are counted per home tile first and the fill also copies
 // them to `out` in that order (the caller adopts it once this returns); without, they
 // stay where they are (the storage order does not change the result, only locality).
-// The fill writes the tile lists into fixed-size buckets and counts every tile exactly;
-// the one synchronisation reads the entry total and the longest list.  If a tile
-// overflowed its bucket, the counts are scanned into starts and the fill runs again into
-// one packed list (grown when the total exceeds its capacity).
+// The fill writes the tile lists into fixed-size buckets and counts every tile exactly.
+// The entry total and the longest list are copied to pinned host memory (on a second
+// stream) and the draw is launched at once, in bucket mode; the host then waits for the
+// copy only, while the draw runs.  If a tile overflowed its bucket, that draw returns at its start on the
+// device, and the counts are scanned into starts, the fill runs again into one packed
+// list (grown when the total exceeds its capacity) and the draw runs on that list.
 extern "C" long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_parts out, float invg, int reorder)
 {
 	if (in.n <= 0)
@@ -1104,13 +1149,24 @@ extern "C" long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_par
 		    (const int *) r->home_start, hcursor, r->list, r->cap, r->bucket, r->nsamp);
 	GGBK_LAUNCH(k_render_bucket_stats, sb, SCAN_BS, 0, r->ntiles, (const int *) r->tile_cursor, r->total);
 	ggbk_probe_end(GGBK_PROBE_RENDER_FILL);
-	long long stats[2] = { 0, 0 };
-	if (ggbk_d2h(stats, r->total, sizeof(stats)))
+	// the copy runs on its own stream, after the binning, beside the draw
+	hipStream_t cs = (hipStream_t) r->copy_stream;
+	if (hipEventRecord((hipEvent_t) r->fill_ev, ggbk_stream()) != hipSuccess ||
+	    hipStreamWaitEvent(cs, (hipEvent_t) r->fill_ev, 0) != hipSuccess ||
+	    hipMemcpyAsync(r->hstats, r->total, 2 * sizeof(long long), hipMemcpyDeviceToHost, cs) != hipSuccess ||
+	    hipEventRecord((hipEvent_t) r->stats_ev, cs) != hipSuccess)
 		return -1;
+	r->bucket_mode = 1;
+	ggbk_probe_begin(GGBK_PROBE_RENDER_DRAW);
+	GGBK_LAUNCH(k_render_draw, (unsigned) r->ntiles, DBS, 0, *r, in, sg, invg);
+	ggbk_probe_end(GGBK_PROBE_RENDER_DRAW);
+	if (hipEventSynchronize((hipEvent_t) r->stats_ev) != hipSuccess)
+		return -1;
+	long long stats[2] = { r->hstats[0], r->hstats[1] };
 	if (stats[0] > INT_MAX / 2)
 		return -1;
-	r->bucket_mode = stats[1] <= BUCKET;
-	if (!r->bucket_mode) {
+	if (stats[1] > BUCKET) {
+		r->bucket_mode = 0;
 		// a tile holds more than a bucket: pack the exact counts into one list
 		if (stats[0] > r->cap) {
 			long long ncap = stats[0] + stats[0] / 4 + 1024;
@@ -1133,10 +1189,10 @@ extern "C" long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_par
 			    (const long long *) r->scan_part, r->total);
 		GGBK_LAUNCH(k_render_fill, pb, PBS, 0, in, out, sg, (const int *) r->tile_start, r->tile_cursor,
 			    (const int *) r->home_start, hcursor, r->list, r->cap, (int *) nullptr, r->nsamp);
+		ggbk_probe_begin(GGBK_PROBE_RENDER_DRAW);
+		GGBK_LAUNCH(k_render_draw, (unsigned) r->ntiles, DBS, 0, *r, in, sg, invg);
+		ggbk_probe_end(GGBK_PROBE_RENDER_DRAW);
 	}
-	ggbk_probe_begin(GGBK_PROBE_RENDER_DRAW);
-	GGBK_LAUNCH(k_render_draw, (unsigned) r->ntiles, DBS, 0, *r, in, sg, invg);
-	ggbk_probe_end(GGBK_PROBE_RENDER_DRAW);
 	if (ggbk_check("render_deposit"))
 		return -1;
 	return stats[0];
