@@ -636,7 +636,7 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 	__shared__ unsigned short pns[WIN];        // sample count by rank (<= 4096)
 	__shared__ unsigned char smap[SCAP];       // rank - r0 of each sample of a range
 	__shared__ float4 pcol[WIN];               // by rank
-	__shared__ int overflow, wtot, ntouched;
+	__shared__ int overflow, ntouched;
 	long long c0 = clock64();
 	int t = xblock();
 	const int lid = threadIdx.x;
@@ -731,6 +731,11 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 		} else if (lid < ((W + 3) & ~3)) {
 			ppid[lid] = INT_MAX;
 		}
+		// the first range's tile counts and overflow flag
+		for (int i = lid; i < NTEX; i += DBS)
+			tcnt[i] = 0;
+		if (lid == 0)
+			overflow = 0;
 		__syncthreads();
 		// B: rank by index; store by rank
 		int rank = lid;
@@ -751,20 +756,31 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 			pinv[rank] = 1.0 / myns;
 		}
 		__syncthreads();
-		// exclusive prefix of ns in rank order (two waves)
-		int nsr = lid < W ? pns[lid] : 0;
-		int incl = wave_incl_scan(nsr);
-		if (__lane_id() == 63 && lid < 64)
-			wtot = incl;
-		__syncthreads();
+		// exclusive prefix of ns in rank order: each wave scans both halves (ranks 0-63 and
+		// 64-127) itself, so no barrier passes a total between the waves
+		int lane = __lane_id();
+		int n0 = lane < W ? pns[lane] : 0;
+		int n1 = 64 + lane < W ? pns[64 + lane] : 0;
+		int i0 = wave_incl_scan(n0), i1 = wave_incl_scan(n1);
+		int s0 = __builtin_amdgcn_readlane(i0, 63);
+		int total = s0 + __builtin_amdgcn_readlane(i1, 63);
+		int mine = lid < 64 ? n0 : n1;
+		int excl = (lid < 64 ? i0 : s0 + i1) - mine;
 		if (lid < W)
-			pst[lid] = incl - nsr + (lid >= 64 ? wtot : 0);
-		if (lid == W - 1)
-			pst[W] = incl + (lid >= 64 ? wtot : 0);
+			pst[lid] = excl;
+		if (lid == 0)
+			pst[W] = total;
+		// the whole window in one range (the usual case): its sample map is written here,
+		// so the range starts without a set-up step of its own
+		const bool whole = total <= SCAP;
+		if (whole && lid < W)
+			for (int k = excl; k < excl + mine; k++)
+				smap[k] = (unsigned char) lid;
 		__syncthreads();
 		csamp += clock64() - ca;
 		cpre += clock64() - ca;
 
+		bool prepared = whole;
 		for (int r0 = 0; r0 < W;) {
 			long long cc = clock64();
 			windows++;
@@ -787,17 +803,20 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 				// a single particle with more samples than a range holds goes to the
 				// in-order fallback directly
 				bool big = j1 - j0 > SCAP;
-				for (int i = lid; i < NTEX; i += DBS)
-					tcnt[i] = 0;
-				if (lid == 0)
-					overflow = big ? 1 : 0;
-				// the rank of every sample of the range (r1 - r0 <= WIN = DBS)
-				if (!big && r0 + lid < r1) {
-					int p = r0 + lid;
-					for (int k = pst[p] - j0; k < pst[p + 1] - j0; k++)
-						smap[k] = (unsigned char) lid;
+				if (!(prepared && r0 == 0 && r1 == W)) {
+					for (int i = lid; i < NTEX; i += DBS)
+						tcnt[i] = 0;
+					if (lid == 0)
+						overflow = big ? 1 : 0;
+					// the rank of every sample of the range (r1 - r0 <= WIN = DBS)
+					if (!big && r0 + lid < r1) {
+						int p = r0 + lid;
+						for (int k = pst[p] - j0; k < pst[p + 1] - j0; k++)
+							smap[k] = (unsigned char) lid;
+					}
+					__syncthreads();
 				}
-				__syncthreads();
+				prepared = false;
 				// C: one thread per (particle, sample) of ranks [r0, r1)
 				for (int j = j0 + lid; !big && j < j1; j += DBS) {
 					int p = r0 + smap[j - j0], k = j - pst[p] + 1;
