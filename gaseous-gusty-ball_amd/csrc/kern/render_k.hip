@@ -262,10 +262,16 @@ __device__ __forceinline__ void for_arc_tiles(bool on, dv3 a, dv3 b, const Splat
 		int tx0 = xlo / TILE, ty0 = ylo / TILE;
 		int w = any ? xhi / TILE - tx0 + 1 : 0;
 		int nt = any ? w * (yhi / TILE - ty0 + 1) : 0;
+		// tile (tx0 + ix, ty0 + iy) at step s = iy w + ix, stepped without a division
+		int ix = 0, iy = 0;
 		for (int s = 0; __any(s < nt); s++) {
 			bool has = s < nt;
-			int t = has ? (f * sg.ntx + ty0 + s / w) * sg.ntx + tx0 + s % w : 0;
+			int t = has ? (f * sg.ntx + ty0 + iy) * sg.ntx + tx0 + ix : 0;
 			fn(has, t);
+			if (++ix == w) {
+				ix = 0;
+				iy++;
+			}
 		}
 	}
 }
