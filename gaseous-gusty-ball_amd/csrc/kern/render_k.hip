@@ -335,10 +335,6 @@ __global__ __launch_bounds__(PBS) void k_render_fill(ggbk_parts ps, ggbk_parts o
 	});
 }
 
-// Exclusive scans of the tile list counts and the home counts, in two passes over
-// blocks of SCAN_BS tiles: local scans plus per-block totals, then each block adds the
-// totals of the blocks before it.  stats[0] = total list entries, stats[1] = longest
-// tile list; stats[2..15] are cleared.
 // Inclusive scan over the 64 lanes of a wave with DPP moves (no LDS): Hillis-Steele
 // within each row of 16 lanes (row_shr 1, 2, 4, 8; lanes shifted in from outside the row
 // add 0), then row 1 and row 3 add the last lane of the row before (row_bcast:15), and
@@ -354,6 +350,10 @@ __device__ __forceinline__ int wave_incl_scan(int v)
 	return v;
 }
 
+// Exclusive scans of the tile list counts and the home counts, in two passes over
+// blocks of SCAN_BS tiles: local scans plus per-block totals, then each block adds the
+// totals of the blocks before it.  stats[0] = total list entries, stats[1] = longest
+// tile list; stats[2..15] are cleared.
 __global__ __launch_bounds__(SCAN_BS) void k_render_scan_local(int ntiles, const int *counts, int *starts,
 							       int *cursor, const int *hcounts, int *hstarts,
 							       int *hcursor, long long *part)
