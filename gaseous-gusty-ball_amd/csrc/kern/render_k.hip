@@ -27,6 +27,7 @@
 // contraction, so texels match the CPU path bit for bit for identical positions.
 #include <hip/hip_runtime.h>
 #include <limits.h>
+#include <type_traits>
 
 #include "dev.h"
 #include "ggbk.h"
@@ -85,6 +86,33 @@ __device__ __forceinline__ bool face_pixel(int f, dv3 d, int R, float &px, float
 		return false;
 	double dc, u, v;
 	gnomonic(f, d, dc, u, v);
+	px = (float) ((u + 1.0) * 0.5 * R - 0.5);
+	py = (float) ((1.0 - v) * 0.5 * R - 0.5);
+	return true;
+}
+
+// face_pixel for a face fixed at compile time (the draw's tile face, chosen once per
+// range): the same arithmetic, with the component selection resolved by the compiler.
+template <int F>
+__device__ __forceinline__ bool face_pixel_c(dv3 d, int R, float &px, float &py)
+{
+	double dc, cr, cu;
+	if constexpr (F == 0) {
+		dc = d.z; cr = d.x; cu = d.y;
+	} else if constexpr (F == 1) {
+		dc = d.x; cr = -d.z; cu = d.y;
+	} else if constexpr (F == 2) {
+		dc = -d.z; cr = -d.x; cu = d.y;
+	} else if constexpr (F == 3) {
+		dc = -d.x; cr = d.z; cu = d.y;
+	} else if constexpr (F == 4) {
+		dc = d.y; cr = d.x; cu = -d.z;
+	} else {
+		dc = -d.y; cr = d.x; cu = d.z;
+	}
+	if (!(dc >= 0.5))
+		return false;
+	double u = cr / dc, v = cu / dc;
 	px = (float) ((u + 1.0) * 0.5 * R - 0.5);
 	py = (float) ((1.0 - v) * 0.5 * R - 0.5);
 	return true;
@@ -823,48 +851,62 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 					__syncthreads();
 				}
 				prepared = false;
-				// C: one thread per (particle, sample) of ranks [r0, r1)
-				for (int j = j0 + lid; !big && j < j1; j += DBS) {
-					int p = r0 + smap[j - j0], k = j - pst[p] + 1;
-					dv3 pa3 = d3(pa[0][p], pa[1][p], pa[2][p]), pb3 = d3(pb[0][p], pb[1][p], pb[2][p]);
-					float px, py;
-					dv3 d = arc_point_t(pa3, pb3, ratio_k_ns(k, pns[p], pinv[p]));
-					if (!face_pixel(tr.f, d, sg.R, px, py))
-						continue;
-					int xlo, xhi, ylo, yhi;
-					if (!tile_clip(tr, px, py, rad, sg.R, xlo, xhi, ylo, yhi))
-						continue;
-					int sl = j - j0; // the range's samples fit SCAP slots
-					sxy[sl] = make_float2(px, py);
-					const unsigned short fv = (unsigned short) ((sl << 7) | (p - r0));
-					bool over = false;
-					if (rad <= 1.0f) {
-						bool on[4];
-						int tx[4], c[4];
-						splat_texels_2x2(tr, px, py, invrad, xlo, xhi, ylo, yhi, on, tx);
-#pragma unroll
-						for (int q = 0; q < 4; q++)
-							c[q] = on[q] ? atomicAdd(&tcnt[tx[q]], 1) : 0;
-#pragma unroll
-						for (int q = 0; q < 4; q++) {
-							if (on[q]) {
-								if (c[q] < CAPT)
-									frag[c[q]][tx[q]] = fv;
+				// C: one thread per (particle, sample) of ranks [r0, r1), specialised for
+				// the tile's face
+				auto sample_pass = [&](auto face) {
+					constexpr int F = decltype(face)::value;
+					for (int j = j0 + lid; j < j1; j += DBS) {
+						int p = r0 + smap[j - j0], k = j - pst[p] + 1;
+						dv3 pa3 = d3(pa[0][p], pa[1][p], pa[2][p]), pb3 = d3(pb[0][p], pb[1][p], pb[2][p]);
+						float px, py;
+						dv3 d = arc_point_t(pa3, pb3, ratio_k_ns(k, pns[p], pinv[p]));
+						if (!face_pixel_c<F>(d, sg.R, px, py))
+							continue;
+						int xlo, xhi, ylo, yhi;
+						if (!tile_clip(tr, px, py, rad, sg.R, xlo, xhi, ylo, yhi))
+							continue;
+						int sl = j - j0; // the range's samples fit SCAP slots
+						sxy[sl] = make_float2(px, py);
+						const unsigned short fv = (unsigned short) ((sl << 7) | (p - r0));
+						bool over = false;
+						if (rad <= 1.0f) {
+							bool on[4];
+							int tx[4], c[4];
+							splat_texels_2x2(tr, px, py, invrad, xlo, xhi, ylo, yhi, on, tx);
+	#pragma unroll
+							for (int q = 0; q < 4; q++)
+								c[q] = on[q] ? atomicAdd(&tcnt[tx[q]], 1) : 0;
+	#pragma unroll
+							for (int q = 0; q < 4; q++) {
+								if (on[q]) {
+									if (c[q] < CAPT)
+										frag[c[q]][tx[q]] = fv;
+									else
+										over = true;
+								}
+							}
+						} else {
+							splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
+								int c = atomicAdd(&tcnt[tx], 1);
+								if (c < CAPT)
+									frag[c][tx] = fv;
 								else
 									over = true;
-							}
+							});
 						}
-					} else {
-						splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
-							int c = atomicAdd(&tcnt[tx], 1);
-							if (c < CAPT)
-								frag[c][tx] = fv;
-							else
-								over = true;
-						});
+						if (over)
+							overflow = 1;
 					}
-					if (over)
-						overflow = 1;
+				};
+				if (!big) {
+					switch (tr.f) {
+					case 0: sample_pass(std::integral_constant<int, 0>()); break;
+					case 1: sample_pass(std::integral_constant<int, 1>()); break;
+					case 2: sample_pass(std::integral_constant<int, 2>()); break;
+					case 3: sample_pass(std::integral_constant<int, 3>()); break;
+					case 4: sample_pass(std::integral_constant<int, 4>()); break;
+					default: sample_pass(std::integral_constant<int, 5>()); break;
+					}
 				}
 				__syncthreads();
 				long long cb = clock64();
