@@ -251,6 +251,40 @@ def test_render_crowded_and_antipodal(ggb, orc, gpu):
     assert np.max(np.abs(a - b)) <= 1
 
 
+@pytest.mark.parametrize("R,radius", [(100, 1.7), (72, 0.6)])
+def test_render_radius_and_partial_tiles(ggb, orc, gpu, R, radius):
+    """Splat radii other than 1 (above 1 the generic texel walk, below it the 2x2 path) on
+    faces whose edge is not a multiple of the 16-texel tile (partial tiles at the right and
+    bottom edges), against the oracle's serial blend."""
+    n = 4000
+    rng = np.random.default_rng(5)
+    pos = rng.standard_normal((n, 3))
+    pos /= np.linalg.norm(pos, axis=1, keepdims=True)
+    prev = pos + 0.03 * rng.standard_normal((n, 3))
+    prev /= np.linalg.norm(prev, axis=1, keepdims=True)
+    P = ggb.Particles(n)
+    P.set(pos, prev)
+    OP = orc.OracleParticles("f64", n)
+    OP.pos[:] = pos.ravel()
+    OP.prev[:] = prev.ravel()
+    img = ggb.strip_to_float(ggb.banded_strip(7))
+    P.color_from_image(img, 64, 1024)
+    OP.color_from_image(img, 64, 1024)
+    Rr = ggb.Render(R, radius)
+    OR = orc.OracleRender("f64", R, radius)
+    for _ in range(2):
+        Rr.fade(0.02)
+        OR.fade(0.02)
+        Rr.deposit(P)
+        OR.deposit(OP)
+    buf = Rr.sync_host()
+    assert np.count_nonzero(OR.buf[..., 3]) > 0.05 * 6 * R * R
+    assert np.max(np.abs(buf - OR.buf)) < 1e-5
+    a, b = Rr.faces8().astype(int), OR.faces8().astype(int)
+    assert np.max(np.abs(a - b)) <= 1
+    assert np.count_nonzero(a != b) <= 0.001 * a[..., 0].size
+
+
 def _faces8_host(buf, g, bg=None):
     """render_texel then render_pixel8 (reference render.c:220-226, 253-279) in float32."""
     c = buf * np.float32(g)
