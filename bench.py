@@ -52,13 +52,13 @@ PROBES = {
 }
 
 
-def kernel_bytes(name, n, P, E, T, Ts, Tm, Tz):
+def kernel_bytes(name, n, P, E, T, Ts, Tm, Tz, reorder_frac=1.0):
     if name == "k_render_draw":      # texels read+written, list entry, FP64 endpoints, colour, index
         return 32 * T + E * (4 + 48 + 16 + 4)
     if name == "render_count":       # current position (FP64) for the home tile
         return P * 24
-    if name == "k_render_fill":      # particle read + reordered copy (68 B each), sample count, list
-        return P * (68 + 68 + 2) + E * 4
+    if name == "k_render_fill":      # endpoints read, sample count, list; on reordering deposits
+        return P * (48 + 2) + reorder_frac * P * (20 + 68) + E * 4  # also colour+index, copy out
     if name == "mg_up(level 0)":     # b, dinv, stencil record, tap weights, coarse x, z out
         return n * (Ts + 4 + 8 + 16 + Tm / 4 + Tm)
     if name == "k_mg_down(level 0)":  # b, dinv, stencil record, r1 out
@@ -373,9 +373,11 @@ def main():
         if tr:
             traffic = tr.get("bytes_per_launch")
     kernels = {}
+    # share of the probed deposits that reordered the particles (those run the count)
+    rfrac = warm.get("render_count", (0, 0))[0] / max(warm.get("k_render_fill", (0, 0))[0], 1)
     for name, (cnt, ms) in warm.items():
         if cnt:
-            b = kernel_bytes(name, g.n, cfg["P"], E, T, Ts, Tm, Tm)
+            b = kernel_bytes(name, g.n, cfg["P"], E, T, Ts, Tm, Tm, rfrac)
             kernels[name] = {"launches_per_step": round(cnt / max(args.warmup - nplain, 1), 2),
                              "avg_us": round(ms * 1000.0 / cnt, 2),
                              "algo_bytes": int(b),

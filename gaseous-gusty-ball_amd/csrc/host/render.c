@@ -25,7 +25,7 @@ struct ggb_render_dev {
 	unsigned long ndeposits;
 };
 #ifndef GGB_REORDER_EVERY
-#define GGB_REORDER_EVERY 4
+#define GGB_REORDER_EVERY 16
 #endif
 
 int render_init(Render *r, int R, float radius, float alpha)
