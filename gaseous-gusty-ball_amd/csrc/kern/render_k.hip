@@ -3,26 +3,27 @@
 // The reference emits one record per (sample, face, 32-row strip), sorts them
 // bin-major / thread-minor and blends each bin in order, so every texel sees its
 // splats in (particle index, sample) order.  Here the unit of work is a face TILE of
-// TILE x TILE texels held in LDS, and the particle arrays are kept in tile order
-// (storage slot s holds particle pid[s]):
-//   1. count  -- each particle adds itself once to every tile that the bounding box of
-//                its arc's footprints (per face) overlaps, and once to its home tile
-//                (the tile of the texel nearest its current position).  For short arcs
-//                the box comes from the two endpoints (the arc is a straight segment on
-//                the face).  Atomics are aggregated per wave, one per distinct tile:
-//                neighbouring slots hold neighbouring particles;
-//   2. scan   -- exclusive scans of both counts;
-//   3. fill   -- the same boxes write the particle's slot into each tile's list, and
-//                each particle is copied to its home tile's range of the next storage
-//                order, which the caller adopts after the draw;
-//   4. draw   -- one workgroup per tile: load the tile into LDS, then per window of
-//                particles compute every (particle, sample) pixel on the tile with one
-//                thread per sample, append each on-tile sample to the texels it touches,
-//                and let each texel sort its few fragments by (particle index, sample)
-//                and blend them in that order -- the reference's per-texel order.  A
-//                tile with more than WIN particles is sorted by particle index and cut
-//                into windows; a window that overflows the LDS sample or fragment
-//                capacity is retried with a quarter of the particles.
+// TILE x TILE texels, and the particle arrays are kept in tile order (storage slot s
+// holds particle pid[s]), restored every few deposits:
+//   1. count  -- (reordering deposits) each particle adds itself to its home tile (the
+//                tile of the texel nearest its current position); atomics are
+//                aggregated per wave, one per distinct tile, since neighbouring slots
+//                hold neighbouring particles; then exclusive scans of the counts;
+//   2. fill   -- every tile that the bounding box of the particle's arc footprints (per
+//                face) overlaps gets the particle's slot in its fixed-size bucket, and
+//                every tile is counted exactly.  For short arcs the box comes from the
+//                two endpoints (the arc is a straight segment on the face).  When
+//                reordering, each particle is also copied to its home tile's range of
+//                the next storage order, which the caller adopts after the draw;
+//   3. draw   -- one workgroup per tile, its texels in registers: per window of
+//                particles, every (particle, sample) pixel on the tile with one thread
+//                per sample, each on-tile sample appended to the texels it touches; each
+//                texel sorts its few fragments by (particle index, sample) and blends
+//                them in that order -- the reference's per-texel order.  A tile with more
+//                than a bucket of particles takes the packed-list path (exact counts,
+//                scanned starts, the fill again), is sorted by particle index and cut
+//                into windows; a range that overflows a texel's fragment capacity is
+//                retried with half the particles.
 // The texel arithmetic is the reference's (render.c:113-154) in float with no FMA
 // contraction, so texels match the CPU path bit for bit for identical positions.
 #include <hip/hip_runtime.h>
@@ -616,7 +617,7 @@ __device__ __forceinline__ void blend_texel(float4 &dd, int t, const TileRect &t
 	dd.w = dd.w * keep + add;
 }
 
-// Compare-exchange and odd-even transposition networks on 32-bit keys held in registers
+// Compare-exchange and odd-even transposition networks on keys held in registers
 // (all indices static after unrolling).
 __device__ __forceinline__ void cswap(unsigned &a, unsigned &b)
 {
@@ -651,7 +652,7 @@ __device__ __forceinline__ void sort_net(unsigned *k)
 // particles; a single particle that still overflows (or alone has more than SCAP
 // samples) is blended in sample order by every thread for its own texels.
 // The sample counts come from the binning (an FP64 atan2 here would double the
-// registers), each thread keeps its two texels in registers, and the LDS arrays (20.4 KB,
+// registers), each thread keeps its two texels in registers, and the LDS arrays (20.3 KB,
 // allocated in 1280-byte granules) leave room for 8 workgroups per CU, which is also the
 // register limit.
 #define SCAP 288         // on-tile samples per range (a slot fits 9 bits, a rank in the range 7)
