@@ -17,13 +17,12 @@
 //                the next storage order, which the caller adopts after the draw;
 //   3. draw   -- one workgroup per tile, its texels in registers: per window of
 //                particles, every (particle, sample) pixel on the tile with one thread
-//                per sample, each on-tile sample appended to the texels it touches; each
-//                texel sorts its few fragments by (particle index, sample) and blends
-//                them in that order -- the reference's per-texel order.  A tile with more
+//                per sample, each on-tile sample marked in the texels it touches; each
+//                texel blends its samples in (particle index, sample) order -- the
+//                reference's per-texel order.  A tile with more
 //                than a bucket of particles takes the packed-list path (exact counts,
 //                scanned starts, the fill again), is sorted by particle index and cut
-//                into windows; a range that overflows a texel's fragment capacity is
-//                retried with half the particles.
+//                into windows.
 // The texel arithmetic is the reference's (render.c:113-154) in float with no FMA
 // contraction, so texels match the CPU path bit for bit for identical positions.
 #include <hip/hip_runtime.h>
@@ -38,7 +37,6 @@
 #define NTEX (TILE * TILE)
 #define WIN 128           // particles per window = draw workgroup size
 #define DBS WIN
-#define CAPT 12           // fragments one texel can hold per window
 #define BUCKET WIN        // slots per tile bucket: a full bucket is one draw window
 #define SCAN_BS 1024
 
@@ -58,25 +56,10 @@ __device__ __forceinline__ int arc_samples(double ang, double spacing)
 	return ns;
 }
 
-__device__ __forceinline__ dv3 arc_point_t(dv3 a, dv3 b, double tt)
-{
-	return vnorm(vadd(vscl(a, 1 - tt), vscl(b, tt)));
-}
-
 __device__ __forceinline__ dv3 arc_point(dv3 a, dv3 b, int k, int ns)
 {
-	return arc_point_t(a, b, (double) k / ns);
-}
-
-// k / ns for 1 <= k <= ns <= 4096 from inv = 1.0 / ns: the rounded product, corrected
-// once by its exact remainder (Markstein's step).  This gives the correctly rounded
-// quotient, the division's own result, for every such pair (all 8,390,656 are checked on
-// the host by the tests).
-__device__ __forceinline__ double ratio_k_ns(int k, int ns, double inv)
-{
-	double q = (double) k * inv;
-	double r = fma(-q, (double) ns, (double) k);
-	return fma(r, inv, q);
+	double tt = (double) k / ns;
+	return vnorm(vadd(vscl(a, 1 - tt), vscl(b, tt)));
 }
 
 // Pixel position of direction d on face f (render.c:91-98); false when the face is
@@ -617,25 +600,6 @@ __device__ __forceinline__ void blend_texel(float4 &dd, int t, const TileRect &t
 	dd.w = dd.w * keep + add;
 }
 
-// Compare-exchange and odd-even transposition networks on keys held in registers
-// (all indices static after unrolling).
-__device__ __forceinline__ void cswap(unsigned &a, unsigned &b)
-{
-	unsigned lo = min(a, b), hi = max(a, b);
-	a = lo;
-	b = hi;
-}
-
-template <int M>
-__device__ __forceinline__ void sort_net(unsigned *k)
-{
-#pragma unroll
-	for (int r = 0; r < M; r++)
-#pragma unroll
-		for (int i = r & 1; i + 1 < M; i += 2)
-			cswap(k[i], k[i + 1]);
-}
-
 // Per window of W particles (W <= WIN):
 //   A  one thread per particle: endpoints, sample count ns, index, colour;
 //   B  the particle's rank by index within the window (the window is index-sorted when the
@@ -644,34 +608,32 @@ __device__ __forceinline__ void sort_net(unsigned *k)
 //   then per range of consecutive ranks whose samples fit SCAP (ranges are in index order):
 //   C  one thread per (particle, sample), its rank read from a per-range sample map: the
 //      sample's pixel on this tile's face; a sample on the tile stores its position in
-//      the slot of its index in the range and appends (slot << 7 | rank - r0) to every
-//      texel it touches.  Slots are numbered in (rank, sample) order, so sorting these
-//      16-bit fragments sorts them by (particle index, sample);
-//   D  one thread per texel: sort its fragments and blend them in that order.
-// A range in which a texel collects more than CAPT fragments is retried with half the
-// particles; a single particle that still overflows (or alone has more than SCAP
-// samples) is blended in sample order by every thread for its own texels.
+//      the slot of its index in the range and sets the slot's bit in the mask of every
+//      texel it touches.  Slots are numbered in (rank, sample) order;
+//   D  one thread per texel: blend the samples of its set bits in ascending slot order,
+//      which is the (particle index, sample) order.  A texel takes any number of
+//      samples of a range: no capacity to overflow, no retries.
+// A single particle with more samples than a range holds is blended in sample order by
+// every thread for its own texels.
 // The sample counts come from the binning (an FP64 atan2 here would double the
-// registers), each thread keeps its two texels in registers, and the LDS arrays (20.3 KB,
+// registers), each thread keeps its two texels in registers, and the LDS arrays (20.0 KB,
 // allocated in 1280-byte granules) leave room for 8 workgroups per CU, which is also the
 // register limit.
-#define SCAP 288         // on-tile samples per range (a slot fits 9 bits, a rank in the range 7)
-static_assert(SCAP <= 512 && WIN <= 128, "a fragment packs slot << 7 | rank - r0 into 16 bits");
+#define SCAP 256         // on-tile samples per range: 8 mask words per texel
+static_assert(SCAP % 32 == 0 && WIN <= 256, "texel masks are whole words; ranks fit the sample map");
 __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts ps, SplatGeo sg, float invg)
 {
 	static_assert(NTEX % DBS == 0, "each thread owns NTEX / DBS texels");
 	constexpr int TPT = NTEX / DBS;
-	__shared__ int tcnt[NTEX];
-	__shared__ unsigned short frag[CAPT][NTEX];  // slot << 7 | rank - r0
+	__shared__ unsigned smask[SCAP / 32][NTEX];  // bit sl: range slot sl touches the texel
 	__shared__ float2 sxy[SCAP];
 	__shared__ double pa[3][WIN], pb[3][WIN];  // by rank
-	__shared__ double pinv[WIN];               // 1 / sample count, by rank
 	__shared__ int pst[WIN + 1];
 	__shared__ __align__(16) int ppid[WIN];    // window order; padded to 4 with INT_MAX
 	__shared__ unsigned short pns[WIN];        // sample count by rank (<= 4096)
 	__shared__ unsigned char smap[SCAP];       // rank - r0 of each sample of a range
 	__shared__ float4 pcol[WIN];               // by rank
-	__shared__ int overflow, ntouched;
+	__shared__ int ntouched;
 	long long c0 = clock64();
 	int t = xblock();
 	const int lid = threadIdx.x;
@@ -766,11 +728,9 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 		} else if (lid < ((W + 3) & ~3)) {
 			ppid[lid] = INT_MAX;
 		}
-		// the first range's tile counts and overflow flag
-		for (int i = lid; i < NTEX; i += DBS)
-			tcnt[i] = 0;
-		if (lid == 0)
-			overflow = 0;
+		// the first range's texel masks
+		for (int i = lid; i < SCAP / 32 * NTEX / 4; i += DBS)
+			((uint4 *) smask)[i] = make_uint4(0, 0, 0, 0);
 		__syncthreads();
 		// B: rank by index; store by rank
 		int rank = lid;
@@ -788,7 +748,6 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 			pb[0][rank] = b.x; pb[1][rank] = b.y; pb[2][rank] = b.z;
 			pcol[rank] = mycol;
 			pns[rank] = (unsigned short) myns;
-			pinv[rank] = 1.0 / myns;
 		}
 		__syncthreads();
 		// exclusive prefix of ns in rank order: each wave scans both halves (ranks 0-63 and
@@ -831,161 +790,122 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 				else
 					hi = mid - 1;
 			}
-			int r1 = lo;
-			bool retry = true;
-			while (retry) {
-				int j0 = pst[r0], j1 = pst[r1];
-				// a single particle with more samples than a range holds goes to the
-				// in-order fallback directly
-				bool big = j1 - j0 > SCAP;
-				if (!(prepared && r0 == 0 && r1 == W)) {
-					for (int i = lid; i < NTEX; i += DBS)
-						tcnt[i] = 0;
-					if (lid == 0)
-						overflow = big ? 1 : 0;
-					// the rank of every sample of the range (r1 - r0 <= WIN = DBS)
-					if (!big && r0 + lid < r1) {
-						int p = r0 + lid;
-						for (int k = pst[p] - j0; k < pst[p + 1] - j0; k++)
-							smap[k] = (unsigned char) lid;
-					}
-					__syncthreads();
+			const int r1 = lo;
+			const int j0 = pst[r0], j1 = pst[r1];
+			// a single particle with more samples than a range holds goes to the in-order
+			// fallback
+			const bool big = j1 - j0 > SCAP;
+			if (!(prepared && r0 == 0 && r1 == W)) {
+				for (int i = lid; i < SCAP / 32 * NTEX / 4; i += DBS)
+					((uint4 *) smask)[i] = make_uint4(0, 0, 0, 0);
+				// the rank of every sample of the range (r1 - r0 <= WIN = DBS)
+				if (!big && r0 + lid < r1) {
+					int p = r0 + lid;
+					for (int k = pst[p] - j0; k < pst[p + 1] - j0; k++)
+						smap[k] = (unsigned char) lid;
 				}
-				prepared = false;
+				__syncthreads();
+			}
+			prepared = false;
+			if (!big) {
 				// C: one thread per (particle, sample) of ranks [r0, r1), specialised for
-				// the tile's face
+				// the tile's face; an on-tile sample stores its pixel in its slot and sets
+				// the slot's bit in every texel it touches
 				auto sample_pass = [&](auto face) {
 					constexpr int F = decltype(face)::value;
 					for (int j = j0 + lid; j < j1; j += DBS) {
 						int p = r0 + smap[j - j0], k = j - pst[p] + 1;
 						dv3 pa3 = d3(pa[0][p], pa[1][p], pa[2][p]), pb3 = d3(pb[0][p], pb[1][p], pb[2][p]);
 						float px, py;
-						dv3 d = arc_point_t(pa3, pb3, ratio_k_ns(k, pns[p], pinv[p]));
-						if (!face_pixel_c<F>(d, sg.R, px, py))
+						if (!face_pixel_c<F>(arc_point(pa3, pb3, k, pns[p]), sg.R, px, py))
 							continue;
 						int xlo, xhi, ylo, yhi;
 						if (!tile_clip(tr, px, py, rad, sg.R, xlo, xhi, ylo, yhi))
 							continue;
-						int sl = j - j0; // the range's samples fit SCAP slots
+						const int sl = j - j0; // the range's samples fit SCAP slots
 						sxy[sl] = make_float2(px, py);
-						const unsigned short fv = (unsigned short) ((sl << 7) | (p - r0));
-						bool over = false;
+						unsigned *mrow = smask[sl >> 5];
+						const unsigned bit = 1u << (sl & 31);
 						if (rad <= 1.0f) {
 							bool on[4];
-							int tx[4], c[4];
+							int tx[4];
 							splat_texels_2x2(tr, px, py, invrad, xlo, xhi, ylo, yhi, on, tx);
-	#pragma unroll
+#pragma unroll
 							for (int q = 0; q < 4; q++)
-								c[q] = on[q] ? atomicAdd(&tcnt[tx[q]], 1) : 0;
-	#pragma unroll
-							for (int q = 0; q < 4; q++) {
-								if (on[q]) {
-									if (c[q] < CAPT)
-										frag[c[q]][tx[q]] = fv;
-									else
-										over = true;
-								}
-							}
+								if (on[q])
+									atomicOr(&mrow[tx[q]], bit);
 						} else {
-							splat_texels(tr, px, py, rad, invrad, sg.R, [&](int tx) {
-								int c = atomicAdd(&tcnt[tx], 1);
-								if (c < CAPT)
-									frag[c][tx] = fv;
-								else
-									over = true;
-							});
+							splat_texels(tr, px, py, rad, invrad, sg.R,
+								     [&](int tx) { atomicOr(&mrow[tx], bit); });
 						}
-						if (over)
-							overflow = 1;
 					}
 				};
-				if (!big) {
-					switch (tr.f) {
-					case 0: sample_pass(std::integral_constant<int, 0>()); break;
-					case 1: sample_pass(std::integral_constant<int, 1>()); break;
-					case 2: sample_pass(std::integral_constant<int, 2>()); break;
-					case 3: sample_pass(std::integral_constant<int, 3>()); break;
-					case 4: sample_pass(std::integral_constant<int, 4>()); break;
-					default: sample_pass(std::integral_constant<int, 5>()); break;
-					}
+				switch (tr.f) {
+				case 0: sample_pass(std::integral_constant<int, 0>()); break;
+				case 1: sample_pass(std::integral_constant<int, 1>()); break;
+				case 2: sample_pass(std::integral_constant<int, 2>()); break;
+				case 3: sample_pass(std::integral_constant<int, 3>()); break;
+				case 4: sample_pass(std::integral_constant<int, 4>()); break;
+				default: sample_pass(std::integral_constant<int, 5>()); break;
 				}
 				__syncthreads();
 				long long cb = clock64();
 				csamp += cb - cc;
-				if (!overflow) {
-					// D: each texel sorts its fragments and blends them in that order.  The
-					// fragment reads are unconditional (entries past n hold stale, unused
-					// values, replaced by 0xffff, above any fragment) so they issue together.
+				// D: each texel blends the samples whose bits it holds, in slot order --
+				// the (particle index, sample) order
+#pragma unroll
+				for (int h = 0; h < TPT; h++) {
+					const int tx = lid + h * DBS;
+					unsigned m[SCAP / 32], any = 0;
+#pragma unroll
+					for (int w = 0; w < SCAP / 32; w++) {
+						m[w] = smask[w][tx];
+						any |= m[w];
+					}
+					if (!any)
+						continue;
+					touched |= 1u << h;
+					float4 dd = tex[h];
+					// 64-bit words: fewer loops whose lanes diverge over the words
+#pragma unroll
+					for (int w = 0; w < SCAP / 64; w++) {
+						unsigned long long bits =
+							((unsigned long long) m[2 * w + 1] << 32) | m[2 * w];
+						while (bits) {
+							const int sl = w * 64 + __ffsll((long long) bits) - 1;
+							bits &= bits - 1;
+							float2 s2 = sxy[sl];
+							float4 c = pcol[r0 + smap[sl]];
+							blend_texel(dd, tx, tr, s2.x, s2.y, invrad, r.alpha * c.w, c, invg);
+						}
+					}
+					tex[h] = dd;
+				}
+				cblend += clock64() - cb;
+			} else {
+				// one particle with more samples than a range holds: every thread walks
+				// them in order and blends those that cover its own texels
+				serial_windows++;
+				dv3 pa3 = d3(pa[0][r0], pa[1][r0], pa[2][r0]);
+				dv3 pb3 = d3(pb[0][r0], pb[1][r0], pb[2][r0]);
+				int ns = pns[r0];
+				float4 c = pcol[r0];
+				for (int k = 1; k <= ns; k++) {
+					float px, py;
+					if (!face_pixel(tr.f, arc_point(pa3, pb3, k, ns), sg.R, px, py))
+						continue;
 #pragma unroll
 					for (int h = 0; h < TPT; h++) {
 						int tx = lid + h * DBS;
-						int n = tcnt[tx];
-						if (n == 0)
-							continue;
-						touched |= 1u << h;
-						unsigned keys[CAPT];
-#pragma unroll
-						for (int q = 0; q < 4; q++) {
-							unsigned v = frag[q][tx];
-							keys[q] = q < n ? v : 0xffffu;
-						}
-						if (__any(n > 4)) {
-#pragma unroll
-							for (int q = 4; q < CAPT; q++) {
-								unsigned v = frag[q][tx];
-								keys[q] = q < n ? v : 0xffffu;
-							}
-						}
-						if (n <= 4)
-							sort_net<4>(keys);
-						else if (CAPT > 8 && n <= 8)
-							sort_net<8>(keys);
-						else
-							sort_net<CAPT>(keys);
-						float4 dd = tex[h];
-#pragma unroll
-						for (int q = 0; q < CAPT; q++) {
-							if (q < n) {
-								float2 s2 = sxy[keys[q] >> 7];
-								float4 c = pcol[r0 + (keys[q] & 127u)];
-								blend_texel(dd, tx, tr, s2.x, s2.y, invrad, r.alpha * c.w, c,
-									    invg);
-							}
-						}
-						tex[h] = dd;
-					}
-					retry = false;
-					r0 = r1;
-				} else if (r1 - r0 > 1) {
-					r1 = r0 + (r1 - r0) / 2;
-				} else {
-					// one particle that alone overflows: every thread walks its samples
-					// in order and blends those that cover its own texels
-					serial_windows++;
-					dv3 pa3 = d3(pa[0][r0], pa[1][r0], pa[2][r0]);
-					dv3 pb3 = d3(pb[0][r0], pb[1][r0], pb[2][r0]);
-					int ns = pns[r0];
-					float4 c = pcol[r0];
-					for (int k = 1; k <= ns; k++) {
-						float px, py;
-						if (!face_pixel(tr.f, arc_point(pa3, pb3, k, ns), sg.R, px, py))
-							continue;
-#pragma unroll
-						for (int h = 0; h < TPT; h++) {
-							int tx = lid + h * DBS;
-							if (texel_in_splat(tr, tx, px, py, rad, invrad, sg.R)) {
-								touched |= 1u << h;
-								blend_texel(tex[h], tx, tr, px, py, invrad, r.alpha * c.w, c,
-									    invg);
-							}
+						if (texel_in_splat(tr, tx, px, py, rad, invrad, sg.R)) {
+							touched |= 1u << h;
+							blend_texel(tex[h], tx, tr, px, py, invrad, r.alpha * c.w, c, invg);
 						}
 					}
-					retry = false;
-					r0 = r1;
 				}
-				__syncthreads();
-				cblend += clock64() - cb;
 			}
+			__syncthreads();
+			r0 = r1;
 		}
 	}
 
