@@ -37,7 +37,7 @@
 #define NTEX (TILE * TILE)
 #define WIN 128           // particles per window = draw workgroup size
 #define DBS WIN
-#define BUCKET WIN        // slots per tile bucket: a full bucket is one draw window
+#define BUCKET (2 * WIN)  // slots per tile bucket: two draw windows
 #define SCAN_BS 1024
 
 struct SplatGeo {
@@ -648,8 +648,8 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 	const float rad = sg.rad, invrad = 1.0f / rad;
 
 	// The tile's count, the entry total and longest list, the tile's texels (thread lid
-	// owns texels lid + h DBS and keeps them in registers) and its bucket entries (at most
-	// BUCKET = DBS, one per thread) are loaded together, unconditionally and in bounds (the
+	// owns texels lid + h DBS and keeps them in registers) and its first window of bucket
+	// entries (one per thread) are loaded together, unconditionally and in bounds (the
 	// buckets exist in both modes): one round trip before the particles' data.  Entries
 	// past the count are stale and never used.
 	const int *cnt_hi = r.bucket_mode ? r.tile_cursor + t : r.tile_start + t + 1;
@@ -667,7 +667,7 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 		if (ly < tr.h && lx < tr.w)
 			tex[h] = gbuf[((size_t) tr.f * r.R + tr.y0 + ly) * r.R + tr.x0 + lx];
 	}
-	static_assert(BUCKET == DBS, "one bucket entry per thread");
+	static_assert(BUCKET >= DBS && BUCKET <= 2 * WIN, "a bucket holds one or two windows");
 	int bslot = r.bucket[(size_t) t * BUCKET + lid];
 	// keeps the entry's load above the early return below (the compiler would sink it
 	// there, into a round trip of its own)
@@ -682,19 +682,23 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 			diag[lid] = 0;
 		return;
 	}
-	const int *lst = r.bucket_mode ? nullptr : r.list + r.tile_start[t];
+	const int *lst = r.bucket_mode ? r.bucket + (size_t) t * BUCKET : r.list + c_lo;
 	if (lid == 0)
 		ntouched = 0;
 
 	// a tile with more than WIN particles is cut into index-ordered windows: sort the
-	// (index, slot) pairs in this tile's part of the scratch array
+	// (index, slot) pairs -- a bucket's (at most two windows) in LDS, in the texel masks'
+	// space, with each thread then keeping its slot of both windows in registers (the
+	// masks are cleared per window); a packed list's in its part of the scratch array
 	unsigned long long *sorted = nullptr;
 	bool is_sorted = L > WIN;
+	int wslot0 = 0, wslot1 = 0;
 	if (is_sorted) {
 		int P = 1;
 		while (P < L)
 			P <<= 1;
-		sorted = r.sort_scratch + 2 * (size_t) r.tile_start[t];
+		static_assert(sizeof(smask) >= 8 * BUCKET, "a bucket's keys fit the mask space");
+		sorted = r.bucket_mode ? (unsigned long long *) smask : r.sort_scratch + 2 * (size_t) c_lo;
 		for (int i = lid; i < P; i += DBS) {
 			unsigned long long key = ~0ull;
 			if (i < L) {
@@ -705,6 +709,10 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 		}
 		__syncthreads();
 		bitonic_sort64(sorted, P);
+		if (r.bucket_mode) {
+			wslot0 = (int) (unsigned) sorted[lid];
+			wslot1 = (int) (unsigned) sorted[WIN + lid];
+		}
 	}
 	__syncthreads();
 
@@ -718,7 +726,8 @@ __global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts p
 		int myns = 0, mypid = 0;
 		dv3 a = d3(0, 0, 0), b = d3(0, 0, 0);
 		if (lid < W) {
-			int slot = is_sorted ? (int) (unsigned) sorted[base + lid] : (lst ? lst[lid] : bslot);
+			int slot = !is_sorted ? (r.bucket_mode ? bslot : lst[lid])
+					      : r.bucket_mode ? (base == 0 ? wslot0 : wslot1) : (int) (unsigned) sorted[base + lid];
 			a = ld3(ps.prev + 3 * (size_t) slot);
 			b = ld3(ps.pos + 3 * (size_t) slot);
 			myns = r.nsamp[slot];

@@ -251,6 +251,41 @@ def test_render_crowded_and_antipodal(ggb, orc, gpu):
     assert np.max(np.abs(a - b)) <= 1
 
 
+def test_render_two_windows_in_a_bucket(ggb, orc, gpu):
+    """A tile holding between one and two windows of particles (129-256): its bucket is
+    sorted by particle index in LDS and cut into two windows, against the oracle."""
+    R, n = 128, 600
+    rng = np.random.default_rng(13)
+    # 200 particles packed into one 16x16-texel tile of face 0, the rest spread out
+    pos = rng.standard_normal((n, 3))
+    pos[:200] = np.array([0.06, 0.05, 1.0]) + 0.02 * rng.uniform(-1, 1, (200, 3))
+    pos /= np.linalg.norm(pos, axis=1, keepdims=True)
+    prev = pos + 0.004 * rng.standard_normal((n, 3))
+    prev /= np.linalg.norm(prev, axis=1, keepdims=True)
+    P = ggb.Particles(n)
+    P.set(pos, prev)
+    OP = orc.OracleParticles("f64", n)
+    OP.pos[:] = pos.ravel()
+    OP.prev[:] = prev.ravel()
+    img = ggb.strip_to_float(ggb.banded_strip(9))
+    P.color_from_image(img, 64, 1024)
+    OP.color_from_image(img, 64, 1024)
+    Rr = ggb.Render(R)
+    OR = orc.OracleRender("f64", R)
+    for _ in range(2):
+        Rr.fade(0.02)
+        OR.fade(0.02)
+        Rr.deposit(P)
+        OR.deposit(OP)
+        st = Rr.last_stats()
+        assert 128 < st[1] <= ggb.lib().ggbk_render_bucket(), st[1]
+        assert st[4] > 0, "no tile was cut into index-sorted windows"
+    buf = Rr.sync_host()
+    assert np.max(np.abs(buf - OR.buf)) < 1e-5
+    a, b = Rr.faces8().astype(int), OR.faces8().astype(int)
+    assert np.max(np.abs(a - b)) <= 1
+
+
 @pytest.mark.parametrize("R,radius", [(100, 1.7), (72, 0.6)])
 def test_render_radius_and_partial_tiles(ggb, orc, gpu, R, radius):
     """Splat radii other than 1 (above 1 the generic texel walk, below it the 2x2 path) on
