@@ -1,0 +1,59 @@
+"""bench.py's command-line contract and workload definition, without a GPU: one GPU
+only (a larger --gpus exits with status 2 and says so), the default timed window is the
+N=512 workload's whole run (2000 steps after 5 warm-up steps), the workload is the one
+the metric is quoted on, and the draw's and fill's algorithmic byte counts follow the
+per-pass model (texels read and written, per-entry particle data)."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def bench():
+    sys.path.insert(0, ROOT)
+    import bench as b
+    return b
+
+
+def test_more_than_one_gpu_is_refused():
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 2
+    assert "one GPU" in p.stderr
+    assert p.stdout == ""
+
+
+def test_default_window_is_the_whole_run():
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"],
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0
+    src = open(os.path.join(ROOT, "bench.py")).read()
+    assert 'add_argument("--steps", type=int, default=2000)' in src
+    assert 'add_argument("--warmup", type=int, default=5)' in src
+
+
+def test_workload_is_the_quoted_configuration(bench):
+    c2 = bench.CONFIGS["c2"]
+    assert (c2["N"], c2["P"], c2["R"]) == (512, 4_000_000, 2048)
+    assert (c2["forcing"], c2["rotation"], c2["cg_tol"]) == (0.5, 10.0, 1e-8)
+    assert c2["precision"] == bench.ggb.PREC_MIXED
+    c4 = bench.CONFIGS["c4"]
+    assert (c4["N"], c4["P"], c4["R"], c4["octaves"]) == (1024, 16_000_000, 4096, 6)
+
+
+def test_byte_model(bench):
+    n, P, E, T = 1_572_864, 4_000_000, 5_000_000, 19_000_000
+    # draw: 16 B read + 16 B written per touched texel; per entry the list entry, the FP64
+    # endpoints, the colour and the index
+    assert bench.kernel_bytes("k_render_draw", n, P, E, T, 8, 4, 4) == 32 * T + E * (4 + 48 + 16 + 4)
+    # fill: endpoints and sample count always; colour, index and the copy only when reordering
+    plain = bench.kernel_bytes("k_render_fill", n, P, E, T, 8, 4, 4, 0.0)
+    reord = bench.kernel_bytes("k_render_fill", n, P, E, T, 8, 4, 4, 1.0)
+    assert plain == P * 50 + E * 4
+    assert reord - plain == P * 88
+    with pytest.raises(KeyError):
+        bench.kernel_bytes("no_such_kernel", n, P, E, T, 8, 4, 4)
