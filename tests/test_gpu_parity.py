@@ -286,6 +286,46 @@ def test_render_two_windows_in_a_bucket(ggb, orc, gpu):
     assert np.max(np.abs(a - b)) <= 1
 
 
+def test_render_developed_flow(ggb, orc, gpu):
+    """One deposit of particles from a developed flow (150 steps at N=128 with forcing 0.5
+    and rotation 10: zonal jets, arcs of up to 9 samples, overlapping trails; about 49
+    particles per tile as at N=512 with 4M particles), against the oracle on the same
+    positions.  Texels then hold many samples of a range, and tiles need several ranges."""
+    N, n, R = 128, 300_000, 512
+    g = ggb.Grid(N)
+    vel0, vband = g.initial_velocity(seed=1, octaves=3)
+    F = ggb.Fluid(g, forcing=0.5, rotation=10.0, cg_tol=1e-8, precision=ggb.PREC_MIXED)
+    F.set_state(vel0, vband)
+    P = ggb.Particles(n)
+    P.color_from_image(ggb.strip_to_float(ggb.banded_strip(1)), 64, 1024)
+    for _ in range(150):
+        dt = g.h / max(F.max_speed(), 1e-6)
+        P.advect(g, F, dt)
+        F.step(dt)
+    pos, prev = P.get()
+    # colours are assigned from the initial lattice positions, as on the device
+    OP = orc.OracleParticles("f64", n)
+    OP.color_from_image(ggb.strip_to_float(ggb.banded_strip(1)), 64, 1024)
+    OP.pos[:] = pos.ravel()
+    OP.prev[:] = prev.ravel()
+    Rr = ggb.Render(R)
+    OR = orc.OracleRender("f64", R)
+    Rr.fade(0.02)
+    OR.fade(0.02)
+    Rr.deposit(P)
+    OR.deposit(OP)
+    st = Rr.last_stats()
+    ntiles = 6 * (R // 16) ** 2
+    print("entries %d, longest list %d, ranges %d for %d tiles" % (st[0], st[1], st[2], ntiles))
+    assert st[2] > ntiles  # some tiles need more than one range
+    buf = Rr.sync_host()
+    assert np.max(np.abs(buf - OR.buf)) < 1e-5
+    a, b = Rr.faces8().astype(int), OR.faces8().astype(int)
+    assert np.max(np.abs(a - b)) <= 1
+    assert np.count_nonzero(a != b) <= 0.001 * a[..., 0].size
+    F.free()
+
+
 @pytest.mark.parametrize("R,radius", [(100, 1.7), (72, 0.6)])
 def test_render_radius_and_partial_tiles(ggb, orc, gpu, R, radius):
     """Splat radii other than 1 (above 1 the generic texel walk, below it the 2x2 path) on
