@@ -11,8 +11,8 @@ render's cost per step roughly doubles, so a short early window would not repres
 
 Reports (one JSON line): fluid steps/s (mixed; the headline `value`), Msplat-particles/s,
 the FP64 path's fluid steps/s on the same workload, the true FP64 relative residual of
-both, the roofline of the dominant kernel (HIP events on the library stream over the
-timed steps), and the CPU baseline timed on this host.
+both, the roofline of the dominant kernel, the draw (HIP events on the library stream over
+the timed steps), and the CPU baseline timed on this host.
 """
 import argparse
 import ctypes as C
@@ -299,6 +299,12 @@ def main():
                     help="skip the mixed-vs-FP64 error run (steps 10, 100, 2000)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain copy kernel of MIB MiB (counter calibration)")
+    # the draw has the most device time over a whole run (a quarter of it at N=512);
+    # over the warm-up's few early steps a multigrid kernel can come out ahead, so the
+    # warm-up's choice ("auto") is not the default
+    ap.add_argument("--roofline-kernel", default="k_render_draw",
+                    choices=sorted(PROBES) + ["auto"],
+                    help="kernel whose roofline is reported (auto: most time in the warm-up)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -338,8 +344,9 @@ def main():
         cnt = L.ggbk_probe_collect(pid, C.byref(ms))
         warm[name] = (cnt, ms.value)
         L.ggbk_probe_enable(pid, 0)
-    dominant = max(warm, key=lambda k: warm[k][1])
-    log("warmup done (%d steps); dominant kernel %s" % (args.warmup, dominant))
+    top = max(warm, key=lambda k: warm[k][1])
+    dominant = top if args.roofline_kernel == "auto" else args.roofline_kernel
+    log("warmup done (%d steps); most warm-up time %s; roofline kernel %s" % (args.warmup, top, dominant))
     # timed region: only the dominant kernel is bracketed by events
     L.ggbk_probe_enable(PROBES[dominant], 1)
     t0 = time.perf_counter()

@@ -616,12 +616,14 @@ __device__ __forceinline__ void blend_texel(float4 &dd, int t, const TileRect &t
 // A single particle with more samples than a range holds is blended in sample order by
 // every thread for its own texels.
 // The sample counts come from the binning (an FP64 atan2 here would double the
-// registers), each thread keeps its two texels in registers, and the LDS arrays (20.0 KB,
-// allocated in 1280-byte granules) leave room for 8 workgroups per CU, which is also the
-// register limit.
-#define SCAP 256         // on-tile samples per range: 8 mask words per texel
-static_assert(SCAP % 32 == 0 && WIN <= 256, "texel masks are whole words; ranks fit the sample map");
-__global__ __launch_bounds__(DBS) void k_render_draw(ggbk_render r, ggbk_parts ps, SplatGeo sg, float invg)
+// registers) and each thread keeps its two texels in registers.  The LDS arrays (14.7 KB,
+// allocated in 1280-byte granules) leave room for 10 workgroups per CU, and the launch
+// bound of 5 waves per SIMD holds the registers to 96, which is the same 10.  Ranges of
+// 128 slots measured faster over a developed flow than 256 (8 workgroups per CU), 192
+// (9) and 64 (more ranges per window).
+#define SCAP 128         // on-tile samples per range: 4 mask words per texel
+static_assert(SCAP % 64 == 0 && WIN <= 256, "texel masks are whole 64-bit words; ranks fit the sample map");
+__global__ __launch_bounds__(DBS, 5) void k_render_draw(ggbk_render r, ggbk_parts ps, SplatGeo sg, float invg)
 {
 	static_assert(NTEX % DBS == 0, "each thread owns NTEX / DBS texels");
 	constexpr int TPT = NTEX / DBS;

@@ -1,8 +1,9 @@
 """bench.py's command-line contract and workload definition, without a GPU: one GPU
 only (a larger --gpus exits with status 2 and says so), the default timed window is the
-N=512 workload's whole run (2000 steps after 5 warm-up steps), the workload is the one
-the metric is quoted on, and the draw's and fill's algorithmic byte counts follow the
-per-pass model (texels read and written, per-entry particle data)."""
+N=512 workload's whole run (2000 steps after 5 warm-up steps), the roofline is the draw's
+by default, the workload is the one the metric is quoted on, and the draw's and fill's
+algorithmic byte counts follow the per-pass model (texels read and written, per-entry
+particle data)."""
 import os
 import subprocess
 import sys
@@ -34,6 +35,14 @@ def test_default_window_is_the_whole_run():
     src = open(os.path.join(ROOT, "bench.py")).read()
     assert 'add_argument("--steps", type=int, default=2000)' in src
     assert 'add_argument("--warmup", type=int, default=5)' in src
+
+
+def test_roofline_kernel_defaults_to_the_draw(bench):
+    # the draw has the most device time over the whole run; the warm-up's early steps
+    # do not decide it unless asked ("auto")
+    src = open(os.path.join(ROOT, "bench.py")).read()
+    assert 'add_argument("--roofline-kernel", default="k_render_draw"' in src
+    assert "k_render_draw" in bench.PROBES
 
 
 def test_workload_is_the_quoted_configuration(bench):
