@@ -213,10 +213,11 @@ def test_ref32_matches_reference(ggb, orc, ref, gpu):
 
 
 def test_render_crowded_and_antipodal(ggb, orc, gpu):
-    """Tiles holding several windows of particles (index-sorted windows, sample ranges
-    split when a texel overflows) and particles whose endpoints are antipodal (every
-    sample of the arc falls on one of two points, so one particle alone overflows a texel
-    and is blended by the in-order fallback), against the oracle's serial blend."""
+    """Tiles holding several windows of particles (index-sorted windows, each cut into
+    several sample ranges) and particles whose endpoints are antipodal (every sample of
+    the arc falls on one of two points, and one particle alone has more samples than a
+    range holds, so it is blended by the in-order fallback), against the oracle's serial
+    blend."""
     R, n, ncl, nanti = 128, 3000, 2000, 24
     rng = np.random.default_rng(11)
     c = np.array([0.3, 0.2, 0.93])
