@@ -37,6 +37,22 @@ void ggb_die(const char *fmt, ...)
 
 int ggb_ensure_device(void) { return ggbk_init(); }
 
+/* -1: not yet read from the environment */
+static int g_sync_mirrors = -1;
+
+int ggb_sync_mirrors(void)
+{
+	int v = __atomic_load_n(&g_sync_mirrors, __ATOMIC_RELAXED);
+	if (v < 0) {
+		const char *e = getenv("GGB_SYNC_MIRRORS");
+		v = e && *e && strcmp(e, "0") != 0;
+		__atomic_store_n(&g_sync_mirrors, v, __ATOMIC_RELAXED);
+	}
+	return v;
+}
+
+void ggb_set_sync_mirrors(int on) { __atomic_store_n(&g_sync_mirrors, on ? 1 : 0, __ATOMIC_RELAXED); }
+
 int ggb_device_init(void) { return ggbk_init(); }
 
 const char *ggb_version(void) { return GGB_VERSION; }
@@ -1088,6 +1104,8 @@ void fluid_step(Fluid *F, double dt)
 	psi_mean_and_velocity(F);
 	if (ggbk_check("fluid_step"))
 		ggb_die("fluid_step: %s", ggbk_last_error());
+	if (ggb_sync_mirrors())
+		fluid_sync_host(F);
 }
 
 double fluid_max_speed(const Fluid *F)
@@ -1440,6 +1458,12 @@ int fluid_adjoint_backward(FluidAdjoint *A, const double *g_omega_T, double *g_o
 			adj_step(A, A->dts[s], t, t + n);
 		}
 	}
+	/* leave the fluid in the rollout's initial state (checkpoint 0), with its velocity
+	 * and maximum speed recomputed, rather than the mid-segment state of the last
+	 * recomputation */
+	ggbk_d2d(D->omega, A->ckpt, 8 * n);
+	ggbk_d2d(D->psi, A->ckpt + n, 8 * n);
+	velocity_device(F);
 	/* the control passes through set-up's mean removal */
 	ggbk_adj_sum(g, A->lam, D->red);
 	ggbk_fin(GGBK_FIN_MEAN_F64, D->s, D->red, D->nb0, 0, 0, g.total_area);

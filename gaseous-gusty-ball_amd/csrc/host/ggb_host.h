@@ -26,6 +26,9 @@ void ggb_grid_dev_release(CubeSphere *cs);
 void *ggb_dmalloc(size_t bytes, const char *what);
 void ggb_die(const char *fmt, ...);
 int ggb_ensure_device(void);
+/* Whether fluid_step and particles_advect refresh the host mirrors (ggb_set_sync_mirrors,
+ * or GGB_SYNC_MIRRORS=1 in the environment). */
+int ggb_sync_mirrors(void);
 
 /* Live fluids, so particles_advect can map F.vel to the device velocity. */
 const Fluid *ggb_fluid_of_vel(const float *vel);

@@ -188,6 +188,8 @@ void particles_advect(Particles *P, const CubeSphere *cs, const float *vel, doub
 		T = GGBK_F32;
 	}
 	ggbk_particles_advect(g, P->n, D->cur.pos, D->cur.prev, dvel, T, dt, D->round32);
+	if (ggb_sync_mirrors())
+		particles_sync_host(P);
 }
 
 void particles_sync_host(Particles *P)

@@ -254,6 +254,17 @@ void render_pixel8(const float c[4], const float *bg, unsigned char *out)
 	}
 }
 
+/* Device staging for the 8-bit outputs (grown on demand). */
+static unsigned char *out8_buf(struct ggb_render_dev *D, size_t bytes)
+{
+	if (D->faces8_bytes < bytes) {
+		ggbk_free(D->faces8);
+		D->faces8 = ggb_dmalloc(bytes, "render 8-bit output");
+		D->faces8_bytes = bytes;
+	}
+	return D->faces8;
+}
+
 /* The whole buffer through render_texel + render_pixel8, converted on the device: the
  * copy back is the 8-bit faces (a quarter or less of the float buffer). */
 void render_faces8(Render *r, const float *bg, unsigned char *out)
@@ -261,12 +272,33 @@ void render_faces8(Render *r, const float *bg, unsigned char *out)
 	struct ggb_render_dev *D = r->dev;
 	int ch = bg ? 3 : 4;
 	size_t ntex = 6 * (size_t) r->R * r->R, bytes = ntex * ch;
-	if (D->faces8_bytes < ntex * 4) {
-		ggbk_free(D->faces8);
-		D->faces8 = ggb_dmalloc(ntex * 4, "render faces8");
-		D->faces8_bytes = ntex * 4;
-	}
-	ggbk_render_faces8(D->k.buf, ntex, (float) r->g, bg, D->faces8);
-	if (ggbk_d2h(out, D->faces8, bytes))
+	unsigned char *d = out8_buf(D, ntex * 4);
+	ggbk_render_faces8(D->k.buf, ntex, (float) r->g, bg, d);
+	if (ggbk_d2h(out, d, bytes))
 		ggb_die("render: %s", ggbk_last_error());
+}
+
+/* main.c:280-295 on the device */
+void render_net8(Render *r, const float *bg, unsigned char *out)
+{
+	struct ggb_render_dev *D = r->dev;
+	size_t bytes = 12 * (size_t) r->R * r->R * 4;
+	unsigned char *d = out8_buf(D, bytes);
+	ggbk_render_net8(D->k.buf, r->R, (float) r->g, bg, d);
+	if (ggbk_d2h(out, d, bytes))
+		ggb_die("render: %s", ggbk_last_error());
+}
+
+/* main.c:297-312 on the device */
+int render_preview8(Render *r, const float *bg, int W, int H, unsigned char *out)
+{
+	struct ggb_render_dev *D = r->dev;
+	if (W < 1 || H < 1)
+		return -1;
+	size_t bytes = (size_t) W * H * (bg ? 3 : 4);
+	unsigned char *d = out8_buf(D, (size_t) W * H * 4);
+	ggbk_render_preview8(D->k.buf, r->R, (float) r->g, W, H, bg, d);
+	if (ggbk_d2h(out, d, bytes))
+		ggb_die("render: %s", ggbk_last_error());
+	return 0;
 }

@@ -277,6 +277,8 @@ void ggbk_render_scale(float *buf, size_t count, float g);
 /* 8-bit texels of the whole buffer scaled by g (the reference's render_texel + to8 /
  * render_pixel8): ch = 4 un-premultiplied RGBA (bg NULL), ch = 3 composited over bg */
 void ggbk_render_faces8(const float *buf, size_t ntexels, float g, const float *bg, unsigned char *out);
+void ggbk_render_net8(const float *buf, int R, float g, const float *bg, unsigned char *out);
+void ggbk_render_preview8(const float *buf, int R, float g, int W, int H, const float *bg, unsigned char *out);
 /* sums the per-tile diagnostics of the last deposit and copies the 16 values
  * (synchronises) */
 int ggbk_render_stats(const ggbk_render *r, long long *out16);

@@ -1010,44 +1010,156 @@ __device__ __forceinline__ unsigned char to8(float v)
 	return (unsigned char) (v * 255.0f + 0.5f);
 }
 
-// One texel per thread: c = texel * g (render.c:220-226), then render_pixel8
-// (render.c:260-279) in the same float operations.
-__global__ void k_render_faces8(const float4 *buf, size_t ntex, float g, int has_bg, float bg0, float bg1, float bg2,
-				unsigned char *out)
+// render_pixel8 (render.c:260-279) of the scaled texel c, in the reference's float
+// operations: 4 bytes (un-premultiplied RGBA) without a background, 3 composited over it.
+struct Bg {
+	int on;
+	float r, g, b;
+};
+
+__device__ __forceinline__ void pixel8(float c0, float c1, float c2, float c3, const Bg &bg, unsigned char o[4])
 {
-	size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= ntex)
-		return;
-	float4 d = buf[i];
-	float c0 = d.x * g, c1 = d.y * g, c2 = d.z * g, c3 = d.w * g;
 	float a = c3;
 	if (a > 1)
 		a = 1;
 	if (a < 0)
 		a = 0;
-	if (!has_bg) {
-		uchar4 o;
+	if (!bg.on) {
 		if (a > 1e-6f) {
-			o.x = to8(c0 / a);
-			o.y = to8(c1 / a);
-			o.z = to8(c2 / a);
+			o[0] = to8(c0 / a);
+			o[1] = to8(c1 / a);
+			o[2] = to8(c2 / a);
 		} else {
-			o.x = o.y = o.z = 0;
+			o[0] = o[1] = o[2] = 0;
 		}
-		o.w = to8(a);
-		((uchar4 *) out)[i] = o;
+		o[3] = to8(a);
 	} else {
-		out[3 * i] = to8(c0 + bg0 * (1 - a));
-		out[3 * i + 1] = to8(c1 + bg1 * (1 - a));
-		out[3 * i + 2] = to8(c2 + bg2 * (1 - a));
+		o[0] = to8(c0 + bg.r * (1 - a));
+		o[1] = to8(c1 + bg.g * (1 - a));
+		o[2] = to8(c2 + bg.b * (1 - a));
+	}
+}
+
+static Bg make_bg(const float *bg)
+{
+	Bg b;
+	b.on = bg ? 1 : 0;
+	b.r = bg ? bg[0] : 0.0f;
+	b.g = bg ? bg[1] : 0.0f;
+	b.b = bg ? bg[2] : 0.0f;
+	return b;
+}
+
+// One texel per thread: c = texel * g (render.c:220-226), then render_pixel8.
+__global__ void k_render_faces8(const float4 *buf, size_t ntex, float g, Bg bg, unsigned char *out)
+{
+	size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= ntex)
+		return;
+	float4 d = buf[i];
+	unsigned char o[4];
+	pixel8(d.x * g, d.y * g, d.z * g, d.w * g, bg, o);
+	if (!bg.on) {
+		((uchar4 *) out)[i] = make_uchar4(o[0], o[1], o[2], o[3]);
+	} else {
+		out[3 * i] = o[0];
+		out[3 * i + 1] = o[1];
+		out[3 * i + 2] = o[2];
 	}
 }
 
 extern "C" void ggbk_render_faces8(const float *buf, size_t ntexels, float g, const float *bg, unsigned char *out)
 {
 	unsigned nb = (unsigned) ((ntexels + 255) / 256);
-	GGBK_LAUNCH(k_render_faces8, nb, 256, 0, (const float4 *) buf, ntexels, g, bg ? 1 : 0, bg ? bg[0] : 0.0f,
-		    bg ? bg[1] : 0.0f, bg ? bg[2] : 0.0f, out);
+	GGBK_LAUNCH(k_render_faces8, nb, 256, 0, (const float4 *) buf, ntexels, g, make_bg(bg), out);
+}
+
+// The 4R x 3R cross net (the reference driver's main.c:280-295): one thread per net pixel;
+// face f occupies cell (cx[f], cy[f]) = (0,1) (1,1) (2,1) (3,1) (0,0) (0,2); a face pixel
+// starts as {0, 0, 0, 255} before render_pixel8 (so it keeps alpha 255 over a background),
+// and pixels outside the six cells are {0, 0, 0, 0}.
+__global__ void k_render_net8(const float4 *buf, int R, float g, Bg bg, uchar4 *out)
+{
+	size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	size_t W = 4 * (size_t) R;
+	if (i >= W * 3 * (size_t) R)
+		return;
+	int X = (int) (i % W), Y = (int) (i / W);
+	int cx = X / R, cy = Y / R, x = X - cx * R, y = Y - cy * R;
+	int f = cy == 1 ? cx : (cx == 0 ? (cy == 0 ? 4 : 5) : -1);
+	unsigned char o[4] = { 0, 0, 0, 0 };
+	if (f >= 0) {
+		float4 d = buf[((size_t) f * R + y) * R + x];
+		o[3] = 255;
+		pixel8(d.x * g, d.y * g, d.z * g, d.w * g, bg, o);
+	}
+	out[i] = make_uchar4(o[0], o[1], o[2], o[3]);
+}
+
+extern "C" void ggbk_render_net8(const float *buf, int R, float g, const float *bg, unsigned char *out)
+{
+	size_t npx = 12 * (size_t) R * R;
+	unsigned nb = (unsigned) ((npx + 255) / 256);
+	GGBK_LAUNCH(k_render_net8, nb, 256, 0, (const float4 *) buf, R, g, make_bg(bg), (uchar4 *) out);
+}
+
+// The equirectangular preview (main.c:297-312): pixel (x, y) looks in the direction of
+// longitude (x + 0.5) / W * 2 pi - pi and latitude pi / 2 - (y + 0.5) / H * pi (vec3.h:55-58),
+// then render_sample_dir (render.c:228-251): gnomonic texel position on the direction's
+// face, clamped, and the float bilinear blend of the four scaled texels, then render_pixel8.
+__global__ void k_render_preview8(const float4 *buf, int R, float g, int W, int H, Bg bg, unsigned char *out)
+{
+	size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= (size_t) W * H)
+		return;
+	int x = (int) (i % (size_t) W), y = (int) (i / (size_t) W);
+	double lat = GGB_PI / 2 - (y + 0.5) / H * GGB_PI;
+	double lon = (x + 0.5) / W * 2 * GGB_PI - GGB_PI;
+	double sla, cla, slo, clo;
+	sincos(lat, &sla, &cla);
+	sincos(lon, &slo, &clo);
+	dv3 d = d3(cla * slo, sla, cla * clo);
+	int f = face_of(d);
+	double dc, u, v;
+	gnomonic(f, d, dc, u, v);
+	double px = (u + 1.0) * 0.5 * R - 0.5, py = (1.0 - v) * 0.5 * R - 0.5;
+	if (px < 0)
+		px = 0;
+	if (py < 0)
+		py = 0;
+	if (px > R - 1)
+		px = R - 1;
+	if (py > R - 1)
+		py = R - 1;
+	int x0 = (int) px, y0 = (int) py;
+	int x1 = x0 + 1 < R ? x0 + 1 : x0, y1 = y0 + 1 < R ? y0 + 1 : y0;
+	float fx = (float) (px - x0), fy = (float) (py - y0);
+	const float4 *face = buf + (size_t) f * R * R;
+	float4 ta = face[(size_t) y0 * R + x0], tb = face[(size_t) y0 * R + x1];
+	float4 tc = face[(size_t) y1 * R + x0], te = face[(size_t) y1 * R + x1];
+	float a[4] = { ta.x * g, ta.y * g, ta.z * g, ta.w * g }, b[4] = { tb.x * g, tb.y * g, tb.z * g, tb.w * g };
+	float c[4] = { tc.x * g, tc.y * g, tc.z * g, tc.w * g }, e[4] = { te.x * g, te.y * g, te.z * g, te.w * g };
+	float o4[4];
+#pragma unroll
+	for (int k = 0; k < 4; k++)
+		o4[k] = (1 - fy) * ((1 - fx) * a[k] + fx * b[k]) + fy * ((1 - fx) * c[k] + fx * e[k]);
+	unsigned char o[4];
+	pixel8(o4[0], o4[1], o4[2], o4[3], bg, o);
+	if (!bg.on) {
+		((uchar4 *) out)[i] = make_uchar4(o[0], o[1], o[2], o[3]);
+	} else {
+		out[3 * i] = o[0];
+		out[3 * i + 1] = o[1];
+		out[3 * i + 2] = o[2];
+	}
+}
+
+extern "C" void ggbk_render_preview8(const float *buf, int R, float g, int W, int H, const float *bg,
+				     unsigned char *out)
+{
+	size_t npx = (size_t) W * H;
+	unsigned nb = (unsigned) ((npx + 255) / 256);
+	GGBK_LAUNCH(k_render_preview8, nb, 256, 0, (const float4 *) buf, R, g, W, H, make_bg(bg), out);
 }
 
 extern "C" int ggbk_render_tile(void) { return TILE; }

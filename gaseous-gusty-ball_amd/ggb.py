@@ -87,11 +87,12 @@ PUBLIC_SYMBOLS = [
     # noise.h
     "noise_init", "simplex3", "curl_noise",
     # ggb_ext.h
-    "ggb_device_init", "ggb_version", "ggb_sync", "fluid_sync_host", "fluid_sync_device",
+    "ggb_device_init", "ggb_version", "ggb_sync", "ggb_set_sync_mirrors", "fluid_sync_host", "fluid_sync_device",
     "fluid_get_f64", "fluid_set_f64", "fluid_solve_psi_device", "fluid_true_relres",
     "fluid_vcycle_f64", "ggb_laplacian_f64", "ggb_applyK_f64", "ggb_curl_f64",
     "ggb_divergence_f64", "ggb_stencil_compress", "particles_sync_host", "particles_get_f64", "particles_set_f64",
     "particles_set_round32", "render_sync_host", "render_last_entries", "render_last_stats", "render_faces8",
+    "render_net8", "render_preview8",
     "fluid_adjoint_new", "fluid_adjoint_free", "fluid_adjoint_forward", "fluid_adjoint_backward",
     # ggb_inputs.h
     "ggb_initial_velocity", "ggb_banded_strip", "ggb_rgba8_to_float",
@@ -155,6 +156,9 @@ def lib():
             "render_sync_host": (None, [P(CRender)]),
             "render_last_entries": (C.c_longlong, [P(CRender)]),
             "render_faces8": (None, [P(CRender), fp, P(C.c_ubyte)]),
+            "render_net8": (None, [P(CRender), fp, P(C.c_ubyte)]),
+            "render_preview8": (C.c_int, [P(CRender), fp, C.c_int, C.c_int, P(C.c_ubyte)]),
+            "render_sample_dir": (None, [P(CRender), C.c_double, C.c_double, C.c_double, fp]),
             "render_last_stats": (C.c_int, [P(CRender), P(C.c_longlong)]),
             "fluid_adjoint_new": (C.c_void_p, [P(CFluid), C.c_int]),
             "fluid_adjoint_free": (None, [C.c_void_p]),
@@ -163,6 +167,7 @@ def lib():
             "ggb_device_init": (C.c_int, []),
             "ggb_version": (C.c_char_p, []),
             "ggb_sync": (None, []),
+            "ggb_set_sync_mirrors": (None, [C.c_int]),
             "ggb_initial_velocity": (C.c_int, [P(CCubeSphere), C.c_uint, C.c_int, C.c_double,
                                               C.c_double, C.c_double, C.c_double, C.c_double,
                                               C.c_int, fp, fp]),
@@ -529,6 +534,32 @@ class Render:
         out = np.zeros((6, self.R, self.R, ch), np.uint8)
         bgp = None if bg is None else fptr(np.asarray(bg, np.float32))
         lib().render_faces8(C.byref(self.c), bgp, out.ctypes.data_as(C.POINTER(C.c_ubyte)))
+        return out
+
+    def net8(self, bg=None):
+        out = np.zeros((3 * self.R, 4 * self.R, 4), np.uint8)
+        bgp = None if bg is None else fptr(np.asarray(bg, np.float32))
+        lib().render_net8(C.byref(self.c), bgp, out.ctypes.data_as(C.POINTER(C.c_ubyte)))
+        return out
+
+    def preview8(self, W=None, H=None, bg=None):
+        W = 2 * self.R if W is None else W
+        H = self.R if H is None else H
+        out = np.zeros((H, W, 4 if bg is None else 3), np.uint8)
+        bgp = None if bg is None else fptr(np.asarray(bg, np.float32))
+        if lib().render_preview8(C.byref(self.c), bgp, W, H, out.ctypes.data_as(C.POINTER(C.c_ubyte))):
+            raise ValueError("bad preview size")
+        return out
+
+    def sample_dir(self, d):
+        """render_sample_dir (host reader over the lazily refreshed mirror)."""
+        out = np.zeros(4, np.float32)
+        lib().render_sample_dir(C.byref(self.c), float(d[0]), float(d[1]), float(d[2]), fptr(out))
+        return out
+
+    def texel(self, f, x, y):
+        out = np.zeros(4, np.float32)
+        lib().render_texel(C.byref(self.c), int(f), int(x), int(y), fptr(out))
         return out
 
 

@@ -33,6 +33,12 @@ int ggb_device_init(void);
 const char *ggb_version(void);
 /* Wait for all queued device work. */
 void ggb_sync(void);
+/* on != 0: fluid_step and particles_advect also refresh the host mirrors (F.omega,
+ * F.omega_band, F.psi, F.vel; P.pos, P.prev), as fluid_sync_host / particles_sync_host
+ * would, so callers that read them after every step -- the reference's own
+ * tests/test_flow.c -- work unchanged, at the cost of a device-to-host copy per call.
+ * The initial setting is GGB_SYNC_MIRRORS from the environment (1 = on, default off). */
+void ggb_set_sync_mirrors(int on);
 
 /* ---- fluid ---- */
 void fluid_sync_host(Fluid *F);   /* device state -> F.omega/omega_band/psi/vel mirrors */
@@ -80,6 +86,16 @@ int render_last_stats(const Render *r, long long *out16);
  * over bg), texel by texel as render_texel then render_pixel8 (reference render.c:220-226,
  * 253-279), converted on the device; only the 8-bit result is copied back. */
 void render_faces8(Render *r, const float *bg, unsigned char *out);
+/* The 4R x 3R cross net [3R][4R][4] of the reference driver's --net output
+ * (main.c:280-295): faces 0-3 along the middle row, 4 above and 5 below face 0, each
+ * pixel as render_pixel8 of render_texel (alpha 255 when bg is given); the rest of the
+ * net transparent black.  Converted on the device. */
+void render_net8(Render *r, const float *bg, unsigned char *out);
+/* The W x H equirectangular preview of the driver's --preview output (main.c:297-312,
+ * which uses W = 2R, H = R): per pixel render_sample_dir (render.c:228-251) in the
+ * direction of its longitude/latitude centre, then render_pixel8; [H][W][4] when bg is
+ * NULL, else [H][W][3].  Converted on the device; returns -1 for W or H < 1. */
+int render_preview8(Render *r, const float *bg, int W, int H, unsigned char *out);
 
 /* ---- discrete adjoint of fluid_step ----
  * Gradient of a loss L(omega_T) of the final vorticity after nsteps steps of fluid_step
@@ -89,7 +105,9 @@ void render_faces8(Render *r, const float *bg, unsigned char *out);
  * reciprocal stencil.  Checkpoints (omega, psi) every checkpoint_every steps; the backward
  * pass recomputes one segment at a time.  The Poisson solves (forward and adjoint) use
  * F->prm.cg_tol; the gradient is exact for exact solves.  Both calls overwrite the
- * fluid's state; after fluid_adjoint_forward it is the final state. */
+ * fluid's state: after fluid_adjoint_forward it is the rollout's final state, after
+ * fluid_adjoint_backward its initial state (omega0 after the mean removal, its psi, and
+ * the velocity and maximum speed of that psi). */
 typedef struct FluidAdjoint FluidAdjoint;
 FluidAdjoint *fluid_adjoint_new(Fluid *F, int checkpoint_every);   /* NULL on error */
 void fluid_adjoint_free(FluidAdjoint *A);

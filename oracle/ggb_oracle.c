@@ -1371,16 +1371,82 @@ void orc_render_pixel8(const float c[4], const float *bg, unsigned char *out)
 	}
 }
 
-/* 8-bit RGBA faces of the whole buffer (bg = NULL path), [6][R][R][4]. */
-void orc_render_faces8(const ORender *r, unsigned char *out)
+/* 8-bit faces of the whole buffer, texel by texel as render_texel then render_pixel8:
+ * [6][R][R][4] un-premultiplied RGBA when bg is NULL, else [6][R][R][3] composited over
+ * bg (the per-face loop of main.c:258-276). */
+void orc_render_faces8(const ORender *r, const float *bg, unsigned char *out)
 {
-	int R = r->R;
+	int R = r->R, ch = bg ? 3 : 4;
 	#pragma omp parallel for schedule(static)
 	for (int f = 0; f < 6; f++)
 		for (int y = 0; y < R; y++)
 			for (int x = 0; x < R; x++) {
 				float c[4];
 				orc_render_texel(r, f, x, y, c);
-				orc_render_pixel8(c, NULL, out + ((((size_t) f * R + y) * R) + x) * 4);
+				orc_render_pixel8(c, bg, out + ((((size_t) f * R + y) * R) + x) * ch);
 			}
+}
+
+/* render.c:228-251: bilinear sample of the faces in direction d */
+void orc_render_sample_dir(const ORender *r, double dx, double dy, double dz, float out[4])
+{
+	v3d d = mk(dx, dy, dz);
+	int f = face_of(d);
+	double dc = vdot(d, FC[f]);
+	double u = vdot(d, FR[f]) / dc;
+	double v = vdot(d, FU[f]) / dc;
+	int R = r->R;
+	double px = (u + 1.0) * 0.5 * R - 0.5, py = (1.0 - v) * 0.5 * R - 0.5;
+	if (px < 0) px = 0;
+	if (py < 0) py = 0;
+	if (px > R - 1) px = R - 1;
+	if (py > R - 1) py = R - 1;
+	int x0 = (int) px, y0 = (int) py;
+	int x1 = x0 + 1 < R ? x0 + 1 : x0, y1 = y0 + 1 < R ? y0 + 1 : y0;
+	float fx = (float) (px - x0), fy = (float) (py - y0);
+	float a[4], b[4], c[4], e[4];
+	orc_render_texel(r, f, x0, y0, a);
+	orc_render_texel(r, f, x1, y0, b);
+	orc_render_texel(r, f, x0, y1, c);
+	orc_render_texel(r, f, x1, y1, e);
+	for (int k = 0; k < 4; k++)
+		out[k] = (1 - fy) * ((1 - fx) * a[k] + fx * b[k]) + fy * ((1 - fx) * c[k] + fx * e[k]);
+}
+
+/* main.c:280-295: the 4R x 3R cross net, RGBA, cells (0,1) (1,1) (2,1) (3,1) (0,0) (0,2)
+ * for faces 0-5, uncovered pixels transparent black; a face pixel's alpha byte is 255
+ * when bg is given (render_pixel8 leaves it). */
+void orc_render_net8(const ORender *r, const float *bg, unsigned char *out)
+{
+	static const int cell_x[6] = { 0, 1, 2, 3, 0, 0 }, cell_y[6] = { 1, 1, 1, 1, 0, 2 };
+	int R = r->R, W = 4 * R;
+	memset(out, 0, (size_t) W * 3 * R * 4);
+	#pragma omp parallel for schedule(static)
+	for (int fy = 0; fy < 6 * R; fy++) {
+		int f = fy / R, y = fy % R;
+		for (int x = 0; x < R; x++) {
+			float c[4];
+			unsigned char px[4] = { 0, 0, 0, 255 };
+			orc_render_texel(r, f, x, y, c);
+			orc_render_pixel8(c, bg, px);
+			memcpy(out + (((size_t) cell_y[f] * R + y) * W + cell_x[f] * R + x) * 4, px, 4);
+		}
+	}
+}
+
+/* main.c:297-312 with vec3.h:55-58: the equirectangular preview, W x H pixels (the
+ * reference uses W = 2R, H = R), 4 channels without bg, 3 with. */
+void orc_render_preview8(const ORender *r, int W, int H, const float *bg, unsigned char *out)
+{
+	int ch = bg ? 3 : 4;
+	#pragma omp parallel for schedule(static)
+	for (int y = 0; y < H; y++) {
+		double lat = M_PI / 2 - (y + 0.5) / H * M_PI;
+		for (int x = 0; x < W; x++) {
+			double lon = (x + 0.5) / W * 2 * M_PI - M_PI;
+			float c[4];
+			orc_render_sample_dir(r, cos(lat) * sin(lon), sin(lat), cos(lat) * cos(lon), c);
+			orc_render_pixel8(c, bg, out + ((size_t) y * W + x) * ch);
+		}
+	}
 }

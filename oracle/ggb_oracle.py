@@ -107,7 +107,12 @@ def olib(kind):
             "orc_render_fade": (None, [P(ORender), C.c_double]),
             "orc_render_deposit": (None, [P(ORender), C.c_int, rp, rp, fp]),
             "orc_render_count_records": (C.c_longlong, [P(ORender), C.c_int, rp, rp]),
-            "orc_render_faces8": (None, [P(ORender), P(C.c_ubyte)]),
+            "orc_render_faces8": (None, [P(ORender), fp, P(C.c_ubyte)]),
+            "orc_render_texel": (None, [P(ORender), C.c_int, C.c_int, C.c_int, fp]),
+            "orc_render_pixel8": (None, [fp, fp, P(C.c_ubyte)]),
+            "orc_render_sample_dir": (None, [P(ORender), C.c_double, C.c_double, C.c_double, fp]),
+            "orc_render_net8": (None, [P(ORender), fp, P(C.c_ubyte)]),
+            "orc_render_preview8": (None, [P(ORender), C.c_int, C.c_int, fp, P(C.c_ubyte)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -328,9 +333,35 @@ class OracleRender:
         return self.o.L.orc_render_count_records(C.byref(self.r), parts.n, self.o.rp(parts.prev),
                                                  self.o.rp(parts.pos))
 
-    def faces8(self):
-        out = np.zeros((6, self.R, self.R, 4), np.uint8)
-        self.o.L.orc_render_faces8(C.byref(self.r), out.ctypes.data_as(C.POINTER(C.c_ubyte)))
+    def _bg(self, bg):
+        if bg is None:
+            return None, 4
+        self._bgkeep = np.ascontiguousarray(bg, np.float32)
+        return _fp(self._bgkeep), 3
+
+    def faces8(self, bg=None):
+        bgp, ch = self._bg(bg)
+        out = np.zeros((6, self.R, self.R, ch), np.uint8)
+        self.o.L.orc_render_faces8(C.byref(self.r), bgp, out.ctypes.data_as(C.POINTER(C.c_ubyte)))
+        return out
+
+    def sample_dir(self, d):
+        out = np.zeros(4, np.float32)
+        self.o.L.orc_render_sample_dir(C.byref(self.r), float(d[0]), float(d[1]), float(d[2]), _fp(out))
+        return out
+
+    def net8(self, bg=None):
+        bgp, _ = self._bg(bg)
+        out = np.zeros((3 * self.R, 4 * self.R, 4), np.uint8)
+        self.o.L.orc_render_net8(C.byref(self.r), bgp, out.ctypes.data_as(C.POINTER(C.c_ubyte)))
+        return out
+
+    def preview8(self, W=None, H=None, bg=None):
+        W = 2 * self.R if W is None else W
+        H = self.R if H is None else H
+        bgp, ch = self._bg(bg)
+        out = np.zeros((H, W, ch), np.uint8)
+        self.o.L.orc_render_preview8(C.byref(self.r), W, H, bgp, out.ctypes.data_as(C.POINTER(C.c_ubyte)))
         return out
 
 
@@ -426,6 +457,9 @@ def rlib(kind="ref"):
             "render_free": (None, [P(RRender)]),
             "render_fade": (None, [P(RRender), C.c_double]),
             "render_deposit": (None, [P(RRender), P(RParticles)]),
+            "render_texel": (None, [P(RRender), C.c_int, C.c_int, C.c_int, fp]),
+            "render_sample_dir": (None, [P(RRender), C.c_double, C.c_double, C.c_double, fp]),
+            "render_pixel8": (None, [fp, fp, P(C.c_ubyte)]),
             "noise_init": (None, [P(RNoise), C.c_uint]),
             "simplex3": (C.c_double, [P(RNoise), RVec3, P(RVec3)]),
             "curl_noise": (RVec3, [P(RNoise), RVec3, P(RCurlParams)]),
@@ -574,6 +608,25 @@ class RefRender:
 
     def deposit(self, parts):
         self.L.render_deposit(C.byref(self.c), C.byref(parts.c))
+
+    def texel(self, f, x, y):
+        out = np.zeros(4, np.float32)
+        self.L.render_texel(C.byref(self.c), int(f), int(x), int(y), _fp(out))
+        return out
+
+    def sample_dir(self, d):
+        out = np.zeros(4, np.float32)
+        self.L.render_sample_dir(C.byref(self.c), float(d[0]), float(d[1]), float(d[2]), _fp(out))
+        return out
+
+    def pixel8(self, c, bg=None):
+        """render_pixel8 of one float texel: 4 bytes without bg, 3 with (the reference
+        writes only those)."""
+        c = np.ascontiguousarray(c, np.float32)
+        out = np.zeros(4, np.uint8)
+        bgk = None if bg is None else np.ascontiguousarray(bg, np.float32)
+        self.L.render_pixel8(_fp(c), None if bgk is None else _fp(bgk), out.ctypes.data_as(C.POINTER(C.c_ubyte)))
+        return out if bg is None else out[:3]
 
 
 class RefNoise:

@@ -162,3 +162,38 @@ def test_torch_harness_gradient_and_descent(ggb, orc, gpu):
         losses.append(Ln)
     print("losses", losses)
     assert losses[-1] < 0.9 * losses[0]
+
+
+def test_torch_harness_two_rollouts_one_context(ggb, orc, gpu):
+    """Two rollouts on one context before their backward passes: each backward returns
+    its own rollout's gradient (the library keeps the last forward's checkpoints only),
+    and after a backward the fluid holds the rollout's initial state with its
+    velocity."""
+    torch = pytest.importorskip("torch")
+    import adjoint
+    levels, g, om0, omb, target, rng = case(ggb, orc, 16, seed=12)
+    dts = np.full(4, 0.6 * g.h)
+    F = gpu_fluid(ggb, g, omb)
+    rc = adjoint.RolloutContext(F, dts, checkpoint_every=2)
+    area = torch.from_numpy(g.area.astype(np.float64))
+    tgt = torch.from_numpy(target)
+    k = torch.tensor(0.5, dtype=torch.float64)
+    wa = torch.from_numpy(om0.copy()).requires_grad_(True)
+    wb = torch.from_numpy(0.7 * om0[::-1].copy()).requires_grad_(True)
+    La = adjoint.area_loss(adjoint.FluidRollout.apply(wa, k, rc), tgt, area)
+    Lb = adjoint.area_loss(adjoint.FluidRollout.apply(wb, k, rc), tgt, area)
+    (La + Lb).backward()
+    for w0, grad in ((wa, wa.grad), (wb, wb.grad)):
+        A = ggb.FluidAdjoint(F, checkpoint_every=4)
+        F.set_forcing(0.5)
+        w = w0.detach().numpy().copy()
+        wT = A.forward(w, dts)
+        g0, _ = A.backward(g.area * (wT - target))
+        assert rel(grad.numpy(), g0) < 1e-12
+        # post-backward state: omega0 after the mean removal, and its velocity
+        om = F.get(ggb.FIELD_OMEGA)
+        ex = w - np.sum(g.area * w) / g.total_area
+        assert rel(om, ex) < 1e-12
+        vel = F.get(ggb.FIELD_VEL).reshape(-1, 3)
+        assert abs(F.max_speed() - np.max(np.linalg.norm(vel, axis=1))) <= 1e-12 * F.max_speed()
+        A.free()

@@ -332,3 +332,129 @@ def test_planted_mistake_is_caught(ggb, orc):
     rms = lambda y: np.sqrt(np.sum((y - ex) ** 2 * g.area) / g.total_area)
     assert rms(good) < 0.05
     assert rms(bad) > 0.05
+
+
+# ---------------------------------------------------------------- output path (render.c:55-66, 220-279;
+# main.c:249-318) against the compiled reference
+
+def _deposited_pair(ggb, orc, R=40, fade=0.3, steps=4, radius=1.5, alpha=0.5):
+    """Oracle (f32) and reference renders after the same particle history."""
+    g = ggb.Grid(16)
+    rg = orc.RefGrid(16)
+    vel0, _ = g.initial_velocity(seed=4, octaves=2)
+    O = orc.OracleFluid("f32", levels_for(ggb, orc, 16))
+    O.set_state(vel0)
+    P = orc.OracleParticles("f32", 2500)
+    RP = orc.RefParticles(2500)
+    img = ggb.strip_to_float(ggb.banded_strip(6))
+    P.color_from_image(img, 64, 1024)
+    RP.color_from_image(img, 64, 1024)
+    Ro = orc.OracleRender("f32", R, radius, alpha)
+    Rr = orc.RefRender(R, radius, alpha)
+    gs = []
+    for _ in range(steps):
+        dt = g.h / max(O.max_speed(), 1e-6)
+        v = O.get(3).astype(np.float32)
+        P.advect(g, v, dt)
+        RP.advect(rg, v, dt)
+        O.step(dt)
+        Ro.fade(fade)
+        Rr.fade(fade)
+        gs.append(Rr.g)
+        Ro.deposit(P)
+        Rr.deposit(RP)
+    return Ro, Rr, gs
+
+
+def test_f32_fade_renormalisation_matches_compiled_reference(ggb, orc, ref):
+    """fade 0.9 over five deposits takes g below 1e-4 (render.c:57): the buffer is scaled
+    by (float) g and g reset to 1 -- the same bits as the reference, at every step."""
+    Ro, Rr, gs = _deposited_pair(ggb, orc, fade=0.9, steps=5)
+    assert 1.0 in gs, "the renormalisation branch did not run: %s" % gs
+    assert np.array_equal(bits(Ro.buf), bits(Rr.buf))
+    assert Ro.g == Rr.g
+    assert np.count_nonzero(Ro.buf[..., 3]) > 100
+
+
+def test_f32_output_readers_match_compiled_reference(ggb, orc, ref):
+    """render_texel + render_pixel8 over every texel, with and without a background, and
+    render_sample_dir at random directions, face centres, edges and corners: the oracle's
+    bytes and floats equal the reference's."""
+    Ro, Rr, _ = _deposited_pair(ggb, orc, fade=0.2, steps=3)
+    R = Ro.R
+    bg = np.array([0.25, 0.5, 0.75], np.float32)
+    f8, f8bg = Ro.faces8(), Ro.faces8(bg)
+    assert np.count_nonzero(f8[..., 3]) > 100
+    for f in range(6):
+        for y in range(R):
+            for x in range(R):
+                c = Rr.texel(f, x, y)
+                assert np.array_equal(f8[f, y, x], Rr.pixel8(c)), (f, x, y)
+                assert np.array_equal(f8bg[f, y, x], Rr.pixel8(c, bg)), (f, x, y)
+    rng = np.random.default_rng(5)
+    dirs = list(rng.normal(size=(3000, 3)))
+    for a in (0.0, 1e-9, 0.3, 1.0 - 1e-12, 1.0):  # up to the face edges and corners
+        for s in ((1, a, 0.2), (a, 1, -a), (-a, 0.4, 1), (1, 1, a), (1, -1, 1)):
+            for sign in (1, -1):
+                dirs.append(sign * np.array(s, float))
+    for d in dirs:
+        assert np.array_equal(bits(Ro.sample_dir(d)), bits(Rr.sample_dir(d))), d
+
+
+def _png(path):
+    from PIL import Image
+    return np.asarray(Image.open(path))
+
+
+def _run_reference_program(tmp, args):
+    import subprocess
+    exe = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "ggbref_main")
+    if not os.path.exists(exe):
+        pytest.skip("reference program (oracle/_ref/ggbref_main) not built")
+    subprocess.run([exe] + args, check=True, cwd=tmp, capture_output=True, timeout=300)
+
+
+def _oracle_run_like_main(ggb, orc, img8, N, M, R, steps, seed, forcing, rotation):
+    """The reference driver's set-up and step loop (main.c:383-434) on the f32 oracle."""
+    levels = levels_for(ggb, orc, N)
+    g = levels[0]
+    vel0, vband = g.initial_velocity(seed=seed)
+    O = orc.OracleFluid("f32", levels, forcing=forcing, rotation=rotation, bfecc=1, cg_tol=1e-4, cg_maxit=500)
+    O.set_state(vel0, vband if forcing > 0 else None)
+    P = orc.OracleParticles("f32", M)
+    h, w = img8.shape[:2]
+    P.color_from_image(ggb.strip_to_float(img8), w, h)
+    Ro = orc.OracleRender("f32", R, 1.0, 0.35)
+    for _ in range(steps):
+        vmax = O.max_speed()
+        dt = 1.0 * g.h / (vmax if vmax > 1e-6 else 1e-6)
+        P.advect(g, O.get(3).astype(np.float32), dt)
+        O.step(dt)
+        Ro.fade(0.02)
+        Ro.deposit(P)
+    return Ro
+
+
+def test_f32_outputs_match_reference_program(ggb, orc, ref, tmp_path):
+    """The reference program itself (main.c + image_io.c + core, one thread) writes its
+    six faces, cross net and equirectangular preview; the oracle run through the same
+    set-up and steps gives the same PNG bytes, with a transparent and an opaque
+    background (main.c:249-318, render.c:220-279)."""
+    pytest.importorskip("PIL")
+    from PIL import Image
+    img8 = ggb.banded_strip(3)
+    Image.fromarray(img8, "RGBA").save(str(tmp_path / "in.png"))
+    N, M, R, steps, seed = 16, 3000, 48, 6, 3
+    Ro = _oracle_run_like_main(ggb, orc, img8, N, M, R, steps, seed, 0.5, 2.0)
+    common = ["-i", "in.png", "--grid", str(N), "--particles", str(M), "--res", str(R), "--steps", str(steps),
+              "--every", str(steps), "--seed", str(seed), "--forcing", "0.5", "--rotation", "2", "--threads", "1",
+              "--net", "--preview"]
+    for bgarg, bg in (("transparent", None), ("4080c0", np.array([0x40, 0x80, 0xc0], np.float32) / np.float32(255))):
+        out = "out_" + bgarg
+        _run_reference_program(str(tmp_path), common + ["-o", out, "--bg", bgarg])
+        faces = Ro.faces8(bg)
+        assert np.count_nonzero(faces) > 0
+        for f in range(6):
+            assert np.array_equal(_png(tmp_path / out / ("gg%d.png" % f)), faces[f]), (bgarg, f)
+        assert np.array_equal(_png(tmp_path / out / "gg-net.png"), Ro.net8(bg)), bgarg
+        assert np.array_equal(_png(tmp_path / out / "gg-preview.png"), Ro.preview8(bg=bg)), bgarg
