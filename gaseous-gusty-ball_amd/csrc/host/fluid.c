@@ -310,6 +310,7 @@ struct ggb_fluid_dev {
 	ggbk_red red;
 	int nb0;                /* blocks of a level-0 pass */
 	int predict;            /* iterations the previous solve needed */
+	int fix_its;            /* >= 0: the next solves run exactly this many iterations */
 	int vmax_valid;         /* s->vmax2 matches vel */
 	/* last-synced copies of the host mirrors (detects caller writes) */
 	float *shadow_omega, *shadow_band, *shadow_psi, *shadow_vel;
@@ -760,6 +761,7 @@ int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
 	D->shadow_psi = falloc(n);
 	D->shadow_vel = falloc(3 * n);
 	D->predict = 1;
+	D->fix_its = -1;
 	if (ggbk_check("fluid_init"))
 		ggb_die("fluid_init: %s", ggbk_last_error());
 	live_add(F);
@@ -977,8 +979,15 @@ static int solve_on(Fluid *F, void *omega, void *psi, int maxit, int sub_mean)
 {
 	struct ggb_fluid_dev *D = F->dev;
 	ggbk_grid g = D->g0;
+	double tol = F->prm.cg_tol;
+	if (D->fix_its >= 0) {
+		/* a prescribed count: no residual is small enough to stop earlier */
+		tol = 0.0;
+		maxit = D->fix_its;
+		D->predict = maxit > 0 ? maxit : 1;
+	}
 	ggbk_cg_start(g, omega, psi, D->rbuf[0], D->T, sub_mean, D->s, D->red);
-	ggbk_fin(GGBK_FIN_START, D->s, D->red, D->nb0, F->prm.cg_tol, maxit, 0);
+	ggbk_fin(GGBK_FIN_START, D->s, D->red, D->nb0, tol, maxit, 0);
 	precondition(D, D->rbuf[0], 1);
 	/* Iterations are enqueued in batches and the host reads the loop flag after each; the
 	 * first batch is the previous solve's count, later batches grow geometrically.  Passes
@@ -1276,6 +1285,9 @@ struct FluidAdjoint {
 	double *lam, *lq, *lq1, *lq2, *ldb, *ldf, *lvel, *lpsi, *om, *z, *dk;
 	unsigned char *rslot;  /* device [maxnb][n]: slot of cell j in the list of its m-th neighbour */
 	double sum_area;
+	int *its;              /* host [nsteps + 1]: CG iterations of the last forward (set-up, steps) */
+	int *fix;              /* host [nfix] or NULL: prescribed iteration counts */
+	int nfix;
 };
 
 FluidAdjoint *fluid_adjoint_new(Fluid *F, int checkpoint_every)
@@ -1337,6 +1349,8 @@ void fluid_adjoint_free(FluidAdjoint *A)
 	for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); k++)
 		ggbk_free(bufs[k]);
 	free(A->dts);
+	free(A->its);
+	free(A->fix);
 	free(A);
 }
 
@@ -1367,6 +1381,15 @@ int fluid_adjoint_forward(FluidAdjoint *A, const double *omega0, const double *d
 		return -1;
 	memcpy(A->dts, dts, sizeof(double) * (size_t) nsteps);
 	A->nsteps = nsteps;
+	free(A->its);
+	A->its = malloc(sizeof(int) * (size_t) (nsteps + 1));
+	if (!A->its)
+		return -1;
+	if (A->fix && A->nfix != nsteps + 1) {
+		fprintf(stderr, "fluid_adjoint_forward: %d prescribed iteration counts for %d solves\n", A->nfix,
+			nsteps + 1);
+		return -1;
+	}
 	/* set-up as fluid_set_state: omega - mean, cold solve, psi mean, velocity */
 	upload(D->omega, omega0, 8 * n);
 	ggbk_remove_mean(D->g0, D->omega, D->T, D->s, D->red);
@@ -1374,14 +1397,19 @@ int fluid_adjoint_forward(FluidAdjoint *A, const double *omega0, const double *d
 	int save = F->prm.cg_maxit;
 	int cold = save * 20 > 20000 ? save * 20 : 20000;
 	D->predict = 1;
+	D->fix_its = A->fix ? A->fix[0] : -1;
 	solve_device(F, cold, 0);
+	A->its[0] = F->cg_iters;
 	D->predict = 1;
 	psi_mean_and_velocity(F);
 	for (int s = 0; s < nsteps; s++) {
 		if (s % A->K == 0)
 			adj_save(D, A->ckpt + 2 * n * (size_t) (s / A->K));
+		D->fix_its = A->fix ? A->fix[s + 1] : -1;
 		fluid_step(F, dts[s]);
+		A->its[s + 1] = F->cg_iters;
 	}
+	D->fix_its = -1;
 	if (omega_T)
 		download(omega_T, D->omega, 8 * n);
 	return ggbk_check("fluid_adjoint_forward") ? -1 : 0;
@@ -1429,13 +1457,45 @@ static void adj_step(FluidAdjoint *A, double dt, const double *om_s, const doubl
 	ggbk_adj_combine(g, A->lq, A->z, A->lam);
 }
 
+int fluid_adjoint_iterations(const FluidAdjoint *A, int *its)
+{
+	if (!A->its)
+		return -1;
+	if (its)
+		memcpy(its, A->its, sizeof(int) * (size_t) (A->nsteps + 1));
+	return A->nsteps + 1;
+}
+
+int fluid_adjoint_fix_iterations(FluidAdjoint *A, const int *its, int count)
+{
+	free(A->fix);
+	A->fix = NULL;
+	A->nfix = 0;
+	if (!its)
+		return 0;
+	if (count < 1)
+		return -1;
+	A->fix = malloc(sizeof(int) * (size_t) count);
+	if (!A->fix)
+		return -1;
+	for (int k = 0; k < count; k++)
+		if (its[k] < 0) {
+			free(A->fix);
+			A->fix = NULL;
+			return -1;
+		}
+	memcpy(A->fix, its, sizeof(int) * (size_t) count);
+	A->nfix = count;
+	return 0;
+}
+
 int fluid_adjoint_backward(FluidAdjoint *A, const double *g_omega_T, double *g_omega0, double *g_forcing)
 {
 	Fluid *F = A->F;
 	struct ggb_fluid_dev *D = F->dev;
 	ggbk_grid g = D->g0;
 	size_t n = (size_t) D->n;
-	if (!A->dts)
+	if (!A->dts || !A->its)
 		return -1;
 	upload(A->lam, g_omega_T, 8 * n);
 	ggbk_zero(A->dk, sizeof(double));
@@ -1450,9 +1510,14 @@ int fluid_adjoint_backward(FluidAdjoint *A, const double *g_omega_T, double *g_o
 		velocity_device(F);
 		for (int s = s0; s < s1; s++) {
 			adj_save(D, A->tape + 2 * n * (size_t) (s - s0));
-			if (s + 1 < s1)
+			if (s + 1 < s1) {
+				/* the forward's own iteration counts (from the same checkpoint the
+				 * solves repeat bit for bit; with a prescribed schedule this keeps it) */
+				D->fix_its = A->its[s + 1];
 				fluid_step(F, A->dts[s]);
+			}
 		}
+		D->fix_its = -1;
 		for (int s = s1 - 1; s >= s0; s--) {
 			const double *t = A->tape + 2 * n * (size_t) (s - s0);
 			adj_step(A, A->dts[s], t, t + n);

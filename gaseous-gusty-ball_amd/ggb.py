@@ -94,6 +94,7 @@ PUBLIC_SYMBOLS = [
     "particles_set_round32", "render_sync_host", "render_last_entries", "render_last_stats", "render_faces8",
     "render_net8", "render_preview8",
     "fluid_adjoint_new", "fluid_adjoint_free", "fluid_adjoint_forward", "fluid_adjoint_backward",
+    "fluid_adjoint_iterations", "fluid_adjoint_fix_iterations",
     # ggb_inputs.h
     "ggb_initial_velocity", "ggb_banded_strip", "ggb_rgba8_to_float",
 ]
@@ -164,6 +165,8 @@ def lib():
             "fluid_adjoint_free": (None, [C.c_void_p]),
             "fluid_adjoint_forward": (C.c_int, [C.c_void_p, dp, dp, C.c_int, dp]),
             "fluid_adjoint_backward": (C.c_int, [C.c_void_p, dp, dp, dp]),
+            "fluid_adjoint_iterations": (C.c_int, [C.c_void_p, ip]),
+            "fluid_adjoint_fix_iterations": (C.c_int, [C.c_void_p, ip, C.c_int]),
             "ggb_device_init": (C.c_int, []),
             "ggb_version": (C.c_char_p, []),
             "ggb_sync": (None, []),
@@ -439,6 +442,25 @@ class FluidAdjoint:
         if lib().fluid_adjoint_backward(self.h, dptr(g), dptr(g0), C.byref(gk)) != 0:
             raise RuntimeError("fluid_adjoint_backward failed")
         return g0, gk.value
+
+    def iterations(self):
+        """CG iteration counts of the last forward: [set-up solve, step 0, step 1, ...]."""
+        n = lib().fluid_adjoint_iterations(self.h, None)
+        if n < 0:
+            raise RuntimeError("no forward pass yet")
+        its = np.zeros(n, np.int32)
+        lib().fluid_adjoint_iterations(self.h, iptr(its))
+        return its
+
+    def fix_iterations(self, its):
+        """Prescribe every solve's iteration count for later forwards (None: tolerance)."""
+        if its is None:
+            rc = lib().fluid_adjoint_fix_iterations(self.h, None, 0)
+        else:
+            a = np.ascontiguousarray(its, np.int32)
+            rc = lib().fluid_adjoint_fix_iterations(self.h, iptr(a), len(a))
+        if rc != 0:
+            raise ValueError("bad iteration schedule")
 
 
 class Particles:

@@ -116,6 +116,16 @@ int fluid_adjoint_forward(FluidAdjoint *A, const double *omega0, const double *d
 			  double *omega_T);
 /* g_omega_T = dL/d omega_T [n] in; g_omega0 [n] and *g_forcing out; 0 on success */
 int fluid_adjoint_backward(FluidAdjoint *A, const double *g_omega_T, double *g_omega0, double *g_forcing);
+/* CG iteration counts of the last fluid_adjoint_forward: its[0] of the set-up solve,
+ * its[1 + s] of step s.  Returns the count (nsteps + 1), or -1 before any forward pass;
+ * its may be NULL to query the count. */
+int fluid_adjoint_iterations(const FluidAdjoint *A, int *its);
+/* its != NULL: later forward passes run each of their count = nsteps + 1 Poisson solves
+ * for exactly its[k] iterations instead of stopping at the tolerance, so the rollout is a
+ * smooth map of omega0 for finite-difference checks (a tolerance test stops neighbouring
+ * inputs after different counts: a jump of the solve's tolerance in the output).  NULL
+ * restores the tolerance test.  0 on success, -1 for a negative count. */
+int fluid_adjoint_fix_iterations(FluidAdjoint *A, const int *its, int count);
 
 #ifdef __cplusplus
 }

@@ -197,3 +197,34 @@ def test_torch_harness_two_rollouts_one_context(ggb, orc, gpu):
         vel = F.get(ggb.FIELD_VEL).reshape(-1, 3)
         assert abs(F.max_speed() - np.max(np.linalg.norm(vel, axis=1))) <= 1e-12 * F.max_speed()
         A.free()
+
+
+def test_adjoint_iteration_schedule(ggb, orc, gpu):
+    """fluid_adjoint_iterations reports each solve's CG count; prescribing the same counts
+    reproduces the rollout bit for bit, and the backward pass replays them (the same
+    gradient)."""
+    levels, g, om0, omb, target, rng = case(ggb, orc, 16, seed=13)
+    dts = np.full(6, 0.6 * g.h)
+    F = gpu_fluid(ggb, g, omb)
+    A = ggb.FluidAdjoint(F, checkpoint_every=3)
+    w1 = A.forward(om0, dts)
+    its = A.iterations()
+    assert len(its) == 7 and np.all(its > 0)
+    g1, k1 = A.backward(g.area * (w1 - target))
+    A.fix_iterations(its)
+    w2 = A.forward(om0, dts)
+    assert np.array_equal(A.iterations(), its)
+    assert np.array_equal(w1.view(np.uint64), w2.view(np.uint64))
+    g2, k2 = A.backward(g.area * (w2 - target))
+    # the transpose scatters accumulate with atomics: equal to rounding, not bitwise
+    assert rel(g2, g1) < 1e-12 and abs(k2 - k1) <= 1e-12 * abs(k1)
+    # two more iterations everywhere: a slightly different (better converged) rollout
+    A.fix_iterations(its + 2)
+    w3 = A.forward(om0, dts)
+    assert np.array_equal(A.iterations(), its + 2)
+    assert 0 < rel(w3, w1) < 1e-8
+    with pytest.raises(RuntimeError):
+        A.fix_iterations(its[:3])
+        A.forward(om0, dts)
+    A.fix_iterations(None)
+    A.free()

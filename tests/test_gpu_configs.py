@@ -12,7 +12,8 @@
     texels within 1 LSB on >= 99.9 %, CG iteration counts equal);
   * N=128 adjoint (50 FP64 steps, checkpoints every 10, loss against the 50-step
     forward run of another noise seed): the gradient against the oracle's FP64 tape
-    adjoint, and against central differences (h = 1e-6) on 64 random cells;
+    adjoint, and against central differences (h = 1e-6) on 64 random cells: their
+    directional derivative to 1e-6, each cell to the FP64 noise floor of the rollout;
   * N=1024 (6,291,456 cells, 16M particles, faces 4096^2, mixed precision, forcing 0.2,
     rotation 5, 6-octave noise): a few steps through size-independent properties, and a
     short adjoint window on the same state.
@@ -151,24 +152,41 @@ def test_c3_adjoint(ggb, orc, gpu):
     assert rel(wT, wTo) < 1e-8
     assert rel(g0, g0o) < 1e-6
     assert abs(gk - gko) <= 1e-6 * abs(gko)
-    # central differences (h = 1e-6) on 64 random cells.  The numerator L(w+) - L(w-) is
+    # Central differences (h = 1e-6) on 64 random cells.  The numerator L(w+) - L(w-) is
     # formed as 0.5 sum A (w+ - w-)(w+ + w- - 2 target), so no two large loss values are
-    # subtracted (the rounding of L itself, 1e-16 L / h, would otherwise dominate).
+    # subtracted.  What remains is the forward map's own FP64 rounding: the Poisson solve
+    # spreads a one-cell perturbation over every cell, each rounded differently in the two
+    # rollouts, which leaves about 1e-12 (1e-15 L) of noise in the numerator whatever h
+    # is -- measured per-cell errors 6.5e-6 / 5.7e-5 / 6.0e-4 at h = 1e-5 / 1e-6 / 1e-7,
+    # with every solve taking the same iteration count in all rollouts.  So:
+    #   * the directional derivative over the 64 cells (d = sign(g) on them, 64 times one
+    #     cell's signal) is checked at h = 1e-6 to 1e-6 relative;
+    #   * each cell's derivative is checked to the noise floor, 1e-4 relative over the 64.
     rng = np.random.default_rng(7)
     cells = rng.choice(g.n, 64, replace=False)
     h = 1e-6
+
+    def fd(d):
+        wp = A.forward(om0 + h * d, dts)
+        wm = A.forward(om0 - h * d, dts)
+        return 0.5 * float(np.sum(area * (wp - wm) * (wp + wm - 2 * target))) / (2 * h)
+
+    d = np.zeros(g.n)
+    d[cells] = np.sign(g0[cells])
+    fdd, ex = fd(d), float(g0 @ d)
+    print("directional FD over 64 cells: %.12e vs adjoint %.12e, relative error %.2e" %
+          (fdd, ex, abs(fdd - ex) / abs(ex)))
+    assert abs(fdd - ex) <= 1e-6 * abs(ex)
     fds = []
     for c in cells:
         e = np.zeros(g.n)
-        e[c] = h
-        wp = A.forward(om0 + e, dts)
-        wm = A.forward(om0 - e, dts)
-        fds.append(0.5 * float(np.sum(area * (wp - wm) * (wp + wm - 2 * target))) / (2 * h))
+        e[c] = 1.0
+        fds.append(fd(e))
     fds = np.array(fds)
     r64 = rel(fds, g0[cells])
     print("per-cell FD on 64 cells: relative error %.2e (max abs %.2e, L = %.3e)" %
           (r64, np.max(np.abs(fds - g0[cells])), loss(wT)))
-    assert r64 <= 1e-6
+    assert r64 <= 1e-4
     A.free()
     F.free()
 
