@@ -299,6 +299,12 @@ struct ggb_fluid_dev {
 	ggbk_grid g0;
 	int nlev;
 	int dense_level;        /* level whose sub-V-cycle is a dense matrix, or -1 */
+	int cheb;               /* mixed precision: the tiled Chebyshev V-cycle (ggbk_vcycle_cheb) */
+	/* mixed precision: the solve of a step starts from psi extrapolated linearly in time */
+	double *psi_prev;       /* psi of the previous step (FP64), or NULL */
+	int prev_valid;         /* psi_prev and dt_prev hold the previous step */
+	double dt_prev;
+	int no_extrap;          /* set while the adjoint replays steps */
 	CubeSphere *coarse;     /* [nlev]; [0] unused */
 	ggbk_level *levels;     /* [nlev] */
 	HostLevel *hl;          /* [nlev] */
@@ -563,18 +569,45 @@ static void h_smooth(const HostLevel *L, const double *b, double *x, double *r, 
 	}
 }
 
-/* x = V_l(b); bufs[l] holds 3 scratch arrays of level l */
-static void h_vcycle(const HostLevel *hl, int nlev, int l, const double *b, double *x, double **bufs)
+/* deg Chebyshev steps for D^-1 K on [1/2, 2] from x (the mixed-precision smoother of
+ * mgt_k.hip, whose first step is the damped Jacobi step); d is scratch */
+static void h_cheb(const HostLevel *L, const double *b, double *x, double *r, double *d, int deg)
+{
+	int n = L->cs->ncells;
+	double sigma = 5.0 / 3.0, delta = 0.75, rho = 1.0 / sigma;
+	for (int k = 0; k < deg; k++) {
+		double cd = 0.0, cr = 0.8;
+		if (k > 0) {
+			double rn = 1.0 / (2 * sigma - rho);
+			cd = rn * rho;
+			cr = 2 * rn / delta;
+			rho = rn;
+		}
+		for (int i = 0; i < n; i++)
+			r[i] = b[i] - hK(L->cs, x, i);
+		for (int i = 0; i < n; i++) {
+			d[i] = (k ? cd * d[i] : 0.0) + cr * L->dinv[i] * r[i];
+			x[i] += d[i];
+		}
+	}
+}
+
+/* x = V_l(b); bufs[l] holds 4 scratch arrays of level l.  deg 1: the reference's damped
+ * Jacobi V(1,1) cycle; deg > 1: Chebyshev smoothing of that degree (the mixed mode's) */
+static void h_vcycle(const HostLevel *hl, int nlev, int l, const double *b, double *x, double **bufs, int deg)
 {
 	const HostLevel *L = &hl[l];
 	int n = L->cs->ncells;
-	double *r = bufs[l];
+	double *r = bufs[l], *d = bufs[l] + 3 * (size_t) n;
 	memset(x, 0, sizeof(double) * n);
 	if (l == nlev - 1) {
 		h_smooth(L, b, x, r, 60);
 		return;
 	}
-	h_smooth(L, b, x, r, 1);
+	if (deg > 1)
+		h_cheb(L, b, x, r, d, deg);
+	else
+		h_smooth(L, b, x, r, 1);
 	for (int i = 0; i < n; i++)
 		r[i] = b[i] - hK(L->cs, x, i);
 	const HostLevel *C = &hl[l + 1];
@@ -586,19 +619,24 @@ static void h_vcycle(const HostLevel *hl, int nlev, int l, const double *b, doub
 			acc += L->rw[k] * r[L->ridx[k]];
 		bc[c] = acc;
 	}
-	h_vcycle(hl, nlev, l + 1, bc, xc, bufs);
+	h_vcycle(hl, nlev, l + 1, bc, xc, bufs, deg);
 	for (int i = 0; i < n; i++) {
 		const int *t = L->ptap + 4 * i;
 		const float *w = L->pw + 4 * i;
 		x[i] += 1.0 * (w[0] * xc[t[0]] + w[1] * xc[t[1]] + w[2] * xc[t[2]] + w[3] * xc[t[3]]);
 	}
-	h_smooth(L, b, x, r, 1);
+	if (deg > 1)
+		h_cheb(L, b, x, r, d, deg);
+	else
+		h_smooth(L, b, x, r, 1);
 }
 
 #define DENSE_MAX 1536
 
-/* Tabulate the sub-V-cycle of level L as M[i][j] = (V_L e_j)_i and upload it. */
-static void build_dense(struct ggb_fluid_dev *D, int L)
+/* Tabulate the sub-V-cycle of level L as M[i][j] = (V_L e_j)_i and upload it: the
+ * reference's V(1,1) cycle (deg 1), or the mixed precision's Chebyshev cycle (deg 3, as the
+ * levels above it smooth). */
+static void build_dense(struct ggb_fluid_dev *D, int L, int deg)
 {
 	int n = D->hl[L].cs->ncells;
 	double *M = malloc(sizeof(double) * (size_t) n * n);
@@ -608,13 +646,13 @@ static void build_dense(struct ggb_fluid_dev *D, int L)
 	{
 		double **bufs = calloc((size_t) D->nlev, sizeof(double *));
 		for (int l = L; l < D->nlev; l++)
-			bufs[l] = calloc(3 * (size_t) D->hl[l].cs->ncells, sizeof(double));
+			bufs[l] = calloc(4 * (size_t) D->hl[l].cs->ncells, sizeof(double));
 		double *e = calloc((size_t) n, sizeof(double)), *x = calloc((size_t) n, sizeof(double));
 		#pragma omp for schedule(dynamic, 8)
 		for (int j = 0; j < n; j++) {
 			memset(e, 0, sizeof(double) * n);
 			e[j] = 1.0;
-			h_vcycle(D->hl, D->nlev, L, e, x, bufs);
+			h_vcycle(D->hl, D->nlev, L, e, x, bufs, deg);
 			for (int i = 0; i < n; i++)
 				M[(size_t) i * n + j] = x[i];
 		}
@@ -642,6 +680,187 @@ static void build_dense(struct ggb_fluid_dev *D, int L)
 	free(M);
 	D->levels[L].M = d;
 	D->dense_level = L;
+}
+
+/* ---- face tiles of a level for the fused Chebyshev passes (mgt_k.hip) ---- */
+
+#define MGT_TILE 16
+#define MGT_CAP 512 /* ext cells a workgroup of the tiled passes holds */
+
+static int cell_is_regular(const CubeSphere *cs, int i)
+{
+	for (int k = 0; k < cs->nnb[i]; k++)
+		if (offset_code(cs->nbr[(size_t) CS_MAXNB * i + k] - i, cs->N) < 0)
+			return 0;
+	return 1;
+}
+
+/* Tiles of 16x16 cells with `deg` rings.  A tile whose rings lie on its face with regular
+ * stencils on rings < deg is interior (no tables); any other gets graph tables: its ext
+ * cells ring by ring in BFS order (ring r = the unvisited stencil neighbours of ring r - 1,
+ * in discovery order) and each stencil neighbour of rings < deg as a slot of that list.
+ * Returns -1 when a graph tile would exceed MGT_CAP ext cells. */
+static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, ggbk_tiles *out)
+{
+	int N = cs->N, n = cs->ncells, tpe = N / MGT_TILE, ntiles = 6 * tpe * tpe;
+	unsigned char *reg = malloc((size_t) n);
+	int *kind = malloc(sizeof(int) * (size_t) ntiles);
+	int *stamp = malloc(sizeof(int) * (size_t) n), *loc = malloc(sizeof(int) * (size_t) n);
+	size_t ecap = (size_t) MGT_CAP * 64, ne = 0;
+	int *gcell = malloc(sizeof(int) * ecap);
+	unsigned short *gnbr = malloc(sizeof(unsigned short) * 8 * ecap);
+	int *goff = malloc(sizeof(int) * ((size_t) ntiles + 1));
+	int *gring = malloc(sizeof(int) * (size_t) ntiles * (deg + 1));
+	if (!reg || !kind || !stamp || !loc || !gcell || !gnbr || !goff || !gring)
+		ggb_die("fluid: out of memory");
+	#pragma omp parallel for schedule(static)
+	for (int i = 0; i < n; i++) {
+		reg[i] = (unsigned char) cell_is_regular(cs, i);
+		stamp[i] = -1;
+	}
+	/* GGB_TILES_GRAPH=1: every tile through the graph tables (both forms give the same bits) */
+	const char *ge = getenv("GGB_TILES_GRAPH");
+	int all_graph = ge && !strcmp(ge, "1");
+	int ngraph = 0, cap = 0, rc = 0;
+	for (int t = 0; t < ntiles && rc == 0; t++) {
+		int f = t / (tpe * tpe), r = t - f * tpe * tpe;
+		int J0 = (r / tpe) * MGT_TILE, I0 = (r % tpe) * MGT_TILE;
+		int interior = !all_graph && I0 >= deg && J0 >= deg && I0 + MGT_TILE - 1 + deg <= N - 1 &&
+			       J0 + MGT_TILE - 1 + deg <= N - 1;
+		for (int y = J0 - (deg - 1); interior && y <= J0 + MGT_TILE - 1 + (deg - 1); y++)
+			for (int x = I0 - (deg - 1); x <= I0 + MGT_TILE - 1 + (deg - 1); x++)
+				if (!reg[(f * N + y) * N + x]) {
+					interior = 0;
+					break;
+				}
+		if (interior) {
+			kind[t] = -1;
+			continue;
+		}
+		kind[t] = ngraph;
+		goff[ngraph] = (int) ne;
+		size_t base = ne;
+		if (ne + MGT_CAP > ecap) {
+			ecap = 2 * ecap + MGT_CAP;
+			gcell = realloc(gcell, sizeof(int) * ecap);
+			gnbr = realloc(gnbr, sizeof(unsigned short) * 8 * ecap);
+			if (!gcell || !gnbr)
+				ggb_die("fluid: out of memory");
+		}
+		int cnt = 0;
+		for (int y = 0; y < MGT_TILE; y++)
+			for (int x = 0; x < MGT_TILE; x++) {
+				int c = (f * N + J0 + y) * N + I0 + x;
+				stamp[c] = t;
+				loc[c] = cnt;
+				gcell[base + cnt++] = c;
+			}
+		int *re = gring + (size_t) ngraph * (deg + 1);
+		re[0] = cnt;
+		int rs = 0;
+		for (int ring = 1; ring <= deg && rc == 0; ring++) {
+			int rend = cnt;
+			for (int e = rs; e < rend && rc == 0; e++) {
+				int c = gcell[base + e];
+				for (int k = 0; k < cs->nnb[c]; k++) {
+					int j = cs->nbr[(size_t) CS_MAXNB * c + k];
+					if (stamp[j] == t)
+						continue;
+					if (cnt == MGT_CAP) {
+						rc = -1;
+						break;
+					}
+					stamp[j] = t;
+					loc[j] = cnt;
+					gcell[base + cnt++] = j;
+				}
+			}
+			rs = rend;
+			re[ring] = cnt;
+		}
+		for (int e = 0; e < re[deg - 1] && rc == 0; e++) {
+			int c = gcell[base + e];
+			for (int k = 0; k < 8; k++) {
+				int slot = e;
+				if (k < cs->nnb[c]) {
+					int j = cs->nbr[(size_t) CS_MAXNB * c + k];
+					if (stamp[j] != t)
+						ggb_die("fluid: tile %d ring cell %d has a neighbour outside its rings", t, c);
+					slot = loc[j];
+				}
+				gnbr[8 * (base + e) + k] = (unsigned short) slot;
+			}
+		}
+		for (int e = re[deg - 1]; e < cnt; e++)
+			for (int k = 0; k < 8; k++)
+				gnbr[8 * (base + e) + k] = (unsigned short) e;
+		ne += (size_t) cnt;
+		if (cnt > cap)
+			cap = cnt;
+		ngraph++;
+	}
+	goff[ngraph] = (int) ne;
+	if (rc == 0) {
+		memset(out, 0, sizeof(*out));
+		out->ntiles = ntiles;
+		out->deg = deg;
+		out->cap = cap;
+		out->ngraph = ngraph;
+		void *d = xbuf(D, sizeof(int) * (size_t) ntiles, "multigrid tiles");
+		upload(d, kind, sizeof(int) * (size_t) ntiles);
+		out->kind = d;
+		d = xbuf(D, sizeof(int) * ((size_t) ngraph + 1), "multigrid tiles");
+		upload(d, goff, sizeof(int) * ((size_t) ngraph + 1));
+		out->goff = d;
+		d = xbuf(D, sizeof(int) * ((size_t) ngraph * (deg + 1) + 1), "multigrid tiles");
+		upload(d, gring, sizeof(int) * (size_t) ngraph * (deg + 1));
+		out->gring = d;
+		d = xbuf(D, sizeof(int) * (ne + 1), "multigrid tiles");
+		upload(d, gcell, sizeof(int) * ne);
+		out->gcell = d;
+		d = xbuf(D, sizeof(unsigned short) * 8 * (ne + 1), "multigrid tiles");
+		upload(d, gnbr, sizeof(unsigned short) * 8 * ne);
+		out->gnbr = d;
+	}
+	free(reg);
+	free(kind);
+	free(stamp);
+	free(loc);
+	free(gcell);
+	free(gnbr);
+	free(goff);
+	free(gring);
+	return rc;
+}
+
+/* The mixed-precision preconditioner (degree 2 on level 0, 3 below) needs a dense level
+ * below level 0 and every level above it a multiple of 16 cells wide; GGB_MIXED_CHEB=0 in
+ * the environment selects the reference V(1,1) cycle instead.  setup_cheb builds the
+ * tiles of those levels. */
+static int cheb_eligible(const struct ggb_fluid_dev *D, const CubeSphere *cs)
+{
+	const char *e = getenv("GGB_MIXED_CHEB");
+	if (e && !strcmp(e, "0"))
+		return 0;
+	int L = D->dense_level;
+	if (L < 1)
+		return 0;
+	for (int l = 0; l < L; l++) {
+		const CubeSphere *c = l == 0 ? cs : &D->coarse[l];
+		if (c->N % MGT_TILE)
+			return 0;
+	}
+	return 1;
+}
+
+static void setup_cheb(struct ggb_fluid_dev *D, const CubeSphere *cs)
+{
+	for (int l = 0; l < D->dense_level; l++) {
+		const CubeSphere *c = l == 0 ? cs : &D->coarse[l];
+		if (build_tiles(D, c, l == 0 ? 2 : 3, &D->levels[l].tl))
+			ggb_die("fluid_init: a multigrid tile of level %d exceeds %d cells", l, MGT_CAP);
+		D->levels[l].xpre = xbuf(D, sizeof(float) * (size_t) c->ncells, "multigrid");
+	}
 }
 
 int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
@@ -751,10 +970,18 @@ int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
 		for (int l = 1; l < D->nlev && L < 0; l++)
 			if (D->coarse[l].ncells <= DENSE_MAX)
 				L = l;
-		if (L > 0 && L < D->nlev)
-			build_dense(D, L);
+		D->dense_level = L > 0 && L < D->nlev ? L : -1;
+		int cheb = D->prec == GGB_PREC_MIXED && cheb_eligible(D, cs);
+		if (D->dense_level > 0)
+			build_dense(D, L, cheb ? 3 : 1);
 		for (int l = 0; l < D->nlev; l++)
 			host_level_free(&D->hl[l]);
+		if (cheb) {
+			setup_cheb(D, cs);
+			D->cheb = 1;
+		}
+		if (D->prec == GGB_PREC_MIXED)
+			D->psi_prev = xbuf(D, sizeof(double) * n, "fluid");
 	}
 	D->shadow_omega = falloc(n);
 	D->shadow_band = falloc(n);
@@ -829,6 +1056,7 @@ static void mirrors_pull(Fluid *F)
 {
 	struct ggb_fluid_dev *D = F->dev;
 	size_t n = D->n;
+	D->prev_valid = 0;
 	if (memcmp(F->omega, D->shadow_omega, 4 * n)) {
 		put_f32(D, D->omega, F->omega, n);
 		memcpy(D->shadow_omega, F->omega, 4 * n);
@@ -866,6 +1094,7 @@ void fluid_sync_device(Fluid *F)
 {
 	struct ggb_fluid_dev *D = F->dev;
 	size_t n = D->n;
+	D->prev_valid = 0;
 	put_f32(D, D->omega, F->omega, n);
 	put_f32(D, D->omega_band, F->omega_band, n);
 	put_f32(D, D->psi, F->psi, n);
@@ -918,6 +1147,7 @@ int fluid_set_f64(Fluid *F, int field, const double *in)
 		return -1;
 	if (field == GGB_FIELD_VEL)
 		D->vmax_valid = 0;
+	D->prev_valid = 0;
 	if (D->T == GGBK_F64) {
 		upload(dst, in, 8 * cnt);
 		return 0;
@@ -937,7 +1167,9 @@ static int mean_op(const struct ggb_fluid_dev *D) { return D->T == GGBK_F64 ? GG
 /* z = M^-1 r and rz (fluid.c:148-160, 315-318, 344-348) */
 static void precondition(struct ggb_fluid_dev *D, const void *r, int first)
 {
-	if (D->nlev > 0)
+	if (D->cheb)
+		ggbk_vcycle_cheb(D->levels, D->nlev, NULL, r, NULL, D->z, D->s, D->red);
+	else if (D->nlev > 0)
 		ggbk_vcycle(D->levels, D->nlev, r, D->T, D->z, D->TM, D->s, D->red);
 	else
 		ggbk_jacobi_precond(D->g0, r, D->z, D->T, D->s, D->red);
@@ -955,6 +1187,11 @@ static void cg_iteration(struct ggb_fluid_dev *D, void *psi, int k)
 	ggbk_cg_Ap(g, D->z, D->TZ, pin, pout, D->Ap, D->T, k == 0, D->s, D->red);
 	ggbk_fin(GGBK_FIN_PAP, D->s, D->red, D->nb0, 0, 0, 0);
 	ggbk_cg_fuse u = { psi, rin, pout, D->Ap };
+	if (D->cheb) {
+		ggbk_vcycle_cheb(D->levels, D->nlev, &u, NULL, rout, D->z, D->s, D->red);
+		ggbk_fin(GGBK_FIN_RZ, D->s, D->red, D->nb0, 0, 0, 0);
+		return;
+	}
 	if (ggbk_vcycle_update(D->levels, D->nlev, &u, rout, D->T, D->z, D->TM, D->s, D->red) == 0) {
 		ggbk_fin(GGBK_FIN_RZ, D->s, D->red, D->nb0, 0, 0, 0);
 		return;
@@ -1039,6 +1276,7 @@ static void velocity_device(Fluid *F)
 int fluid_solve_psi_device(Fluid *F)
 {
 	struct ggb_fluid_dev *D = F->dev;
+	D->prev_valid = 0;
 	int it = solve_device(F, F->prm.cg_maxit, 0);
 	ggbk_remove_mean(D->g0, D->psi, D->T, D->s, D->red);
 	if (ggbk_check("solve"))
@@ -1066,6 +1304,7 @@ void fluid_compute_velocity(Fluid *F)
 void fluid_set_state(Fluid *F, const float *vel_initial, const float *vel_band)
 {
 	struct ggb_fluid_dev *D = F->dev;
+	D->prev_valid = 0;
 	ggbk_grid g = D->g0;
 	size_t n = D->n;
 	void *tmp = ggb_dmalloc(3 * D->tsz * n, "set_state");
@@ -1109,6 +1348,13 @@ void fluid_step(Fluid *F, double dt)
 	ggbk_limited_relax(g, D->q2, D->q0, D->dep_b, D->q1, bfecc, D->omega_band, D->omega, T, twoOmega, relax,
 			   D->red);
 	ggbk_fin(mean_op(D), D->s, D->red, D->nb0, 0, 0, g.total_area);
+	if (D->psi_prev && !D->no_extrap) {
+		/* psi(t + dt) ~ psi(t) + dt / dt_prev (psi(t) - psi(t - dt_prev)) */
+		double c = D->prev_valid ? dt / D->dt_prev : 0.0;
+		ggbk_extrapolate(D->n, (double *) D->psi, D->psi_prev, c);
+		D->dt_prev = dt;
+		D->prev_valid = 1;
+	}
 	solve_device(F, F->prm.cg_maxit, 1);
 	psi_mean_and_velocity(F);
 	if (ggbk_check("fluid_step"))
@@ -1174,7 +1420,10 @@ int fluid_vcycle_f64(Fluid *F, const double *b, double *x)
 	ggbk_scalars *ds = ggb_dmalloc(sizeof(ggbk_scalars), "vcycle");
 	upload(ds, &hs, sizeof(hs));
 	ggbk_red none = { NULL, 0 };
-	ggbk_vcycle(D->levels, D->nlev, db, D->T, dz, D->TM, ds, none);
+	if (D->cheb)
+		ggbk_vcycle_cheb(D->levels, D->nlev, NULL, db, NULL, dz, ds, none);
+	else
+		ggbk_vcycle(D->levels, D->nlev, db, D->T, dz, D->TM, ds, none);
 	ggbk_convert(tmp, GGBK_F64, dz, D->TM, n);
 	download(x, tmp, 8 * n);
 	ggbk_free(db);
@@ -1402,6 +1651,9 @@ int fluid_adjoint_forward(FluidAdjoint *A, const double *omega0, const double *d
 	A->its[0] = F->cg_iters;
 	D->predict = 1;
 	psi_mean_and_velocity(F);
+	/* the solves start from the previous psi itself: a recomputed segment then repeats
+	 * the forward pass bit for bit from its checkpoint */
+	D->no_extrap = 1;
 	for (int s = 0; s < nsteps; s++) {
 		if (s % A->K == 0)
 			adj_save(D, A->ckpt + 2 * n * (size_t) (s / A->K));
@@ -1410,6 +1662,8 @@ int fluid_adjoint_forward(FluidAdjoint *A, const double *omega0, const double *d
 		A->its[s + 1] = F->cg_iters;
 	}
 	D->fix_its = -1;
+	D->no_extrap = 0;
+	D->prev_valid = 0;
 	if (omega_T)
 		download(omega_T, D->omega, 8 * n);
 	return ggbk_check("fluid_adjoint_forward") ? -1 : 0;
@@ -1508,6 +1762,7 @@ int fluid_adjoint_backward(FluidAdjoint *A, const double *g_omega_T, double *g_o
 		ggbk_d2d(D->omega, ck, 8 * n);
 		ggbk_d2d(D->psi, ck + n, 8 * n);
 		velocity_device(F);
+		D->no_extrap = 1;
 		for (int s = s0; s < s1; s++) {
 			adj_save(D, A->tape + 2 * n * (size_t) (s - s0));
 			if (s + 1 < s1) {
@@ -1518,6 +1773,7 @@ int fluid_adjoint_backward(FluidAdjoint *A, const double *g_omega_T, double *g_o
 			}
 		}
 		D->fix_its = -1;
+		D->no_extrap = 0;
 		for (int s = s1 - 1; s >= s0; s--) {
 			const double *t = A->tape + 2 * n * (size_t) (s - s0);
 			adj_step(A, A->dts[s], t, t + n);
@@ -1529,6 +1785,7 @@ int fluid_adjoint_backward(FluidAdjoint *A, const double *g_omega_T, double *g_o
 	ggbk_d2d(D->omega, A->ckpt, 8 * n);
 	ggbk_d2d(D->psi, A->ckpt + n, 8 * n);
 	velocity_device(F);
+	D->prev_valid = 0;
 	/* the control passes through set-up's mean removal */
 	ggbk_adj_sum(g, A->lam, D->red);
 	ggbk_fin(GGBK_FIN_MEAN_F64, D->s, D->red, D->nb0, 0, 0, g.total_area);

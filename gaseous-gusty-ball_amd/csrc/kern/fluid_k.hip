@@ -646,6 +646,23 @@ extern "C" void ggbk_jacobi_precond(ggbk_grid g, const void *r, void *z, int T, 
 		GGBK_LAUNCH(k_jacobi_precond<float>, NB(g.n), BS, 0, g, (const float *) r, (float *) z, s, red);
 }
 
+// Linear extrapolation in time of the stream function, the warm start of the next solve:
+// psi_prev <- psi, psi <- psi + c (psi - psi_prev)
+__global__ __launch_bounds__(BS) void k_extrapolate(int n, double *psi, double *psi_prev, double c)
+{
+	int i = xblock() * BS + threadIdx.x;
+	if (i >= n)
+		return;
+	double a = psi[i], b = psi_prev[i];
+	psi_prev[i] = a;
+	psi[i] = a + c * (a - b);
+}
+
+extern "C" void ggbk_extrapolate(int n, double *psi, double *psi_prev, double c)
+{
+	GGBK_LAUNCH(k_extrapolate, NB(n), BS, 0, n, psi, psi_prev, c);
+}
+
 extern "C" void ggbk_cg_zero_if_bb0(ggbk_grid g, void *psi, int T, const ggbk_scalars *s)
 {
 	if (T == GGBK_F64)

@@ -165,6 +165,8 @@ void ggbk_cg_update(ggbk_grid g, void *psi, void *r, const void *p, const void *
 		    const ggbk_scalars *s, ggbk_red red);
 /* Jacobi preconditioner z = dinv r and the r.z partial (no multigrid) */
 void ggbk_jacobi_precond(ggbk_grid g, const void *r, void *z, int T, const ggbk_scalars *s, ggbk_red red);
+/* warm start by linear extrapolation in time: psi_prev <- psi, psi <- psi + c (psi - psi_prev) */
+void ggbk_extrapolate(int n, double *psi, double *psi_prev, double c);
 /* bb == 0 exit: psi = 0 when status says so */
 void ggbk_cg_zero_if_bb0(ggbk_grid g, void *psi, int T, const ggbk_scalars *s);
 
@@ -173,11 +175,28 @@ void ggbk_cg_zero_if_bb0(ggbk_grid g, void *psi, int T, const ggbk_scalars *s);
  * levels[l].x/b/r are the per-level TM scratch; level 0's x and b are unused.  A level
  * with M != NULL applies the precomputed matrix of the V-cycle below it (x = M b;
  * M is [n][n] row-major, double when TM is double else float). */
+/* 16x16 face tiles of a level for the fused multi-sweep smoothing passes (mixed precision):
+ * a workgroup owns a tile (ring 0) and recomputes its `deg` surrounding rings of cells, so
+ * `deg` Chebyshev steps need no other synchronisation than the workgroup's own barriers.
+ * An interior tile (kind -1) has its rings on its own face with regular stencils, found by
+ * index arithmetic; a tile near a face seam has graph tables: its ring cells in BFS order
+ * over the Voronoi stencil, and each stencil neighbour as a slot of that list. */
+typedef struct {
+	int ntiles, deg, cap, ngraph;   /* cap: largest ext cell count of a graph tile */
+	const int *kind;                /* [ntiles] -1 interior, else graph index */
+	const int *goff;                /* [ngraph + 1] first ext cell of each graph tile */
+	const int *gring;               /* [ngraph][deg + 1] ext cells in rings 0..r (cumulative) */
+	const int *gcell;               /* [ext] cell index of each ext cell, ring by ring */
+	const unsigned short *gnbr;     /* [ext][8] slot of each stencil neighbour, self-padded */
+} ggbk_tiles;
+
 typedef struct {
 	ggbk_grid g;
 	ggbk_xfer xf;     /* transfer to the next coarser level (unused on the coarsest) */
 	void *x, *b, *r;  /* TM scratch */
 	const void *M;    /* dense sub-V-cycle operator, or NULL */
+	ggbk_tiles tl;    /* multi-sweep tiles (tl.ntiles == 0: none) */
+	void *xpre;       /* TM: the pre-smoothed x of the tiled passes */
 } ggbk_level;
 /* z = V(r); with red.part != NULL also the r.z partial (slot RZ) of ggbk_nblocks(n0) blocks */
 void ggbk_vcycle(const ggbk_level *levels, int nlev, const void *rin, int T, void *zout, int TM,
@@ -191,6 +210,14 @@ typedef struct {
 } ggbk_cg_fuse;
 int ggbk_vcycle_update(const ggbk_level *levels, int nlev, const ggbk_cg_fuse *u, void *r_out, int T,
 		       void *zout, int TM, ggbk_scalars *s, ggbk_red red);
+/* The mixed-precision preconditioner: a V-cycle with Chebyshev smoothing of degree
+ * levels[l].tl.deg (1/2 <= lambda(D^-1 K) <= 2, the Gershgorin bound) on the tiled levels,
+ * float storage and arithmetic, down to the first level with a dense operator.  r (FP64)
+ * in, z (float) out with the r.z partial; with u != NULL the PCG update is fused into the
+ * first pass as in ggbk_vcycle_update (r_out written, r.r partial, fold enqueued).
+ * Needs levels[0..L-1] tiled and levels[L].M (L >= 1). */
+void ggbk_vcycle_cheb(const ggbk_level *levels, int nlev, const ggbk_cg_fuse *u, const void *r, void *r_out,
+		      void *zout, ggbk_scalars *s, ggbk_red red);
 
 /* ---- velocity (reference fluid.c:360-376) + max speed (fluid.c:480-491) ---- */
 /* [psi_out = psi_in - mean] vel = p x grad(psi_out); vmax partial (slot VMAX) */

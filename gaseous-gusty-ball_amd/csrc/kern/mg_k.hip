@@ -534,3 +534,17 @@ extern "C" int ggbk_vcycle_update(const ggbk_level *levels, int nlev, const ggbk
 		vcycle_level<float, float>(levels, nlev, 0, (const float *) r_out, (float *) zout, s, red, u);
 	return 0;
 }
+
+// float restriction and dense bottom for the mixed-precision V-cycle (mgt_k.hip)
+extern "C" void ggbk_mg_restrict_f32(ggbk_xfer xf, const float *rf, float *bc, const ggbk_scalars *s)
+{
+	if (xf.rpad == 18)
+		GGBK_LAUNCH((k_mg_restrict<float, 18>), NB(xf.nc), BS, 0, xf, rf, bc, s);
+	else
+		GGBK_LAUNCH((k_mg_restrict<float, GGBK_RMAX>), NB(xf.nc), BS, 0, xf, rf, bc, s);
+}
+
+extern "C" void ggbk_mg_dense_f32(const float *M, int n, const float *b, float *x, const ggbk_scalars *s)
+{
+	GGBK_LAUNCH((k_mg_dense<float, float, float>), (unsigned) n, DENSE_BS, 0, M, n, b, x, s);
+}

@@ -1,0 +1,79 @@
+"""Restatement of the mixed-precision preconditioner in float64 numpy -- test
+infrastructure: the checker of the GPU's Chebyshev V-cycle (csrc/kern/mgt_k.hip).
+
+The V-cycle follows the reference's (fluid.c:113-146: zero start, pre-smooth, residual,
+restriction R = P^T, coarse correction, prolongation, post-smooth, 60 damped Jacobi
+sweeps on the coarsest level) with one change the mixed mode makes: the smoother is the
+Chebyshev iteration on D^-1 K for the interval [1/2, 2], degree 2 on level 0 and 3 below
+(the library tabulates the cycle below its first level of at most 1536 cells as a dense
+matrix; that changes only the rounding).  With degree 1 everywhere the Chebyshev step is
+the damped Jacobi step, so the restatement is then the reference V-cycle, which the CPU
+tests check against the oracle's.
+"""
+import numpy as np
+import scipy.sparse as sp
+
+
+def kmat(g):
+    """K x = sum_k w_k (x_i - x_nb(k)) as a sparse matrix (float weights, float64 sums)."""
+    n = g.n
+    nbr = g.nbr.reshape(n, 8)
+    w = g.w.reshape(n, 8).astype(np.float64)
+    rows, cols, vals = [], [], []
+    for k in range(8):
+        i = np.nonzero(k < g.nnb)[0]
+        rows += [i, i]
+        cols += [i, nbr[i, k]]
+        vals += [w[i, k], -w[i, k]]
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+
+
+def cheb_coefficients(deg):
+    """(cd_k, cr_k): d_k = cd_k d_{k-1} + cr_k D^-1 r_k for [1/2, 2] (sigma 5/3, delta 3/4)."""
+    sigma, delta = 5.0 / 3.0, 0.75
+    rho = 1.0 / sigma
+    out = [(0.0, 1.0 / 1.25)]
+    for _ in range(1, deg):
+        rn = 1.0 / (2 * sigma - rho)
+        out.append((rn * rho, 2 * rn / delta))
+        rho = rn
+    return out
+
+
+class MixedVcycle:
+    def __init__(self, levels, transfers):
+        """levels: ggb.Grid per level; transfers[l] = (taps [nf, 4], weights [nf, 4])."""
+        self.levels = levels
+        self.K = [kmat(g) for g in levels]
+        self.dinv = [np.float32(1.0 / K.diagonal()).astype(np.float64) for K in self.K]
+        self.P = []
+        for l, (pt, pw) in enumerate(transfers):
+            nf, nc = levels[l].n, levels[l + 1].n
+            self.P.append(sp.csr_matrix((np.asarray(pw, np.float64).reshape(nf, 4).ravel(),
+                                         (np.repeat(np.arange(nf), 4), np.asarray(pt).reshape(nf, 4).ravel())),
+                                        shape=(nf, nc)))
+        self.R = [P.T.tocsr() for P in self.P]
+
+    def smooth(self, l, b, x, deg):
+        """deg Chebyshev steps from x on K x = b."""
+        d = None
+        for cd, cr in cheb_coefficients(deg):
+            r = self.dinv[l] * (b - self.K[l] @ x)
+            d = cr * r if d is None else cd * d + cr * r
+            x = x + d
+        return x
+
+    def jacobi(self, l, b, x, sweeps):
+        for _ in range(sweeps):
+            x = x + 0.8 * self.dinv[l] * (b - self.K[l] @ x)
+        return x
+
+    def __call__(self, b, l=0, degrees=None):
+        """z = M b; degrees(l) gives the smoothing degree (default 2 on level 0, 3 below)."""
+        deg = (2 if l == 0 else 3) if degrees is None else degrees(l)
+        n = self.levels[l].n
+        if l == len(self.levels) - 1:
+            return self.jacobi(l, b, np.zeros(n), 60)
+        x = self.smooth(l, b, np.zeros(n), deg)
+        x = x + self.P[l] @ self(self.R[l] @ (b - self.K[l] @ x), l + 1, degrees)
+        return self.smooth(l, b, x, deg)

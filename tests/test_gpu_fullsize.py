@@ -96,4 +96,4 @@ def test_fullsize_mixed_matches_fp64(ggb, grid):
         err = np.linalg.norm(a - b) / np.linalg.norm(a)
         print("field %d: mixed vs FP64 relative difference %.2e" % (k, err))
         assert err < 1e-6
-    assert res[1][3] <= 1.5e-8 and res[0][3] <= 1.5e-8
+    assert res[1][3] <= 1e-8 and res[0][3] <= 1e-8
