@@ -496,7 +496,8 @@ static ggbk_xfer build_transfer(struct ggb_fluid_dev *D, const CubeSphere *fine,
 	x.Nc = coarse->N;
 	/* computed indices pay off where the tables stream from HBM; the small levels' tables
 	 * stay in L2, and there the index arithmetic costs more than the loads it saves */
-	x.analytic = nc >= 65536 && transfer_is_analytic(fine, coarse, ptap, rstart, ridx, rw, pw);
+	x.rana = transfer_is_analytic(fine, coarse, ptap, rstart, ridx, rw, pw);
+	x.analytic = nc >= 65536 && x.rana;
 	void *d;
 	d = xbuf(D, sizeof(int) * 8 * (size_t) nf, "transfer");
 	upload(d, pt, sizeof(int) * 8 * (size_t) nf);
@@ -504,7 +505,7 @@ static ggbk_xfer build_transfer(struct ggb_fluid_dev *D, const CubeSphere *fine,
 	d = xbuf(D, sizeof(float) * 4 * (size_t) nf, "transfer");
 	upload(d, pw, sizeof(float) * 4 * (size_t) nf);
 	x.pw = d;
-	if (x.analytic) {
+	if (x.rana) {
 		/* the weights of the face-interior rows alone, coalesced per entry group */
 		float *w16 = calloc(16 * (size_t) nc, sizeof(float));
 		if (!w16)
@@ -685,7 +686,6 @@ static void build_dense(struct ggb_fluid_dev *D, int L, int deg)
 /* ---- face tiles of a level for the fused Chebyshev passes (mgt_k.hip) ---- */
 
 #define MGT_TILE 16
-#define MGT_CAP 512 /* ext cells a workgroup of the tiled passes holds */
 
 static int cell_is_regular(const CubeSphere *cs, int i)
 {
@@ -699,19 +699,28 @@ static int cell_is_regular(const CubeSphere *cs, int i)
  * stencils on rings < deg is interior (no tables); any other gets graph tables: its ext
  * cells ring by ring in BFS order (ring r = the unvisited stencil neighbours of ring r - 1,
  * in discovery order) and each stencil neighbour of rings < deg as a slot of that list.
- * Returns -1 when a graph tile would exceed MGT_CAP ext cells. */
-static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, ggbk_tiles *out)
+ * Returns -1 when a graph tile would exceed capmax ext cells. */
+static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, const HostLevel *rl, int rana,
+		       int capmax, ggbk_tiles *out)
 {
+	/* rl != NULL: tiles of a down pass, whose restriction through rl's R into the next
+	 * level is fused; an interior tile then has deg + 1 rings (the rows of R reach one ring
+	 * beyond the fine tile) and the bilinear row pattern (rana) */
+	int hs = rl ? deg + 1 : deg;
 	int N = cs->N, n = cs->ncells, tpe = N / MGT_TILE, ntiles = 6 * tpe * tpe;
 	unsigned char *reg = malloc((size_t) n);
 	int *kind = malloc(sizeof(int) * (size_t) ntiles);
+	int *tinfo = calloc((size_t) ntiles * 8, sizeof(int));
 	int *stamp = malloc(sizeof(int) * (size_t) n), *loc = malloc(sizeof(int) * (size_t) n);
-	size_t ecap = (size_t) MGT_CAP * 64, ne = 0;
+	size_t ecap = (size_t) capmax * 64, ne = 0;
 	int *gcell = malloc(sizeof(int) * ecap);
 	unsigned short *gnbr = malloc(sizeof(unsigned short) * 8 * ecap);
 	int *goff = malloc(sizeof(int) * ((size_t) ntiles + 1));
 	int *gring = malloc(sizeof(int) * (size_t) ntiles * (deg + 1));
-	if (!reg || !kind || !stamp || !loc || !gcell || !gnbr || !goff || !gring)
+	size_t rcap = rl ? (size_t) 64 * GGBK_TROW * 64 : 1, nr = 0;
+	unsigned short *grow = malloc(sizeof(unsigned short) * rcap);
+	float *grw = malloc(sizeof(float) * rcap);
+	if (!reg || !kind || !tinfo || !stamp || !loc || !gcell || !gnbr || !goff || !gring || !grow || !grw)
 		ggb_die("fluid: out of memory");
 	#pragma omp parallel for schedule(static)
 	for (int i = 0; i < n; i++) {
@@ -725,10 +734,10 @@ static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, g
 	for (int t = 0; t < ntiles && rc == 0; t++) {
 		int f = t / (tpe * tpe), r = t - f * tpe * tpe;
 		int J0 = (r / tpe) * MGT_TILE, I0 = (r % tpe) * MGT_TILE;
-		int interior = !all_graph && I0 >= deg && J0 >= deg && I0 + MGT_TILE - 1 + deg <= N - 1 &&
-			       J0 + MGT_TILE - 1 + deg <= N - 1;
-		for (int y = J0 - (deg - 1); interior && y <= J0 + MGT_TILE - 1 + (deg - 1); y++)
-			for (int x = I0 - (deg - 1); x <= I0 + MGT_TILE - 1 + (deg - 1); x++)
+		int interior = !all_graph && (!rl || rana) && I0 >= hs && J0 >= hs && I0 + MGT_TILE - 1 + hs <= N - 1 &&
+			       J0 + MGT_TILE - 1 + hs <= N - 1;
+		for (int y = J0 - (hs - 1); interior && y <= J0 + MGT_TILE - 1 + (hs - 1); y++)
+			for (int x = I0 - (hs - 1); x <= I0 + MGT_TILE - 1 + (hs - 1); x++)
 				if (!reg[(f * N + y) * N + x]) {
 					interior = 0;
 					break;
@@ -740,8 +749,8 @@ static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, g
 		kind[t] = ngraph;
 		goff[ngraph] = (int) ne;
 		size_t base = ne;
-		if (ne + MGT_CAP > ecap) {
-			ecap = 2 * ecap + MGT_CAP;
+		if (ne + capmax > ecap) {
+			ecap = 2 * ecap + capmax;
 			gcell = realloc(gcell, sizeof(int) * ecap);
 			gnbr = realloc(gnbr, sizeof(unsigned short) * 8 * ecap);
 			if (!gcell || !gnbr)
@@ -755,6 +764,35 @@ static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, g
 				loc[c] = cnt;
 				gcell[base + cnt++] = c;
 			}
+		if (rl) {
+			/* ring 0 also holds the rest of the support of the 8x8 coarse rows */
+			int Nc = N / 2;
+			if (nr + (size_t) 64 * GGBK_TROW > rcap) {
+				rcap = 2 * rcap + (size_t) 64 * GGBK_TROW;
+				grow = realloc(grow, sizeof(unsigned short) * rcap);
+				grw = realloc(grw, sizeof(float) * rcap);
+				if (!grow || !grw)
+					ggb_die("fluid: out of memory");
+			}
+			for (int q = 0; q < 64 && rc == 0; q++) {
+				int cc = (f * Nc + J0 / 2 + q / 8) * Nc + I0 / 2 + q % 8;
+				int len = rl->rstart[cc + 1] - rl->rstart[cc];
+				if (len > GGBK_TROW)
+					ggb_die("fluid: a restriction row of %d entries exceeds %d", len, GGBK_TROW);
+				for (int k = rl->rstart[cc]; k < rl->rstart[cc + 1]; k++) {
+					int j = rl->ridx[k];
+					if (stamp[j] == t)
+						continue;
+					if (cnt == capmax) {
+						rc = -1;
+						break;
+					}
+					stamp[j] = t;
+					loc[j] = cnt;
+					gcell[base + cnt++] = j;
+				}
+			}
+		}
 		int *re = gring + (size_t) ngraph * (deg + 1);
 		re[0] = cnt;
 		int rs = 0;
@@ -766,7 +804,7 @@ static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, g
 					int j = cs->nbr[(size_t) CS_MAXNB * c + k];
 					if (stamp[j] == t)
 						continue;
-					if (cnt == MGT_CAP) {
+					if (cnt == capmax) {
 						rc = -1;
 						break;
 					}
@@ -794,6 +832,23 @@ static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, g
 		for (int e = re[deg - 1]; e < cnt; e++)
 			for (int k = 0; k < 8; k++)
 				gnbr[8 * (base + e) + k] = (unsigned short) e;
+		if (rl && rc == 0) {
+			int Nc = N / 2;
+			for (int q = 0; q < 64; q++) {
+				int cc = (f * Nc + J0 / 2 + q / 8) * Nc + I0 / 2 + q % 8;
+				size_t o = nr + (size_t) q * GGBK_TROW;
+				int e = 0;
+				for (int k = rl->rstart[cc]; k < rl->rstart[cc + 1]; k++, e++) {
+					grow[o + e] = (unsigned short) loc[rl->ridx[k]];
+					grw[o + e] = rl->rw[k];
+				}
+				for (; e < GGBK_TROW; e++) {
+					grow[o + e] = 0;
+					grw[o + e] = 0.0f;
+				}
+			}
+			nr += (size_t) 64 * GGBK_TROW;
+		}
 		ne += (size_t) cnt;
 		if (cnt > cap)
 			cap = cnt;
@@ -801,35 +856,74 @@ static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, g
 	}
 	goff[ngraph] = (int) ne;
 	if (rc == 0) {
+		/* per tile: kind, first ext cell, ext count, ring ends; whether the interior tiles
+		 * are exactly the geometrically interior ones */
+		int geo = 1;
+		for (int t = 0; t < ntiles; t++) {
+			int f = t / (tpe * tpe), r = t - f * tpe * tpe;
+			int J0 = (r / tpe) * MGT_TILE, I0 = (r % tpe) * MGT_TILE;
+			int gi = I0 >= hs && J0 >= hs && I0 + MGT_TILE - 1 + hs <= N - 1 && J0 + MGT_TILE - 1 + hs <= N - 1;
+			int *ti = tinfo + 8 * (size_t) t;
+			ti[0] = kind[t];
+			if (kind[t] >= 0) {
+				int k = kind[t];
+				ti[1] = goff[k];
+				ti[2] = goff[k + 1] - goff[k];
+				for (int q = 0; q <= deg; q++)
+					ti[3 + q] = gring[(size_t) k * (deg + 1) + q];
+			}
+			if (gi != (kind[t] < 0))
+				geo = 0;
+		}
+		/* the weight class of each ext cell, from the device stencil records */
+		ggbk_grid gg = ggb_grid_dev(cs);
+		int *srec = malloc(sizeof(int) * 2 * (size_t) n);
+		int *gcc = malloc(sizeof(int) * 2 * (ne + 1));
+		if (!gcc || !srec)
+			ggb_die("fluid: out of memory");
+		download(srec, gg.srec, sizeof(int) * 2 * (size_t) n);
+		for (size_t e = 0; e < ne; e++) {
+			gcc[2 * e] = gcell[e];
+			gcc[2 * e + 1] = srec[2 * (size_t) gcell[e]] & 0x7fffffff;
+		}
+		free(srec);
 		memset(out, 0, sizeof(*out));
 		out->ntiles = ntiles;
 		out->deg = deg;
 		out->cap = cap;
 		out->ngraph = ngraph;
-		void *d = xbuf(D, sizeof(int) * (size_t) ntiles, "multigrid tiles");
-		upload(d, kind, sizeof(int) * (size_t) ntiles);
-		out->kind = d;
-		d = xbuf(D, sizeof(int) * ((size_t) ngraph + 1), "multigrid tiles");
-		upload(d, goff, sizeof(int) * ((size_t) ngraph + 1));
-		out->goff = d;
-		d = xbuf(D, sizeof(int) * ((size_t) ngraph * (deg + 1) + 1), "multigrid tiles");
-		upload(d, gring, sizeof(int) * (size_t) ngraph * (deg + 1));
-		out->gring = d;
-		d = xbuf(D, sizeof(int) * (ne + 1), "multigrid tiles");
-		upload(d, gcell, sizeof(int) * ne);
-		out->gcell = d;
+		out->geo = geo;
+		out->hs = hs;
+		void *d = xbuf(D, sizeof(int) * 8 * (size_t) ntiles, "multigrid tiles");
+		upload(d, tinfo, sizeof(int) * 8 * (size_t) ntiles);
+		out->tinfo = d;
+		d = xbuf(D, sizeof(int) * 2 * (ne + 1), "multigrid tiles");
+		upload(d, gcc, sizeof(int) * 2 * ne);
+		out->gcc = d;
+		free(gcc);
 		d = xbuf(D, sizeof(unsigned short) * 8 * (ne + 1), "multigrid tiles");
 		upload(d, gnbr, sizeof(unsigned short) * 8 * ne);
 		out->gnbr = d;
+		if (rl) {
+			d = xbuf(D, sizeof(unsigned short) * (nr + 1), "multigrid tiles");
+			upload(d, grow, sizeof(unsigned short) * nr);
+			out->grow = d;
+			d = xbuf(D, sizeof(float) * (nr + 1), "multigrid tiles");
+			upload(d, grw, sizeof(float) * nr);
+			out->grw = d;
+		}
 	}
 	free(reg);
 	free(kind);
+	free(tinfo);
 	free(stamp);
 	free(loc);
 	free(gcell);
 	free(gnbr);
 	free(goff);
 	free(gring);
+	free(grow);
+	free(grw);
 	return rc;
 }
 
@@ -853,12 +947,21 @@ static int cheb_eligible(const struct ggb_fluid_dev *D, const CubeSphere *cs)
 	return 1;
 }
 
+/* capacities of the tiled passes' workgroups (mgt_k.hip): 2 ext cells per thread, 3 in
+ * the down passes of the coarse levels (degree 3, one ring more) */
+#define MGT_CAP_UP 512
+#define MGT_CAP_DOWN0 512
+#define MGT_CAP_DOWN 768
+
 static void setup_cheb(struct ggb_fluid_dev *D, const CubeSphere *cs)
 {
 	for (int l = 0; l < D->dense_level; l++) {
 		const CubeSphere *c = l == 0 ? cs : &D->coarse[l];
-		if (build_tiles(D, c, l == 0 ? 2 : 3, &D->levels[l].tl))
-			ggb_die("fluid_init: a multigrid tile of level %d exceeds %d cells", l, MGT_CAP);
+		int deg = l == 0 ? 2 : 3, capd = l == 0 ? MGT_CAP_DOWN0 : MGT_CAP_DOWN;
+		if (build_tiles(D, c, deg, NULL, 0, MGT_CAP_UP, &D->levels[l].tl))
+			ggb_die("fluid_init: a multigrid tile of level %d exceeds %d cells", l, MGT_CAP_UP);
+		if (build_tiles(D, c, deg, &D->hl[l], D->levels[l].xf.rana, capd, &D->levels[l].td))
+			ggb_die("fluid_init: a multigrid tile of level %d exceeds %d cells", l, capd);
 		D->levels[l].xpre = xbuf(D, sizeof(float) * (size_t) c->ncells, "multigrid");
 	}
 }
@@ -974,12 +1077,12 @@ int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
 		int cheb = D->prec == GGB_PREC_MIXED && cheb_eligible(D, cs);
 		if (D->dense_level > 0)
 			build_dense(D, L, cheb ? 3 : 1);
-		for (int l = 0; l < D->nlev; l++)
-			host_level_free(&D->hl[l]);
 		if (cheb) {
 			setup_cheb(D, cs);
 			D->cheb = 1;
 		}
+		for (int l = 0; l < D->nlev; l++)
+			host_level_free(&D->hl[l]);
 		if (D->prec == GGB_PREC_MIXED)
 			D->psi_prev = xbuf(D, sizeof(double) * n, "fluid");
 	}

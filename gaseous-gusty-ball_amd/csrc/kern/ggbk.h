@@ -47,6 +47,7 @@ typedef struct {
 	int nf, nc, rmax, rpad;  /* rpad: padded row width, 18 or GGBK_RMAX */
 	int Nc, analytic;  /* coarse face size; indices of face-interior rows follow the
 	                    * bilinear pattern (only the weights are read for those) */
+	int rana, pad_;    /* the pattern holds (rw16 is then tabulated), whatever the size */
 	const float *pw;   /* [nf][4] weights of each fine cell's taps */
 	const float *rw16; /* [4][nc][4] weights of the 16 entries of face-interior rows of R, in
 	                    * row order (entry e in element e / 4, lane e % 4) */
@@ -183,19 +184,30 @@ void ggbk_cg_zero_if_bb0(ggbk_grid g, void *psi, int T, const ggbk_scalars *s);
  * over the Voronoi stencil, and each stencil neighbour as a slot of that list. */
 typedef struct {
 	int ntiles, deg, cap, ngraph;   /* cap: largest ext cell count of a graph tile */
-	const int *kind;                /* [ntiles] -1 interior, else graph index */
-	const int *goff;                /* [ngraph + 1] first ext cell of each graph tile */
-	const int *gring;               /* [ngraph][deg + 1] ext cells in rings 0..r (cumulative) */
-	const int *gcell;               /* [ext] cell index of each ext cell, ring by ring */
+	int geo;                        /* 1: the interior tiles are exactly those whose hs
+	                                 * rings lie within their face (the kernel then tells
+	                                 * them by their index alone) */
+	int hs, pad_[2];                /* rings of an interior tile */
+	const int *tinfo;               /* [ntiles][8] kind (-1 interior, else graph index),
+	                                 * first ext cell, ext count, ext cells in rings 0..r */
+	const int *gcc;                 /* [ext][2] cell index and stencil weight class of each
+	                                 * ext cell of the graph tiles, ring by ring */
 	const unsigned short *gnbr;     /* [ext][8] slot of each stencil neighbour, self-padded */
+	/* tiles of a down pass with the restriction fused: ring 0 of a graph tile is the fine
+	 * support of the rows of R of the tile's 8x8 coarse cells (its own 16x16 cells first),
+	 * and each row is listed by slot; an interior tile's rows follow the bilinear pattern */
+	const unsigned short *grow;     /* [ngraph][64][GGBK_TROW] slots of each coarse row */
+	const float *grw;               /* [ngraph][64][GGBK_TROW] their weights (zero-padded) */
 } ggbk_tiles;
+#define GGBK_TROW GGBK_RMAX
 
 typedef struct {
 	ggbk_grid g;
 	ggbk_xfer xf;     /* transfer to the next coarser level (unused on the coarsest) */
 	void *x, *b, *r;  /* TM scratch */
 	const void *M;    /* dense sub-V-cycle operator, or NULL */
-	ggbk_tiles tl;    /* multi-sweep tiles (tl.ntiles == 0: none) */
+	ggbk_tiles tl;    /* multi-sweep tiles of the up pass (tl.ntiles == 0: none) */
+	ggbk_tiles td;    /* tiles of the down pass (restriction into the next level fused) */
 	void *xpre;       /* TM: the pre-smoothed x of the tiled passes */
 } ggbk_level;
 /* z = V(r); with red.part != NULL also the r.z partial (slot RZ) of ggbk_nblocks(n0) blocks */

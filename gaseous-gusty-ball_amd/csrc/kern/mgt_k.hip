@@ -93,27 +93,32 @@ struct ExtCell {
 	float dinv;
 };
 
-// Sets up ext cell e of tile `tile` (ring >= D: value only, no stencil).
-template <int D>
-__device__ __forceinline__ void ext_cell(const ggbk_grid &g, const ggbk_tiles &tl, int tile, int kind, int e, int next,
-					 const int *ring_end, ExtCell &x)
+// Sets up ext cell e of tile `tile`: an interior tile's HS-ring square, or a graph tile's
+// list (rings 0..HG, from ext cell `off`); cells on rings < its tile's ring count get
+// their stencil.
+template <int HS, int HG>
+__device__ __forceinline__ void ext_cell(const ggbk_grid &g, const ggbk_tiles &tl, int tile, int kind, int off, int e,
+					 int next, const int *ring_end, ExtCell &x)
 {
 	x.c = -1;
 	if (e >= next)
 		return;
-	constexpr int W = MGT_T + 2 * D;
-	int c;
+	constexpr int W = MGT_T + 2 * HS;
+	int c, nst, cls = 0;
 	if (kind < 0) {
 		int N = g.N, tpe = N / MGT_T;
 		int f = tile / (tpe * tpe), r = tile - f * tpe * tpe;
 		int J0 = (r / tpe) * MGT_T, I0 = (r % tpe) * MGT_T;
 		int ey, ex, ring;
-		square_pos<D>(e, ey, ex, ring);
-		c = (f * N + J0 - D + ey) * N + I0 - D + ex;
+		square_pos<HS>(e, ey, ex, ring);
+		c = (f * N + J0 - HS + ey) * N + I0 - HS + ex;
 		x.slot = ey * W + ex;
 		x.ring = ring;
-		if (ring < D) {
-			unsigned code = (unsigned) ((const int2 *) g.srec)[c].y;
+		nst = HS;
+		if (ring < HS) {
+			int2 rc = ((const int2 *) g.srec)[c];
+			cls = rc.x & 0x7fffffff;
+			unsigned code = (unsigned) rc.y;
 #pragma unroll
 			for (int k = 0; k < 8; k++) {
 				int dy = (int) ((code >> (4 * k + 2)) & 3u) - 1, dx = (int) ((code >> (4 * k)) & 3u) - 1;
@@ -121,15 +126,17 @@ __device__ __forceinline__ void ext_cell(const ggbk_grid &g, const ggbk_tiles &t
 			}
 		}
 	} else {
-		int off = tl.goff[kind];
-		c = tl.gcell[off + e];
+		int2 cc = ((const int2 *) tl.gcc)[off + e];
+		c = cc.x;
+		cls = cc.y;
 		x.slot = e;
 		int ring = 0;
 #pragma unroll
-		for (int r = 0; r < D; r++)
+		for (int r = 0; r < HG; r++)
 			ring += e >= ring_end[r];
 		x.ring = ring;
-		if (ring < D) {
+		nst = HG;
+		if (ring < HG) {
 			const uint4 *q = (const uint4 *) (tl.gnbr + 8 * (size_t) (off + e));
 			uint4 v = *q;
 			x.nb[0] = (unsigned short) (v.x & 0xffff);
@@ -143,8 +150,7 @@ __device__ __forceinline__ void ext_cell(const ggbk_grid &g, const ggbk_tiles &t
 		}
 	}
 	x.c = c;
-	if (x.ring < D) {
-		int cls = ((const int2 *) g.srec)[c].x & 0x7fffffff;
+	if (x.ring < nst) {
 		const float4 *wp = (const float4 *) (g.scw + 8 * (size_t) cls);
 		float4 wa = wp[0], wb = wp[1];
 		x.w[0] = wa.x;
@@ -170,48 +176,73 @@ __device__ __forceinline__ float kx_lds(const ExtCell &x, const float *xs)
 	return s;
 }
 
-// Tile index, kind and the graph tile's ext count and ring ends, shared by the workgroup.
-template <int D>
-__device__ __forceinline__ int tile_setup(const ggbk_tiles &tl, int tile, int &kind, int *ring_end)
+// The tile's kind and ext count, and a graph tile's first ext cell and ring ends (rings
+// 0..HG).  With tl.geo the interior tiles are told by their index, without a load.
+template <int HS, int HG>
+__device__ __forceinline__ int tile_setup(const ggbk_grid &g, const ggbk_tiles &tl, int tile, int &kind, int &off,
+					  int *ring_end)
 {
-	kind = tl.kind[tile];
-	if (kind < 0) {
-		constexpr int W = MGT_T + 2 * D;
-		return W * W;
+	constexpr int W = MGT_T + 2 * HS;
+	off = 0;
+	if (tl.geo) {
+		int N = g.N, tpe = N / MGT_T;
+		int f = tile / (tpe * tpe), r = tile - f * tpe * tpe;
+		int J0 = (r / tpe) * MGT_T, I0 = (r % tpe) * MGT_T;
+		if (I0 >= HS && J0 >= HS && I0 + MGT_T - 1 + HS <= N - 1 && J0 + MGT_T - 1 + HS <= N - 1) {
+			kind = -1;
+			return W * W;
+		}
 	}
+	const int4 *ti = (const int4 *) (tl.tinfo + 8 * (size_t) tile);
+	int4 a = ti[0], b = ti[1];
+	kind = a.x;
+	if (kind < 0)
+		return W * W;
+	off = a.y;
+	int rv[5] = { a.w, b.x, b.y, b.z, b.w };
 #pragma unroll
-	for (int r = 0; r <= D; r++)
-		ring_end[r] = tl.gring[kind * (D + 1) + r];
-	return ring_end[D];
+	for (int r = 0; r <= HG; r++)
+		ring_end[r] = rv[r];
+	return a.z;
 }
 
-// Pre-smoothing pass (down): x = D Chebyshev steps from 0 on b; r = b - K x.
-// UPD: level 0 of a PCG iteration, where b = r_in - alpha Ap is the updated CG residual
-// (stored as r_out on the tile, with psi += alpha p and the r.r partial).
-template <int D, typename TB, bool UPD>
-__global__ __launch_bounds__(MGT_BS) void k_mgt_down(ggbk_grid g, ggbk_tiles tl, const TB *b, float *r, float *xpre,
-						       double *psi, const double *r_in, double *r_out, const double *p,
-						       const double *Ap, const ggbk_scalars *s, ggbk_red red)
+// Pre-smoothing pass with the restriction into the next level fused (down).  The
+// workgroup owns an 8x8 block of coarse cells, whose rows of R cover its 16x16 fine tile
+// and one ring of fine cells: x = D Chebyshev steps from 0 on b there and on D rings
+// around (D + 1 rings of an interior tile's square, D around a graph tile's support), the
+// fine residual r = b - K x on that support, and each coarse cell's b = sum of its row of
+// R times r (float, in row order).  x is stored for the fine tile (the up pass starts from
+// it).  UPD: level 0 of a PCG iteration, where b = r_in - alpha Ap is the updated CG
+// residual (stored as r_out on the tile, with psi += alpha p and the r.r partial).
+template <int D, typename TB, bool UPD, int PER>
+__global__ __launch_bounds__(MGT_BS) void k_mgt_down(ggbk_grid g, ggbk_tiles tl, ggbk_xfer xf, const TB *b, float *bc,
+						       float *xpre, double *psi, const double *r_in, double *r_out,
+						       const double *p, const double *Ap, const ggbk_scalars *s,
+						       ggbk_red red)
 {
-	__shared__ float xa[MGT_CAP], xb[MGT_CAP];
+	constexpr int CAP = PER * MGT_BS, HS = D + 1, W = MGT_T + 2 * HS;
+	__shared__ float xa[CAP], xb[CAP];
 	int act = s->active;
 	double alpha = UPD ? s->alpha : 0.0;
-	int tile = xblock(), kind, ring_end[D + 1];
-	int next = tile_setup<D>(tl, tile, kind, ring_end);
-	ExtCell xc[MGT_PER];
-	float bv[MGT_PER], dv[MGT_PER];
+	int tile = xblock(), kind, off, ring_end[D + 1];
+	int next = tile_setup<HS, D>(g, tl, tile, kind, off, ring_end);
+	int hk = kind < 0 ? HS : D; // rings of this tile's ext set
+	ExtCell xc[PER];
+	float bv[PER], dv[PER];
 	double rr = 0;
 #pragma unroll
-	for (int m = 0; m < MGT_PER; m++) {
-		ext_cell<D>(g, tl, tile, kind, threadIdx.x + m * MGT_BS, next, ring_end, xc[m]);
+	for (int m = 0; m < PER; m++) {
+		int e = threadIdx.x + m * MGT_BS;
+		ext_cell<HS, D>(g, tl, tile, kind, off, e, next, ring_end, xc[m]);
 		bv[m] = 0.0f;
 		dv[m] = 0.0f;
 		if (xc[m].c >= 0) {
 			int c = xc[m].c;
+			bool own = kind < 0 ? xc[m].ring == 0 : e < MGT_T * MGT_T;
 			double bd;
 			if (UPD) {
 				bd = r_in[c] - alpha * Ap[c];
-				if (xc[m].ring == 0) {
+				if (own) {
 					double pv = psi[c] + alpha * p[c];
 					if (act) {
 						psi[c] = pv;
@@ -232,8 +263,8 @@ __global__ __launch_bounds__(MGT_BS) void k_mgt_down(ggbk_grid g, ggbk_tiles tl,
 #pragma unroll
 	for (int k = 1; k < D; k++) {
 #pragma unroll
-		for (int m = 0; m < MGT_PER; m++) {
-			if (xc[m].c >= 0 && xc[m].ring <= D - k) {
+		for (int m = 0; m < PER; m++) {
+			if (xc[m].c >= 0 && xc[m].ring <= hk - k) {
 				float res = bv[m] - kx_lds(xc[m], xs);
 				dv[m] = cheb_cd(k) * dv[m] + cheb_cr(k) * xc[m].dinv * res;
 				xn[xc[m].slot] = xs[xc[m].slot] + dv[m];
@@ -244,12 +275,47 @@ __global__ __launch_bounds__(MGT_BS) void k_mgt_down(ggbk_grid g, ggbk_tiles tl,
 		xs = xn;
 		xn = t;
 	}
+	// fine residual on the support (into the free buffer) and x on the fine tile
 #pragma unroll
-	for (int m = 0; m < MGT_PER; m++) {
-		if (xc[m].c >= 0 && xc[m].ring == 0 && act) {
-			r[xc[m].c] = bv[m] - kx_lds(xc[m], xs);
-			xpre[xc[m].c] = xs[xc[m].slot];
+	for (int m = 0; m < PER; m++) {
+		if (xc[m].c >= 0 && xc[m].ring <= hk - D) {
+			xn[xc[m].slot] = bv[m] - kx_lds(xc[m], xs);
+			bool own = kind < 0 ? xc[m].ring == 0 : threadIdx.x + m * MGT_BS < MGT_T * MGT_T;
+			if (own && act)
+				xpre[xc[m].c] = xs[xc[m].slot];
 		}
+	}
+	__syncthreads();
+	if (threadIdx.x < 64) {
+		int N = g.N, tpe = N / MGT_T, Nc = N / 2;
+		int f = tile / (tpe * tpe), rt = tile - f * tpe * tpe;
+		int J0 = (rt / tpe) * MGT_T, I0 = (rt % tpe) * MGT_T;
+		int J = J0 / 2 + threadIdx.x / 8, I = I0 / 2 + threadIdx.x % 8;
+		int cc = (f * Nc + J) * Nc + I;
+		float acc = 0.0f;
+		if (kind < 0) {
+			// the 4x4 fine block x in [2I - 1, 2I + 2], y in [2J - 1, 2J + 2], row-major,
+			// with the weights of the row as four 16-byte groups
+			const float4 *wq = (const float4 *) xf.rw16;
+			size_t nc = (size_t) xf.nc;
+			int sy = 2 * J - 1 - (J0 - HS), sx = 2 * I - 1 - (I0 - HS);
+#pragma unroll
+			for (int q = 0; q < 4; q++) {
+				float4 wv = wq[q * nc + cc];
+				const float *row = xn + (sy + q) * W + sx;
+				acc += wv.x * row[0];
+				acc += wv.y * row[1];
+				acc += wv.z * row[2];
+				acc += wv.w * row[3];
+			}
+		} else {
+			size_t o = ((size_t) kind * 64 + threadIdx.x) * GGBK_TROW;
+#pragma unroll 4
+			for (int k = 0; k < GGBK_TROW; k++)
+				acc += tl.grw[o + k] * xn[tl.grow[o + k]];
+		}
+		if (act)
+			bc[cc] = acc;
 	}
 	if (UPD && act) {
 		double bsum = block_sum<MGT_BS>(rr);
@@ -287,14 +353,14 @@ __global__ __launch_bounds__(MGT_BS) void k_mgt_up(ggbk_grid g, ggbk_tiles tl, g
 {
 	__shared__ float xa[MGT_CAP], xb[MGT_CAP];
 	int act = s->active;
-	int tile = xblock(), kind, ring_end[D + 1];
-	int next = tile_setup<D>(tl, tile, kind, ring_end);
+	int tile = xblock(), kind, off, ring_end[D + 1];
+	int next = tile_setup<D, D>(g, tl, tile, kind, off, ring_end);
 	ExtCell xc[MGT_PER];
 	float bv[MGT_PER], dv[MGT_PER];
 	double bz[MGT_PER];
 #pragma unroll
 	for (int m = 0; m < MGT_PER; m++) {
-		ext_cell<D>(g, tl, tile, kind, threadIdx.x + m * MGT_BS, next, ring_end, xc[m]);
+		ext_cell<D, D>(g, tl, tile, kind, off, threadIdx.x + m * MGT_BS, next, ring_end, xc[m]);
 		bv[m] = 0.0f;
 		dv[m] = 0.0f;
 		bz[m] = 0.0;
@@ -344,15 +410,15 @@ __global__ __launch_bounds__(MGT_BS) void k_mgt_up(ggbk_grid g, ggbk_tiles tl, g
 	}
 }
 
-// Restriction and dense bottom (mg_k.hip)
-extern "C" void ggbk_mg_restrict_f32(ggbk_xfer xf, const float *rf, float *bc, const ggbk_scalars *s);
+// Dense bottom (mg_k.hip)
 extern "C" void ggbk_mg_dense_f32(const float *M, int n, const float *b, float *x, const ggbk_scalars *s);
 
-template <int D>
-static void down_level(const ggbk_level &L, const float *b, ggbk_scalars *s)
+// down pass of level L into level C's b
+template <int D, int PER>
+static void down_level(const ggbk_level &L, const ggbk_level &C, const float *b, ggbk_scalars *s)
 {
 	ggbk_red none = { nullptr, 0 };
-	GGBK_LAUNCH((k_mgt_down<D, float, false>), L.tl.ntiles, MGT_BS, 0, L.g, L.tl, b, (float *) L.r,
+	GGBK_LAUNCH((k_mgt_down<D, float, false, PER>), L.td.ntiles, MGT_BS, 0, L.g, L.td, L.xf, b, (float *) C.b,
 		    (float *) L.xpre, nullptr, nullptr, nullptr, nullptr, nullptr, s, none);
 }
 
@@ -368,44 +434,34 @@ extern "C" void ggbk_vcycle_cheb(const ggbk_level *lv, int nlev, const ggbk_cg_f
 				 void *zout, ggbk_scalars *s, ggbk_red red)
 {
 	const ggbk_level &L0 = lv[0];
-	// level 0 down (degree 2), fused with the PCG update when u is given
+	// level 0 down (degree 2) into level 1, fused with the PCG update when u is given
 	if (u) {
 		ggbk_probe_begin(GGBK_PROBE_CG_UPDATE);
-		GGBK_LAUNCH((k_mgt_down<2, double, true>), L0.tl.ntiles, MGT_BS, 0, L0.g, L0.tl, (const double *) nullptr,
-			    (float *) L0.r, (float *) L0.xpre, (double *) u->psi, (const double *) u->r_in,
-			    (double *) r_out, (const double *) u->p, (const double *) u->Ap, s, red);
+		GGBK_LAUNCH((k_mgt_down<2, double, true, 2>), L0.td.ntiles, MGT_BS, 0, L0.g, L0.td, L0.xf,
+			    (const double *) nullptr, (float *) lv[1].b, (float *) L0.xpre, (double *) u->psi,
+			    (const double *) u->r_in, (double *) r_out, (const double *) u->p, (const double *) u->Ap, s,
+			    red);
 		ggbk_probe_end(GGBK_PROBE_CG_UPDATE);
-		ggbk_fin(GGBK_FIN_RR, s, red, L0.tl.ntiles, 0, 0, 0);
+		ggbk_fin(GGBK_FIN_RR, s, red, L0.td.ntiles, 0, 0, 0);
 	} else {
 		ggbk_probe_begin(GGBK_PROBE_MG_DOWN0);
 		ggbk_red none = { nullptr, 0 };
-		GGBK_LAUNCH((k_mgt_down<2, double, false>), L0.tl.ntiles, MGT_BS, 0, L0.g, L0.tl, (const double *) r,
-			    (float *) L0.r, (float *) L0.xpre, nullptr, nullptr, nullptr, nullptr, nullptr, s, none);
+		GGBK_LAUNCH((k_mgt_down<2, double, false, 2>), L0.td.ntiles, MGT_BS, 0, L0.g, L0.td, L0.xf, (const double *) r,
+			    (float *) lv[1].b, (float *) L0.xpre, nullptr, nullptr, nullptr, nullptr, nullptr, s, none);
 		ggbk_probe_end(GGBK_PROBE_MG_DOWN0);
 	}
 	const double *b0 = (const double *) (u ? r_out : r);
-	// restrict and smooth down to the dense level
-	int l = 0;
-	for (;;) {
-		const ggbk_level &C = lv[l + 1];
-		ggbk_mg_restrict_f32(lv[l].xf, (const float *) lv[l].r, (float *) C.b, s);
-		if (C.M) {
-			ggbk_mg_dense_f32((const float *) C.M, C.g.n, (const float *) C.b, (float *) C.x, s);
-			break;
-		}
+	// smooth and restrict down to the dense level
+	int l = 1;
+	while (!lv[l].M) {
+		down_level<3, 3>(lv[l], lv[l + 1], (const float *) lv[l].b, s);
 		l++;
-		if (lv[l].tl.deg == 3)
-			down_level<3>(lv[l], (const float *) lv[l].b, s);
-		else
-			down_level<2>(lv[l], (const float *) lv[l].b, s);
 	}
+	ggbk_mg_dense_f32((const float *) lv[l].M, lv[l].g.n, (const float *) lv[l].b, (float *) lv[l].x, s);
 	// back up
-	for (; l >= 1; l--) {
+	for (l = l - 1; l >= 1; l--) {
 		const ggbk_level &L = lv[l];
-		if (L.tl.deg == 3)
-			up_level<3>(L, (const float *) L.b, (const float *) lv[l + 1].x, (float *) L.x, s);
-		else
-			up_level<2>(L, (const float *) L.b, (const float *) lv[l + 1].x, (float *) L.x, s);
+		up_level<3>(L, (const float *) L.b, (const float *) lv[l + 1].x, (float *) L.x, s);
 	}
 	ggbk_probe_begin(GGBK_PROBE_MG_UP0);
 	GGBK_LAUNCH((k_mgt_up<2, double, true>), L0.tl.ntiles, MGT_BS, 0, L0.g, L0.tl, L0.xf, b0, (const float *) L0.xpre,
