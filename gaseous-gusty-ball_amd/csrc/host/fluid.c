@@ -1619,6 +1619,25 @@ void fluid_remove_mean(const CubeSphere *cs, float *f)
 	ggbk_free(red.part);
 }
 
+/* fluid_remove_mean on device memory, allocating nothing (ggb_ext.h) */
+int fluid_remove_mean_noalloc(const CubeSphere *cs, float *f_dev)
+{
+	if (cs->ncells > GGBK_NOALLOC_CELLS) {
+		fprintf(stderr, "fluid_remove_mean_noalloc: %d cells exceed the scratch's %d\n", cs->ncells,
+			GGBK_NOALLOC_CELLS);
+		return -2;
+	}
+	if (!cs->dev) {
+		fprintf(stderr, "fluid_remove_mean_noalloc: the grid's tables are not on the device\n");
+		return -1;
+	}
+	int rc = ggbk_remove_mean_noalloc(cs->dev->g, f_dev);
+	if (rc)
+		fprintf(stderr, "fluid_remove_mean_noalloc: %s\n",
+			rc == -2 ? "grid exceeds the scratch" : "device scratch unavailable or launch failed");
+	return rc;
+}
+
 /* ---------------- discrete adjoint ----------------
  *
  * Gradient of a loss L(omega_T) with respect to the initial vorticity and the forcing

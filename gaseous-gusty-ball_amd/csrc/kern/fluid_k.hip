@@ -586,6 +586,50 @@ extern "C" void ggbk_remove_mean(ggbk_grid g, void *f, int T, ggbk_scalars *s, g
 	ggbk_sub_mean(g.n, f, T, s);
 }
 
+// ---------------- scratch of the entry point that allocates nothing ----------------
+// ggbk_remove_mean_noalloc's reduction scratch: the scalar block, then the per-block
+// partials of every slot, for grids of up to GGBK_NOALLOC_CELLS cells (1.4 MB).  Each GPU
+// holds its own copy for the life of the process; host code reaches it only through the
+// address hipGetSymbolAddress returned on that device (the symbol's name is a host-side
+// stand-in, not a device address).
+#define NOALLOC_BLOCKS ((GGBK_NOALLOC_CELLS + BS - 1) / BS)
+#define NOALLOC_HEAD 256
+static_assert(sizeof(ggbk_scalars) <= NOALLOC_HEAD, "scalar block exceeds its scratch slot");
+static __device__ __align__(256) unsigned char scratch[NOALLOC_HEAD + sizeof(double) * GGBK_NSLOTS * NOALLOC_BLOCKS];
+#define NOALLOC_MAX_DEVICES 64
+static void *g_scratch_dev[NOALLOC_MAX_DEVICES];
+
+extern "C" int ggbk_noalloc_init(void)
+{
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+		return -1;
+	void *ptr = nullptr;
+	if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+		return -1;
+	g_scratch_dev[dev] = ptr;
+	return 0;
+}
+
+extern "C" int ggbk_remove_mean_noalloc(ggbk_grid g, float *f)
+{
+	if (g.n > GGBK_NOALLOC_CELLS)
+		return -2;
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES || !g_scratch_dev[dev])
+		return -1;
+	unsigned char *base = (unsigned char *) g_scratch_dev[dev];
+	ggbk_scalars *s = (ggbk_scalars *) base;
+	ggbk_red red;
+	red.part = (double *) (base + NOALLOC_HEAD);
+	red.stride = (int) NB(g.n);
+	// every partial the fold reads is written first; the scalars start cleared
+	if (ggbk_zero(s, sizeof(ggbk_scalars)))
+		return -1;
+	ggbk_remove_mean(g, f, GGBK_F32, s, red);
+	return ggbk_check("remove_mean_noalloc") ? -1 : 0;
+}
+
 extern "C" void ggbk_cg_start(ggbk_grid g, void *omega, const void *psi, void *r, int T, int sub_mean,
 			      const ggbk_scalars *s, ggbk_red red)
 {

@@ -153,6 +153,12 @@ void ggbk_mean_partial(ggbk_grid g, const void *f, int T, ggbk_red red);
 void ggbk_sub_mean(int n, void *f, int T, const ggbk_scalars *s);
 /* f -= area-weighted mean, rounded to T as the reference rounds it (fluid.c:264-275) */
 void ggbk_remove_mean(ggbk_grid g, void *f, int T, ggbk_scalars *s, ggbk_red red);
+/* The same on float f in device memory, with the device's static scratch in place of
+ * allocated scalars and partials: 0, -1 (scratch not set up), -2 (more than
+ * GGBK_NOALLOC_CELLS cells).  ggbk_noalloc_init looks the scratch up on the current device. */
+#define GGBK_NOALLOC_CELLS 6291456
+int ggbk_noalloc_init(void);
+int ggbk_remove_mean_noalloc(ggbk_grid g, float *f);
 
 /* ---- PCG (reference fluid.c:288-358) ---- */
 /* [omega -= mean if sub_mean] r = b - K psi with b = -area*omega; bb, rr partials */

@@ -39,6 +39,11 @@ extern "C" int ggbk_init(void)
 	e = hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking);
 	if (e != hipSuccess)
 		return fail("ggbk_init: hipStreamCreate", e);
+	if (ggbk_noalloc_init()) {
+		snprintf(g_err, sizeof(g_err), "ggbk_init: the device scratch could not be located");
+		fprintf(stderr, "ggb: %s\n", g_err);
+		return -1;
+	}
 	g_ready = 1;
 	return 0;
 }

@@ -94,7 +94,7 @@ PUBLIC_SYMBOLS = [
     "particles_set_round32", "render_sync_host", "render_last_entries", "render_last_stats", "render_faces8",
     "render_net8", "render_preview8",
     "fluid_adjoint_new", "fluid_adjoint_free", "fluid_adjoint_forward", "fluid_adjoint_backward",
-    "fluid_adjoint_iterations", "fluid_adjoint_fix_iterations",
+    "fluid_adjoint_iterations", "fluid_adjoint_fix_iterations", "fluid_remove_mean_noalloc",
     # ggb_inputs.h
     "ggb_initial_velocity", "ggb_banded_strip", "ggb_rgba8_to_float",
 ]
@@ -181,6 +181,9 @@ def lib():
             "ggbk_probe_collect": (C.c_int, [C.c_int, P(C.c_double)]),
             "ggbk_copy_kernel": (None, [C.c_void_p, C.c_void_p, C.c_size_t]),
             "ggbk_malloc": (C.c_void_p, [C.c_size_t]),
+            "ggbk_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+            "ggbk_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+            "fluid_remove_mean_noalloc": (C.c_int, [P(CCubeSphere), C.c_void_p]),
             "ggbk_free": (None, [C.c_void_p]),
             "ggbk_event_new": (C.c_void_p, []),
             "ggbk_event_record": (None, [C.c_void_p]),

@@ -67,6 +67,19 @@ int ggb_divergence_f64(const CubeSphere *cs, const double *vel, double *out);
  * itself (cubesphere.h nbr/w). */
 int ggb_stencil_compress(const CubeSphere *cs, int *rec, float *cw, int *xn, int *ncls, int *nirr);
 
+/* fluid_remove_mean (reference fluid.c:264-275; float f -= its area-weighted mean) on
+ * f_dev, n = cs->ncells floats in device memory, allocating nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Its reduction scratch (the
+ * scalar block and per-block partials) is a static array each GPU holds for the life of
+ * the process, sized for grids of up to 6,291,456 cells (N = 1024): larger grids return -2
+ * with a message.  The grid's device tables must already exist (any earlier operator,
+ * fluid_init or particles_advect on this CubeSphere uploads them); otherwise -1.  The work
+ * is enqueued on the library stream and not waited for (ggb_sync waits); calls that
+ * overlap on one GPU share its scratch and race.  The result equals fluid_remove_mean's
+ * bit for bit.  0 on success. */
+int fluid_remove_mean_noalloc(const CubeSphere *cs, float *f_dev);
+
 /* ---- particles ---- */
 void particles_sync_host(Particles *P); /* device -> pos/prev mirrors */
 int particles_get_f64(const Particles *P, double *pos, double *prev);
