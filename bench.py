@@ -41,6 +41,7 @@ CONFIGS = {
                precision=ggb.PREC_MIXED, octaves=6),
 }
 HBM_PEAK = 8.0e12
+FP64_PEAK = 78.6e12  # MI355X FP64 vector (non-matrix)
 
 # Probed kernels (ids of the library's launch probes) and their algorithmic bytes per
 # launch under the per-pass model: every pass reads its inputs and writes its outputs
@@ -55,26 +56,79 @@ PROBES = {
 }
 
 
-def kernel_bytes(name, n, P, E, T, Ts, Tm, Tz, reorder_frac=1.0):
-    if name == "k_render_draw":      # texels read+written, list entry, FP64 endpoints, colour, index
-        return 32 * T + E * (4 + 48 + 16 + 4)
+def kernel_bytes(name, n, P, E, T, Ts, Tm, Tz, reorder_frac=1.0, cheb=False):
+    if name == "k_render_draw":      # the render contract: prev, pos, colour per particle (f32) + texel RMW
+        return 40 * P + 32 * T
     if name == "render_count":       # current position (FP64) for the home tile
         return P * 24
     if name == "k_render_fill":      # endpoints read, sample count, list; on reordering deposits
         return P * (48 + 2) + reorder_frac * P * (20 + 68) + E * 4  # also colour+index, copy out
-    if name == "mg_up(level 0)":     # b, dinv, stencil record, tap weights, coarse x, z out
-        return n * (Ts + 4 + 8 + 16 + Tm / 4 + Tm)
-    if name == "k_mg_down(level 0)":  # b, dinv, stencil record, r1 out
-        return n * (Ts + 4 + 8 + Tm)
+    if name == "mg_up(level 0)":
+        if cheb:                     # b, x_pre, tap weights, coarse x, stencil record, dinv, z out
+            return n * (Ts + 4 + 16 + 1 + 8 + 4 + 4)
+        return n * (Ts + 4 + 8 + 16 + Tm / 4 + Tm)   # b, dinv, stencil record, tap weights, coarse x, z out
+    if name == "k_mg_down(level 0)":
+        if cheb:                     # b, stencil record, dinv; x_pre and the coarse b out
+            return n * (Ts + 8 + 4 + 4 + 1)
+        return n * (Ts + 4 + 8 + Tm)  # b, dinv, stencil record, r1 out
     if name == "k_cg_Ap":            # z, p in, stencil record, p out, Ap out
         return n * (Tz + Ts + 8 + 2 * Ts)
-    if name == "k_cg_update_down":   # psi, r, p, Ap, dinv, stencil record in; psi, r, r1 out
-        return n * (6 * Ts + 12 + Tm)
+    if name == "k_cg_update_down":
+        if cheb:                     # psi, r, p, Ap, stencil record, dinv in; psi, r, x_pre, coarse b out
+            return n * (6 * Ts + 12 + 4 + 1)
+        return n * (6 * Ts + 12 + Tm)  # psi, r, p, Ap, dinv, stencil record in; psi, r, r1 out
     if name == "k_particles_advect":  # pos in, pos+prev out (FP64), velocity field once
         return P * 72 + n * 3 * Ts
     if name == "k_departures":       # vel, omega, centre in; q0, dep_b, dep_f out
         return n * (3 * Ts + Ts + 12 + Ts + 6 * Ts)
     raise KeyError(name)
+
+
+def fluid_step_bytes(N, its, vcycles, precision, cheb):
+    """Algorithmic bytes of one fluid step under the per-pass model (each pass reads its
+    inputs and writes its outputs once; gathered neighbours, the stencil's class weights
+    and the small transfer tables are cache hits): the fixed passes plus, per V-cycle, the
+    level-0 down and up passes and every coarser level's, and per CG iteration the Ap pass.
+    Returns (bytes, breakdown)."""
+    n = 6 * N * N
+    Ts = 4 if precision == ggb.PREC_REF32 else 8
+    Tm = 8 if precision == ggb.PREC_FP64 else 4
+    fixed = {
+        "departures": n * 100 if Ts == 8 else n * 52,
+        "advect + bfecc + limiter": n * (40 + 48 + 64) * Ts // 8,
+        "cg_start": n * (3 * Ts + 4 + 8),
+        "velocity + mean": n * (2 * Ts + 96 + 32 + 12 + 3 * Ts + 12),
+        "extrapolated start": n * 4 * Ts if precision == ggb.PREC_MIXED else 0,
+    }
+    per_cycle = 0.0
+    levels = []
+    m = n
+    Nl = N
+    while Nl >= 4 and m > 1536:
+        levels.append(m)
+        if Nl % 2:
+            break
+        Nl //= 2
+        m = 6 * Nl * Nl
+    for l, nl in enumerate(levels):
+        if l == 0:
+            continue
+        # a coarse level's down pass (b, stencil record, dinv in; x_pre and the next level's b
+        # out) and up pass (b, x_pre, taps and weights, coarse x, stencil record, dinv in; x out)
+        per_cycle += nl * (4.5 * Tm + 56)
+    if m <= 1536:
+        per_cycle += Tm * m * m + 2 * Tm * m  # the dense bottom operator and its vectors
+    up0 = kernel_bytes("mg_up(level 0)", n, 0, 0, 0, Ts, Tm, Tm, cheb=cheb)
+    down0 = kernel_bytes("k_mg_down(level 0)", n, 0, 0, 0, Ts, Tm, Tm, cheb=cheb)
+    upd0 = kernel_bytes("k_cg_update_down", n, 0, 0, 0, Ts, Tm, Tm, cheb=cheb)
+    ap = kernel_bytes("k_cg_Ap", n, 0, 0, 0, Ts, Tm, Tm)
+    solver = {
+        "V-cycles": vcycles * (per_cycle + up0) + down0 + (vcycles - 1) * upd0,
+        "Ap": its * ap,
+    }
+    b = dict(fixed)
+    b.update(solver)
+    return float(sum(b.values())), {k: int(v) for k, v in b.items()}
 
 
 def log(*a):
@@ -122,8 +176,11 @@ def cpu_baseline(cfg, steps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ggb_oracle as O
 
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    cores = min(cores, 16)
+    # every core this process may run on, as many as OMP_NUM_THREADS allows (a one-GPU
+    # share of the host's cores where the host sets it)
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    cores = min(avail, omp) if omp > 0 else avail
     O.set_omp_threads(cores)
     N = cfg["N"]
     nl = O.olib("f64").orc_mg_nlev(N)
@@ -162,7 +219,46 @@ def cpu_baseline(cfg, steps):
     except Exception:
         model = "unknown"
     res["cpu_model"] = model
+    res["host_cpus"] = os.cpu_count()
+    res["affinity_cpus"] = avail
     return res
+
+
+def reference_c1(steps=10):
+    """The reference's own fluid (its -O3 build: float PCG, tol 1e-4) at N=256 on this
+    host's cores, from the same seeded initial flow: fluid steps/s over `steps` steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ggb_oracle as O
+    if not O.available("ref_fast"):
+        return None
+    cfg = CONFIGS["c1"]
+    g = ggb.Grid(cfg["N"])
+    vel0, _ = g.initial_velocity(seed=1, octaves=cfg["octaves"])
+    rg = O.RefGrid(cfg["N"], kind="ref_fast")
+    RF = O.RefFluid(rg, forcing=0.0, rotation=0.0, cg_tol=cfg["cg_tol"], cg_maxit=500)
+    RF.set_state(vel0)
+    t = 0.0
+    its = 0
+    for _ in range(steps):
+        dt = g.h / max(RF.max_speed(), 1e-6)
+        t0 = time.perf_counter()
+        RF.step(dt)
+        t += time.perf_counter() - t0
+        its += RF.cg_iters
+    return {"fluid_steps_per_s": round(steps / t, 3), "cg_iters_per_step": its / steps, "steps": steps}
+
+
+def gpu_c1(steps=200, warmup=5):
+    """The FP64 drop-in on the reference's default grid (N=256, tol 1e-4): fluid steps/s."""
+    cfg = CONFIGS["c1"]
+    g, F, _, _ = setup(cfg, ggb.PREC_FP64, with_particles=False)
+    timers = {k: ggb.Timer() for k in "pfr"}
+    acc = dict(p=0.0, f=0.0, r=0.0, its=0, entries=0)
+    run_steps(g, F, None, None, warmup, timers, dict(acc))
+    run_steps(g, F, None, None, steps, timers, acc)
+    F.free()
+    return {"fluid_steps_per_s": round(1000.0 / (acc["f"] / steps), 2), "cg_iters_per_step": acc["its"] / steps,
+            "steps": steps}
 
 
 def hbm_used_bytes():
@@ -178,14 +274,18 @@ def hbm_used_bytes():
 
 
 def precision_leg(cfg, checkpoints=(10, 100, 2000), seed=1):
-    """Mixed (FP32 V-cycle in FP64 CG) against FP64 on the same workload: both fluids and
+    """Mixed (FP32 V-cycle in FP64 CG) against FP64 on the same workload: the fluids and
     their particles stepped in lockstep with the FP64 run's dt; relative L2 errors of
-    omega, psi and u and the largest particle position error (rad) at the checkpoints."""
+    omega, psi and u and the largest particle position error (rad) at the checkpoints.
+    A control run, FP64 at a ten times tighter tolerance, gives the same differences for
+    FP64 itself: the flow amplifies any change within the solve's tolerance, so after a
+    few hundred steps the two FP64 runs part as far as mixed and FP64 do."""
     g = ggb.Grid(cfg["N"])
     vel0, vband = g.initial_velocity(seed=seed, octaves=cfg["octaves"])
     runs = []
-    for prec in (ggb.PREC_FP64, ggb.PREC_MIXED):
-        F = ggb.Fluid(g, forcing=cfg["forcing"], rotation=cfg["rotation"], cg_tol=cfg["cg_tol"],
+    for prec, tol in ((ggb.PREC_FP64, cfg["cg_tol"]), (ggb.PREC_MIXED, cfg["cg_tol"]),
+                      (ggb.PREC_FP64, 0.1 * cfg["cg_tol"])):
+        F = ggb.Fluid(g, forcing=cfg["forcing"], rotation=cfg["rotation"], cg_tol=tol,
                       cg_maxit=500, precision=prec)
         F.set_state(vel0, vband if cfg["forcing"] > 0 else None)
         runs.append((F, ggb.Particles(cfg["P"])))
@@ -197,13 +297,16 @@ def precision_leg(cfg, checkpoints=(10, 100, 2000), seed=1):
             P.advect(g, F, dt)
             F.step(dt)
         if step in checkpoints:
-            (F64, P64), (Fm, Pm) = runs
-            e = {}
-            for name, fld in (("omega", ggb.FIELD_OMEGA), ("psi", ggb.FIELD_PSI), ("u", ggb.FIELD_VEL)):
-                e[name] = rel(Fm.get(fld), F64.get(fld))
-            e["particles_max_rad"] = float(np.max(np.abs(Pm.get()[0] - P64.get()[0])))
-            out[str(step)] = e
-            log("mixed vs fp64 at step %d: %s" % (step, e))
+            (F64, P64) = runs[0]
+            p64 = P64.get()[0]
+            for key, (Fo, Po) in (("mixed", runs[1]), ("fp64_tol_tenth", runs[2])):
+                e = {}
+                for name, fld in (("omega", ggb.FIELD_OMEGA), ("psi", ggb.FIELD_PSI), ("u", ggb.FIELD_VEL)):
+                    e[name] = rel(Fo.get(fld), F64.get(fld))
+                e["particles_max_rad"] = float(np.max(np.abs(Po.get()[0] - p64)))
+                out.setdefault(key, {})[str(step)] = e
+            log("at step %d: mixed vs fp64 %s; fp64 at tol/10 vs fp64 %s" %
+                (step, out["mixed"][str(step)], out["fp64_tol_tenth"][str(step)]))
     for F, P in runs:
         P.free()
         F.free()
@@ -295,6 +398,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-c1", action="store_true", help="skip the N=256 drop-in leg")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the mixed-vs-FP64 error run (steps 10, 100, 2000)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -383,20 +487,38 @@ def main():
     # recorded in profiles/traffic.json by the counter runs of this workload
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        tr = json.load(open(tpath)).get(args.config, {}).get(dominant)
-        if tr:
-            traffic = tr.get("bytes_per_launch")
+    tjson = json.load(open(tpath)).get(args.config, {}) if os.path.exists(tpath) else {}
+    if tjson.get(dominant):
+        traffic = tjson[dominant].get("bytes_per_launch")
+    cheb = cfg["precision"] == ggb.PREC_MIXED and os.environ.get("GGB_MIXED_CHEB") != "0" and cfg["N"] % 16 == 0
+    its = acc["its"] / K
+    fbytes, fparts = fluid_step_bytes(cfg["N"], its, its + 1, cfg["precision"], cheb)
+    fach = fbytes / (ms_fluid / 1000.0)
+    W = 8 if cfg["precision"] != ggb.PREC_REF32 else 4
+    survey = (47 * W + 44 + 16 * 6 + (13 * W + 24 + (4.0 / 3.0) * (14.5 * (4 if cfg["precision"] != ggb.PREC_FP64
+                                                                          else 8) + 40 + 48)) * its) * g.n
     kernels = {}
     # share of the probed deposits that reordered the particles (those run the count)
     rfrac = warm.get("render_count", (0, 0))[0] / max(warm.get("k_render_fill", (0, 0))[0], 1)
     for name, (cnt, ms) in warm.items():
         if cnt:
-            b = kernel_bytes(name, g.n, cfg["P"], E, T, Ts, Tm, Tm, rfrac)
+            b = kernel_bytes(name, g.n, cfg["P"], E, T, Ts, Tm, Tm, rfrac, cheb=cheb)
+            avg = ms / 1000.0 / cnt
             kernels[name] = {"launches_per_step": round(cnt / max(args.warmup - nplain, 1), 2),
-                             "avg_us": round(ms * 1000.0 / cnt, 2),
+                             "avg_us": round(avg * 1e6, 2),
                              "algo_bytes": int(b),
-                             "achieved_GBps": round(b / (ms / 1000.0 / cnt) / 1e9, 1)}
+                             "achieved_GBps": round(b / avg / 1e9, 1)}
+            tk = tjson.get(name, {})
+            if tk.get("bytes_per_launch"):
+                kernels[name]["traffic_bytes"] = tk["bytes_per_launch"]
+                kernels[name]["traffic_GBps"] = round(tk["bytes_per_launch"] / avg / 1e9, 1)
+            if tk.get("fp64_flops_per_launch"):
+                # FP64 VALU rate: counted flops per launch (committed counter profile) over
+                # the live average duration, against the MI355X FP64 vector peak
+                tf = tk["fp64_flops_per_launch"] / avg / 1e12
+                kernels[name]["fp64_valu"] = {"flops_per_launch": tk["fp64_flops_per_launch"],
+                                              "achieved": round(tf, 2), "peak": FP64_PEAK / 1e12,
+                                              "unit": "TFLOP/s", "frac": round(tf * 1e12 / FP64_PEAK, 4)}
     out = {
         "metric": "fluid steps/s @N=512 + Msplat-particles/s; mixed vs FP64 at 1e-8 residual",
         "value": round(fluid_sps, 3),
@@ -429,6 +551,11 @@ def main():
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4),
                      "traffic": traffic, "launches": dcount, "avg_us": round(avg_s * 1e6, 2),
                      "algo_bytes_per_launch": int(nbytes)},
+        "fluid_roofline": {"bound": "hbm", "scope": "every pass of the fluid step", "achieved": round(fach / 1e9, 1),
+                           "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(fach / HBM_PEAK, 4),
+                           "algo_bytes_per_step": int(fbytes), "passes": fparts,
+                           "survey_model_bytes_per_step": int(survey),
+                           "survey_model_frac": round(survey / (ms_fluid / 1000.0) / HBM_PEAK, 4)},
         "kernels_warmup": kernels,
     }
     if hbm_peak is not None:
@@ -451,7 +578,9 @@ def main():
 
     if not args.no_precision and cfg["precision"] == ggb.PREC_MIXED and args.config == "c2":
         t0 = time.perf_counter()
-        out["mixed_vs_fp64_error"] = precision_leg(cfg)
+        pl = precision_leg(cfg)
+        out["mixed_vs_fp64_error"] = pl["mixed"]
+        out["fp64_tol_tenth_vs_fp64_error"] = pl["fp64_tol_tenth"]
         log("precision leg %.1fs" % (time.perf_counter() - t0))
 
     if not args.no_adjoint:
@@ -461,6 +590,16 @@ def main():
         else:
             out["adjoint"] = adjoint_leg()
         log("adjoint leg %.1fs" % (time.perf_counter() - t0))
+
+    if not args.no_c1:
+        # the reference's own default grid and tolerance: the FP64 drop-in on the GPU, and
+        # (with the CPU leg) the reference's own float fluid on this host
+        t0 = time.perf_counter()
+        out["c1_drop_in"] = {"workload": "N=256 (393216 cells), CG tol 1e-4, no forcing or rotation",
+                             "gpu_fp64": gpu_c1()}
+        if not args.no_cpu:
+            out["c1_drop_in"]["reference_cpu"] = reference_c1()
+        log("c1 leg %.1fs" % (time.perf_counter() - t0))
 
     if not args.no_cpu:
         t0 = time.perf_counter()
@@ -473,8 +612,9 @@ def main():
             "kind": "port",
             "sample": "%d steps of the same workload after setup: fluid by the FP64 C restatement of the "
                       "reference (the reference's float PCG cannot reach tol 1e-8); particles and render by "
-                      "the reference's own -O3 build; OpenMP on %d threads of %s" %
-                      (cb["steps"], cb["cores"], cb["cpu_model"]),
+                      "the reference's own -O3 build; OpenMP on %d threads (the cores this process may use, "
+                      "capped by OMP_NUM_THREADS; the host shows %d) of %s" %
+                      (cb["steps"], cb["cores"], cb["host_cpus"], cb["cpu_model"]),
         }
         if cb["have_ref"]:
             out["cpu_baseline"]["msplat_particles_per_s"] = round(cfg["P"] * cb["steps"] / cb["render_s"] / 1e6, 3)

@@ -1,9 +1,10 @@
 """bench.py's command-line contract and workload definition, without a GPU: one GPU
 only (a larger --gpus exits with status 2 and says so), the default timed window is the
 N=512 workload's whole run (2000 steps after 5 warm-up steps), the roofline is the draw's
-by default, the workload is the one the metric is quoted on, and the draw's and fill's
-algorithmic byte counts follow the per-pass model (texels read and written, per-entry
-particle data)."""
+by default, the workload is the one the metric is quoted on, the draw's algorithmic bytes
+follow the render contract (prev, pos and colour per particle in float, texels read and
+written), the fill's the per-pass model, and the fluid step's aggregate is the sum of its
+passes (every V-cycle's and CG iteration's passes, plus the fixed ones)."""
 import os
 import subprocess
 import sys
@@ -56,9 +57,9 @@ def test_workload_is_the_quoted_configuration(bench):
 
 def test_byte_model(bench):
     n, P, E, T = 1_572_864, 4_000_000, 5_000_000, 19_000_000
-    # draw: 16 B read + 16 B written per touched texel; per entry the list entry, the FP64
-    # endpoints, the colour and the index
-    assert bench.kernel_bytes("k_render_draw", n, P, E, T, 8, 4, 4) == 32 * T + E * (4 + 48 + 16 + 4)
+    # draw: 40 B per particle (prev, pos, colour in float) + 16 B read and 16 B written per
+    # touched texel
+    assert bench.kernel_bytes("k_render_draw", n, P, E, T, 8, 4, 4) == 40 * P + 32 * T
     # fill: endpoints and sample count always; colour, index and the copy only when reordering
     plain = bench.kernel_bytes("k_render_fill", n, P, E, T, 8, 4, 4, 0.0)
     reord = bench.kernel_bytes("k_render_fill", n, P, E, T, 8, 4, 4, 1.0)
@@ -66,3 +67,20 @@ def test_byte_model(bench):
     assert reord - plain == P * 88
     with pytest.raises(KeyError):
         bench.kernel_bytes("no_such_kernel", n, P, E, T, 8, 4, 4)
+
+
+def test_fluid_step_byte_model(bench):
+    ggb = bench.ggb
+    b6, parts6 = bench.fluid_step_bytes(512, 6, 7, ggb.PREC_MIXED, True)
+    b7, parts7 = bench.fluid_step_bytes(512, 7, 8, ggb.PREC_MIXED, True)
+    n = 6 * 512 * 512
+    # one more iteration adds one Ap pass, one fused update + down pass, one level-0 up pass
+    # and one pass over the coarse levels -- about 160-200 B per cell in mixed precision
+    extra = (b7 - b6) / n
+    assert 150 < extra < 220, extra
+    assert parts7["Ap"] - parts6["Ap"] == bench.kernel_bytes("k_cg_Ap", n, 0, 0, 0, 8, 4, 4)
+    assert sum(parts6.values()) == int(b6) or abs(sum(parts6.values()) - b6) < len(parts6)
+    # FP64 storage moves more per V-cycle
+    b6d, _ = bench.fluid_step_bytes(512, 6, 7, ggb.PREC_FP64, False)
+    assert b6d > b6
+
