@@ -300,10 +300,10 @@ struct ggb_fluid_dev {
 	int nlev;
 	int dense_level;        /* level whose sub-V-cycle is a dense matrix, or -1 */
 	int cheb;               /* mixed precision: the tiled Chebyshev V-cycle (ggbk_vcycle_cheb) */
-	/* mixed precision: the solve of a step starts from psi extrapolated linearly in time */
-	double *psi_prev;       /* psi of the previous step (FP64), or NULL */
-	int prev_valid;         /* psi_prev and dt_prev hold the previous step */
-	double dt_prev;
+	/* mixed precision: the solve of a step starts from psi extrapolated in time */
+	double *psi_prev[2];    /* psi of the previous two steps (FP64, [0] the newer), or NULL */
+	int prev_valid;         /* how many of them (and of dt_prev) hold past steps: 0..2 */
+	double dt_prev[2];      /* the steps that led to them: [0] to psi_prev[0] */
 	int no_extrap;          /* set while the adjoint replays steps */
 	CubeSphere *coarse;     /* [nlev]; [0] unused */
 	ggbk_level *levels;     /* [nlev] */
@@ -311,6 +311,8 @@ struct ggb_fluid_dev {
 	void **xbufs;
 	int nxbufs;
 	void *omega, *omega_band, *psi, *psi_alt, *vel, *q0, *q1, *q2, *dep_b, *dep_f, *rbuf[2], *pbuf[2], *Ap, *z;
+	double *Az;             /* mixed-precision PCG: A z of the last V-cycle */
+	int nbt;                /* blocks of level 0's up pass (the fold of the mixed PCG) */
 	ggbk_scalars *s;        /* device */
 	ggbk_scalars *hs;       /* pinned host copy */
 	ggbk_red red;
@@ -958,12 +960,16 @@ static void setup_cheb(struct ggb_fluid_dev *D, const CubeSphere *cs)
 	for (int l = 0; l < D->dense_level; l++) {
 		const CubeSphere *c = l == 0 ? cs : &D->coarse[l];
 		int deg = l == 0 ? 2 : 3, capd = l == 0 ? MGT_CAP_DOWN0 : MGT_CAP_DOWN;
-		if (build_tiles(D, c, deg, NULL, 0, MGT_CAP_UP, &D->levels[l].tl))
+		/* level 0's up pass takes one ring more (A z on the tile from z on ring 1) */
+		int up_rings = l == 0 ? deg + 1 : deg;
+		if (build_tiles(D, c, up_rings, NULL, 0, MGT_CAP_UP, &D->levels[l].tl))
 			ggb_die("fluid_init: a multigrid tile of level %d exceeds %d cells", l, MGT_CAP_UP);
 		if (build_tiles(D, c, deg, &D->hl[l], D->levels[l].xf.rana, capd, &D->levels[l].td))
 			ggb_die("fluid_init: a multigrid tile of level %d exceeds %d cells", l, capd);
 		D->levels[l].xpre = xbuf(D, sizeof(float) * (size_t) c->ncells, "multigrid");
 	}
+	D->Az = xbuf(D, sizeof(double) * (size_t) cs->ncells, "fluid");
+	D->nbt = D->levels[0].tl.ntiles;
 }
 
 int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
@@ -1083,8 +1089,12 @@ int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
 		}
 		for (int l = 0; l < D->nlev; l++)
 			host_level_free(&D->hl[l]);
-		if (D->prec == GGB_PREC_MIXED)
-			D->psi_prev = xbuf(D, sizeof(double) * n, "fluid");
+		if (D->prec == GGB_PREC_MIXED) {
+			for (int h = 0; h < 2; h++) {
+				D->psi_prev[h] = xbuf(D, sizeof(double) * n, "fluid");
+				ggbk_zero(D->psi_prev[h], sizeof(double) * n);
+			}
+		}
 	}
 	D->shadow_omega = falloc(n);
 	D->shadow_band = falloc(n);
@@ -1270,9 +1280,7 @@ static int mean_op(const struct ggb_fluid_dev *D) { return D->T == GGBK_F64 ? GG
 /* z = M^-1 r and rz (fluid.c:148-160, 315-318, 344-348) */
 static void precondition(struct ggb_fluid_dev *D, const void *r, int first)
 {
-	if (D->cheb)
-		ggbk_vcycle_cheb(D->levels, D->nlev, NULL, r, NULL, D->z, D->s, D->red);
-	else if (D->nlev > 0)
+	if (D->nlev > 0)
 		ggbk_vcycle(D->levels, D->nlev, r, D->T, D->z, D->TM, D->s, D->red);
 	else
 		ggbk_jacobi_precond(D->g0, r, D->z, D->T, D->s, D->red);
@@ -1290,11 +1298,6 @@ static void cg_iteration(struct ggb_fluid_dev *D, void *psi, int k)
 	ggbk_cg_Ap(g, D->z, D->TZ, pin, pout, D->Ap, D->T, k == 0, D->s, D->red);
 	ggbk_fin(GGBK_FIN_PAP, D->s, D->red, D->nb0, 0, 0, 0);
 	ggbk_cg_fuse u = { psi, rin, pout, D->Ap };
-	if (D->cheb) {
-		ggbk_vcycle_cheb(D->levels, D->nlev, &u, NULL, rout, D->z, D->s, D->red);
-		ggbk_fin(GGBK_FIN_RZ, D->s, D->red, D->nb0, 0, 0, 0);
-		return;
-	}
 	if (ggbk_vcycle_update(D->levels, D->nlev, &u, rout, D->T, D->z, D->TM, D->s, D->red) == 0) {
 		ggbk_fin(GGBK_FIN_RZ, D->s, D->red, D->nb0, 0, 0, 0);
 		return;
@@ -1310,6 +1313,57 @@ static void read_scalars(struct ggb_fluid_dev *D)
 {
 	if (ggbk_d2h(D->hs, D->s, sizeof(ggbk_scalars)))
 		ggb_die("device error: %s", ggbk_last_error());
+}
+
+/* The mixed-precision PCG, the same iteration as the reference's (fluid.c:288-358) with the
+ * work regrouped so that each iteration is one V-cycle and one fold: the V-cycle of r_k
+ * also yields A z_k and the partials from which the fold forms rz, beta, p.Ap =
+ * z.Az + 2 beta z.Ap + beta^2 p.Ap (A symmetric), alpha and the residual norm after the
+ * coming update, ||r - alpha Ap'||^2; the next V-cycle's first pass applies that update
+ * (p' = z + beta p, Ap' = Az + beta Ap).  The stopping test thus sees each update's
+ * residual before the update is applied, and the V-cycle after the last one is skipped.
+ * Batches as in solve_on; a batch's last update is applied alone, and when the loop goes
+ * on its residual is then preconditioned without an update. */
+static void solve_cheb(struct ggb_fluid_dev *D, void *psi, int maxit)
+{
+	ggbk_cg_cheb c;
+	memset(&c, 0, sizeof(c));
+	double *ap[2] = { D->Ap, D->pbuf[1] };
+	c.psi = psi;
+	c.p_in = c.p_out = D->pbuf[0];
+	c.Az = D->Az;
+	c.z = D->z;
+	c.r = D->rbuf[0];
+	ggbk_vcycle_cheb(D->levels, D->nlev, &c, NULL, D->z, D->s, D->red);
+	ggbk_fin(GGBK_FIN_CHEB_FIRST, D->s, D->red, D->nbt, 0, 0, 0);
+	int batch = D->predict > 1 ? D->predict : 1;
+	int k = 0;
+	for (;;) {
+		for (int b = 0; b < batch && k < maxit; b++, k++) {
+			c.update = 1;
+			c.first = k == 0;
+			c.r = D->rbuf[k & 1];
+			c.r_out = D->rbuf[(k + 1) & 1];
+			c.Ap_in = ap[k & 1];
+			c.Ap_out = ap[(k + 1) & 1];
+			if (b == batch - 1 || k == maxit - 1) {
+				ggbk_cg_update_cheb(D->g0, &c, D->s);
+				ggbk_fin(GGBK_FIN_CHEB_END, D->s, D->red, 0, 0, 0, 0);
+			} else {
+				ggbk_vcycle_cheb(D->levels, D->nlev, &c, NULL, D->z, D->s, D->red);
+				ggbk_fin(GGBK_FIN_CHEB, D->s, D->red, D->nbt, 0, 0, 0);
+			}
+		}
+		read_scalars(D);
+		if (!D->hs->active || k >= maxit)
+			break;
+		c.update = 0;
+		c.r = D->rbuf[k & 1];
+		c.Ap_in = ap[k & 1];
+		ggbk_vcycle_cheb(D->levels, D->nlev, &c, NULL, D->z, D->s, D->red);
+		ggbk_fin(GGBK_FIN_CHEB, D->s, D->red, D->nbt, 0, 0, 0);
+		batch = batch < 64 ? 2 * batch : 64;
+	}
 }
 
 /* PCG for K psi = -area * omega on device arrays (reference fluid.c:288-358), psi
@@ -1328,6 +1382,10 @@ static int solve_on(Fluid *F, void *omega, void *psi, int maxit, int sub_mean)
 	}
 	ggbk_cg_start(g, omega, psi, D->rbuf[0], D->T, sub_mean, D->s, D->red);
 	ggbk_fin(GGBK_FIN_START, D->s, D->red, D->nb0, tol, maxit, 0);
+	if (D->cheb) {
+		solve_cheb(D, psi, maxit);
+		goto done;
+	}
 	precondition(D, D->rbuf[0], 1);
 	/* Iterations are enqueued in batches and the host reads the loop flag after each; the
 	 * first batch is the previous solve's count, later batches grow geometrically.  Passes
@@ -1342,6 +1400,7 @@ static int solve_on(Fluid *F, void *omega, void *psi, int maxit, int sub_mean)
 			break;
 		batch = batch < 64 ? 2 * batch : 64;
 	}
+done:
 	ggbk_cg_zero_if_bb0(g, psi, D->T, D->s);
 	F->cg_iters = D->hs->it;
 	F->cg_relres = D->hs->bb == 0.0 ? 0.0 : sqrt(D->hs->rr / D->hs->bb);
@@ -1451,12 +1510,25 @@ void fluid_step(Fluid *F, double dt)
 	ggbk_limited_relax(g, D->q2, D->q0, D->dep_b, D->q1, bfecc, D->omega_band, D->omega, T, twoOmega, relax,
 			   D->red);
 	ggbk_fin(mean_op(D), D->s, D->red, D->nb0, 0, 0, g.total_area);
-	if (D->psi_prev && !D->no_extrap) {
-		/* psi(t + dt) ~ psi(t) + dt / dt_prev (psi(t) - psi(t - dt_prev)) */
-		double c = D->prev_valid ? dt / D->dt_prev : 0.0;
-		ggbk_extrapolate(D->n, (double *) D->psi, D->psi_prev, c);
-		D->dt_prev = dt;
-		D->prev_valid = 1;
+	if (D->psi_prev[0] && !D->no_extrap) {
+		/* psi(t + dt) by the Lagrange polynomial through the last three solutions, at times
+		 * t, t - h1, t - h1 - h2 (fewer while the history fills up) */
+		double x = dt, h1 = D->dt_prev[0], h2 = D->dt_prev[1], c0 = 1.0, c1 = 0.0, c2 = 0.0;
+		if (D->prev_valid >= 2) {
+			c0 = (x + h1) * (x + h1 + h2) / (h1 * (h1 + h2));
+			c1 = -x * (x + h1 + h2) / (h1 * h2);
+			c2 = x * (x + h1) / ((h1 + h2) * h2);
+		} else if (D->prev_valid == 1) {
+			c0 = 1.0 + x / h1;
+			c1 = -x / h1;
+		}
+		ggbk_extrapolate(D->n, (double *) D->psi, D->psi_prev[0], D->psi_prev[1], c0, c1, c2);
+		double *t = D->psi_prev[0];
+		D->psi_prev[0] = D->psi_prev[1];
+		D->psi_prev[1] = t;
+		D->dt_prev[1] = D->dt_prev[0];
+		D->dt_prev[0] = dt;
+		D->prev_valid = D->prev_valid < 2 ? D->prev_valid + 1 : 2;
 	}
 	solve_device(F, F->prm.cg_maxit, 1);
 	psi_mean_and_velocity(F);
@@ -1520,11 +1592,12 @@ int fluid_vcycle_f64(Fluid *F, const double *b, double *x)
 	ggbk_scalars hs;
 	memset(&hs, 0, sizeof(hs));
 	hs.active = 1;
+	hs.vc = 1;
 	ggbk_scalars *ds = ggb_dmalloc(sizeof(ggbk_scalars), "vcycle");
 	upload(ds, &hs, sizeof(hs));
 	ggbk_red none = { NULL, 0 };
 	if (D->cheb)
-		ggbk_vcycle_cheb(D->levels, D->nlev, NULL, db, NULL, dz, ds, none);
+		ggbk_vcycle_cheb(D->levels, D->nlev, NULL, db, dz, ds, none);
 	else
 		ggbk_vcycle(D->levels, D->nlev, db, D->T, dz, D->TM, ds, none);
 	ggbk_convert(tmp, GGBK_F64, dz, D->TM, n);

@@ -73,12 +73,130 @@ __device__ __forceinline__ double fold(const double *p, int nb)
 __device__ __forceinline__ double fold_sum(const double *p, int nb) { return fold<false>(p, nb); }
 __device__ __forceinline__ double fold_max(const double *p, int nb) { return fold<true>(p, nb); }
 
+// K slots folded together, each in the same fixed order as fold_sum (thread t adds
+// p[t], p[t + FIN_BS], ... in turn): groups of G slots with all FOLD_BATCH loads of each in
+// flight at once, so a fold of up to FOLD_BATCH FIN_BS partials takes ceil(K / G) rounds
+// of memory latency.  Results valid in thread 0.
+template <int K, int G = 3>
+__device__ __forceinline__ void fold_sums(const double *p, size_t stride, const int (&slot)[K], int nb, double *out)
+{
+	__shared__ double wsum[K][FIN_BS / 64];
+	double acc[K];
+#pragma unroll
+	for (int k = 0; k < K; k++)
+		acc[k] = 0;
+	for (int i = threadIdx.x; i < nb; i += FOLD_BATCH * FIN_BS) {
+#pragma unroll
+		for (int k0 = 0; k0 < K; k0 += G) {
+			double v[G][FOLD_BATCH];
+#pragma unroll
+			for (int g = 0; g < G; g++)
+#pragma unroll
+				for (int q = 0; q < FOLD_BATCH; q++)
+					if (k0 + g < K)
+						v[g][q] = p[(size_t) slot[k0 + g] * stride + min(i + q * FIN_BS, nb - 1)];
+#pragma unroll
+			for (int g = 0; g < G; g++)
+#pragma unroll
+				for (int q = 0; q < FOLD_BATCH; q++)
+					if (k0 + g < K && i + q * FIN_BS < nb)
+						acc[k0 + g] += v[g][q];
+		}
+	}
+	int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+	for (int k = 0; k < K; k++) {
+		double v = wave_sum(acc[k]);
+		if (lane == 0)
+			wsum[k][wid] = v;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+#pragma unroll
+		for (int k = 0; k < K; k++) {
+			double r = 0;
+			for (int w = 0; w < FIN_BS / 64; w++)
+				r += wsum[k][w];
+			out[k] = r;
+		}
+	}
+}
+
 __global__ __launch_bounds__(FIN_BS) void k_fin(int op, ggbk_scalars *s, ggbk_red red, int nb, double tol, int maxit,
 						 double total_area)
 {
 	// the scalars an update reads are loaded together with the partials, and the loop
 	// flag is checked only after the fold, so the kernel waits for memory once
 	bool guarded = op == GGBK_FIN_PAP || op == GGBK_FIN_RR || op == GGBK_FIN_RZ_FIRST || op == GGBK_FIN_RZ;
+	if (op == GGBK_FIN_CHEB_END) {
+		if (threadIdx.x == 0 && s->active && s->stop) {
+			s->active = 0;
+			s->vc = 0;
+		}
+		return;
+	}
+	if (op == GGBK_FIN_CHEB_FIRST || op == GGBK_FIN_CHEB) {
+		// slots RZ, ZAZ .. APAP and RR (nine sums)
+		int act = s->active, stop = s->stop;
+		double rz_old = s->rz, pAp_old = s->pAp, tol2 = s->tol2;
+		int it = s->it, smaxit = s->maxit;
+		double v[9];
+		const int slots[9] = { GGBK_SL_RZ,   GGBK_SL_ZAZ,  GGBK_SL_ZAP, GGBK_SL_RAZ, GGBK_SL_AZAZ,
+				       GGBK_SL_AZAP, GGBK_SL_RAP,  GGBK_SL_APAP, GGBK_SL_RR };
+		fold_sums<9>(red.part, red.stride, slots, nb, v);
+		if (threadIdx.x != 0 || !act)
+			return;
+		if (stop) {
+			// the last update has been applied
+			s->active = 0;
+			s->vc = 0;
+			return;
+		}
+		// with Ap the previous direction's image: r.Ap and Ap.Ap are summed directly (v[6],
+		// v[7]); only p.Ap carries over from the previous fold
+		double rz = v[0], zAz = v[1], zAp = v[2], rAz = v[3], AzAz = v[4], AzAp = v[5], rAp0 = v[6], ApAp0 = v[7];
+		// r.r summed from r itself: the prediction below loses about two digits to
+		// cancellation, and chaining predictions would compound that each iteration
+		double rr = v[8];
+		double beta, pAp, rAp, ApAp;
+		if (op == GGBK_FIN_CHEB_FIRST) {
+			beta = 0.0;
+			pAp = zAz;
+			rAp = rAz;
+			ApAp = AzAz;
+		} else {
+			beta = rz / rz_old;
+			pAp = zAz + 2.0 * beta * zAp + beta * beta * pAp_old;
+			rAp = rAz + beta * rAp0;
+			ApAp = AzAz + 2.0 * beta * AzAp + beta * beta * ApAp0;
+		}
+		s->rz = rz;
+		s->beta = beta;
+		s->pAp = pAp;
+		if (pAp <= 0) {
+			s->status = 1;
+			s->active = 0;
+			s->vc = 0;
+			return;
+		}
+		double alpha = rz / pAp;
+		s->alpha = alpha;
+		// the residual after the coming update, r' = r - alpha Ap'
+		double rrn = fmax(rr - 2.0 * alpha * rAp + alpha * alpha * ApAp, 0.0);
+		s->rr = rrn;
+		it = it + 1;
+		s->it = it;
+		if (rrn <= tol2) {
+			s->stop = 1;
+			s->vc = 0;
+			s->status = 0;
+		} else if (it >= smaxit) {
+			s->stop = 1;
+			s->vc = 0;
+			s->status = 3;
+		}
+		return;
+	}
 	int act = s->active;
 	double rz = s->rz, tol2 = s->tol2;
 	int it = s->it, smaxit = s->maxit;
@@ -118,6 +236,7 @@ __global__ __launch_bounds__(FIN_BS) void k_fin(int op, ggbk_scalars *s, ggbk_re
 		s->it = 0;
 		s->maxit = maxit;
 		s->rz = 0;
+		s->stop = 0;
 		if (bb == 0.0) {
 			s->status = 2;
 			s->tol2 = 0;
@@ -127,6 +246,7 @@ __global__ __launch_bounds__(FIN_BS) void k_fin(int op, ggbk_scalars *s, ggbk_re
 			s->tol2 = tol * tol * bb;
 			s->active = (rr > s->tol2 && maxit > 0) ? 1 : 0;
 		}
+		s->vc = s->active;
 		break;
 	}
 	case GGBK_FIN_PAP: {
@@ -587,15 +707,16 @@ extern "C" void ggbk_remove_mean(ggbk_grid g, void *f, int T, ggbk_scalars *s, g
 }
 
 // ---------------- scratch of the entry point that allocates nothing ----------------
-// ggbk_remove_mean_noalloc's reduction scratch: the scalar block, then the per-block
-// partials of every slot, for grids of up to GGBK_NOALLOC_CELLS cells (1.4 MB).  Each GPU
+// ggbk_remove_mean_noalloc's reduction scratch: the scalar block, then one per-block
+// partial each, for grids of up to GGBK_NOALLOC_CELLS cells (0.2 MB; the pass and its fold
+// use the MEAN slot only, so the partials are addressed with slot stride 0).  Each GPU
 // holds its own copy for the life of the process; host code reaches it only through the
 // address hipGetSymbolAddress returned on that device (the symbol's name is a host-side
 // stand-in, not a device address).
 #define NOALLOC_BLOCKS ((GGBK_NOALLOC_CELLS + BS - 1) / BS)
 #define NOALLOC_HEAD 256
 static_assert(sizeof(ggbk_scalars) <= NOALLOC_HEAD, "scalar block exceeds its scratch slot");
-static __device__ __align__(256) unsigned char scratch[NOALLOC_HEAD + sizeof(double) * GGBK_NSLOTS * NOALLOC_BLOCKS];
+static __device__ __align__(256) unsigned char scratch[NOALLOC_HEAD + sizeof(double) * NOALLOC_BLOCKS];
 #define NOALLOC_MAX_DEVICES 64
 static void *g_scratch_dev[NOALLOC_MAX_DEVICES];
 
@@ -622,7 +743,7 @@ extern "C" int ggbk_remove_mean_noalloc(ggbk_grid g, float *f)
 	ggbk_scalars *s = (ggbk_scalars *) base;
 	ggbk_red red;
 	red.part = (double *) (base + NOALLOC_HEAD);
-	red.stride = (int) NB(g.n);
+	red.stride = 0; // the MEAN slot only: every slot at the same place
 	// every partial the fold reads is written first; the scalars start cleared
 	if (ggbk_zero(s, sizeof(ggbk_scalars)))
 		return -1;
@@ -690,21 +811,23 @@ extern "C" void ggbk_jacobi_precond(ggbk_grid g, const void *r, void *z, int T, 
 		GGBK_LAUNCH(k_jacobi_precond<float>, NB(g.n), BS, 0, g, (const float *) r, (float *) z, s, red);
 }
 
-// Linear extrapolation in time of the stream function, the warm start of the next solve:
-// psi_prev <- psi, psi <- psi + c (psi - psi_prev)
-__global__ __launch_bounds__(BS) void k_extrapolate(int n, double *psi, double *psi_prev, double c)
+// Extrapolation in time of the stream function, the warm start of the next solve:
+// psi <- c0 psi + c1 h1 + c2 h2 (h1, h2 the two previous steps' psi), and h2 <- the old psi
+// (the host then swaps h1 and h2, so h1 is always the newest)
+__global__ __launch_bounds__(BS) void k_extrapolate(int n, double *psi, const double *h1, double *h2, double c0,
+						    double c1, double c2)
 {
 	int i = xblock() * BS + threadIdx.x;
 	if (i >= n)
 		return;
-	double a = psi[i], b = psi_prev[i];
-	psi_prev[i] = a;
-	psi[i] = a + c * (a - b);
+	double a = psi[i], b = h1[i], c = h2[i];
+	h2[i] = a;
+	psi[i] = c0 * a + c1 * b + c2 * c;
 }
 
-extern "C" void ggbk_extrapolate(int n, double *psi, double *psi_prev, double c)
+extern "C" void ggbk_extrapolate(int n, double *psi, const double *h1, double *h2, double c0, double c1, double c2)
 {
-	GGBK_LAUNCH(k_extrapolate, NB(n), BS, 0, n, psi, psi_prev, c);
+	GGBK_LAUNCH(k_extrapolate, NB(n), BS, 0, n, psi, h1, h2, c0, c1, c2);
 }
 
 extern "C" void ggbk_cg_zero_if_bb0(ggbk_grid g, void *psi, int T, const ggbk_scalars *s)

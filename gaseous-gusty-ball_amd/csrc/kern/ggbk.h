@@ -67,8 +67,10 @@ typedef struct {
 	double vmax2;      /* max |vel|^2 of the last velocity pass */
 	double mean;       /* area-weighted mean of the last remove-mean */
 	int it, maxit;
-	int active;        /* CG loop running */
+	int active;        /* CG loop running (the next update is applied) */
 	int status;        /* 0 running/converged, 1 pAp <= 0, 2 bb == 0, 3 maxit */
+	int vc;            /* mixed-precision PCG: the passes after the next update are needed */
+	int stop;          /* mixed-precision PCG: the next update is the last */
 	unsigned int ticket[8];
 } ggbk_scalars;
 
@@ -133,7 +135,11 @@ typedef struct {
 	double *part;
 	int stride;
 } ggbk_red;
-enum { GGBK_SL_BB = 0, GGBK_SL_RR0, GGBK_SL_PAP, GGBK_SL_RR, GGBK_SL_RZ, GGBK_SL_MEAN, GGBK_SL_VMAX, GGBK_NSLOTS };
+enum { GGBK_SL_BB = 0, GGBK_SL_RR0, GGBK_SL_PAP, GGBK_SL_RR, GGBK_SL_RZ, GGBK_SL_MEAN, GGBK_SL_VMAX,
+       /* the mixed-precision PCG's fold after each V-cycle (z its output, r the residual it
+        * preconditioned, Ap the current search direction's image) */
+       GGBK_SL_ZAZ, GGBK_SL_ZAP, GGBK_SL_RAZ, GGBK_SL_AZAZ, GGBK_SL_AZAP, GGBK_SL_RAP, GGBK_SL_APAP,
+       GGBK_NSLOTS };
 enum {
 	GGBK_FIN_START = 0, /* bb, rr -> tol2, active, status */
 	GGBK_FIN_PAP,       /* pAp -> alpha, or stop */
@@ -143,6 +149,13 @@ enum {
 	GGBK_FIN_MEAN_F64,  /* mean = sum / total_area */
 	GGBK_FIN_MEAN_F32,  /* mean rounded to float */
 	GGBK_FIN_VMAX,      /* vmax2 = max */
+	/* mixed-precision PCG, after a V-cycle: rz, beta, pAp = z.Az + 2 beta z.Ap + beta^2
+	 * pAp, alpha, and the residual after the coming update, r.r - 2 alpha r.Ap' + alpha^2
+	 * Ap'.Ap' with Ap' = Az + beta Ap; the coming update is the last when that meets the
+	 * tolerance (or the iteration cap).  _FIRST: no previous direction (p = z). */
+	GGBK_FIN_CHEB_FIRST,
+	GGBK_FIN_CHEB,
+	GGBK_FIN_CHEB_END,  /* after an update applied alone: the loop ends if it was the last */
 };
 /* nb = number of blocks that wrote partials */
 void ggbk_fin(int op, ggbk_scalars *s, ggbk_red red, int nb, double tol, int maxit, double total_area);
@@ -172,8 +185,8 @@ void ggbk_cg_update(ggbk_grid g, void *psi, void *r, const void *p, const void *
 		    const ggbk_scalars *s, ggbk_red red);
 /* Jacobi preconditioner z = dinv r and the r.z partial (no multigrid) */
 void ggbk_jacobi_precond(ggbk_grid g, const void *r, void *z, int T, const ggbk_scalars *s, ggbk_red red);
-/* warm start by linear extrapolation in time: psi_prev <- psi, psi <- psi + c (psi - psi_prev) */
-void ggbk_extrapolate(int n, double *psi, double *psi_prev, double c);
+/* warm start by extrapolation in time: psi <- c0 psi + c1 h1 + c2 h2, h2 <- the old psi */
+void ggbk_extrapolate(int n, double *psi, const double *h1, double *h2, double c0, double c1, double c2);
 /* bb == 0 exit: psi = 0 when status says so */
 void ggbk_cg_zero_if_bb0(ggbk_grid g, void *psi, int T, const ggbk_scalars *s);
 
@@ -228,14 +241,34 @@ typedef struct {
 } ggbk_cg_fuse;
 int ggbk_vcycle_update(const ggbk_level *levels, int nlev, const ggbk_cg_fuse *u, void *r_out, int T,
 		       void *zout, int TM, ggbk_scalars *s, ggbk_red red);
-/* The mixed-precision preconditioner: a V-cycle with Chebyshev smoothing of degree
- * levels[l].tl.deg (1/2 <= lambda(D^-1 K) <= 2, the Gershgorin bound) on the tiled levels,
- * float storage and arithmetic, down to the first level with a dense operator.  r (FP64)
- * in, z (float) out with the r.z partial; with u != NULL the PCG update is fused into the
- * first pass as in ggbk_vcycle_update (r_out written, r.r partial, fold enqueued).
- * Needs levels[0..L-1] tiled and levels[L].M (L >= 1). */
-void ggbk_vcycle_cheb(const ggbk_level *levels, int nlev, const ggbk_cg_fuse *u, const void *r, void *r_out,
-		      void *zout, ggbk_scalars *s, ggbk_red red);
+/* The mixed-precision preconditioner: a V-cycle with Chebyshev smoothing (degree 2 on
+ * level 0, 3 below; 1/2 <= lambda(D^-1 K) <= 2, the Gershgorin bound) on the tiled levels,
+ * float storage and arithmetic, down to the first level with a dense operator (levels[0..L-1]
+ * tiled, levels[L].M set, L >= 1; level 0's up tiles have 3 rings).
+ * c == NULL: z = M r (stores gated by s->vc).  Otherwise the PCG's V-cycle: with c->update
+ * its first pass applies the update (the direction p = z + beta p, Ap = Az + beta Ap, first:
+ * p = z, Ap = Az; psi += alpha p; r_out = r - alpha Ap) and preconditions r_out, else it
+ * preconditions c->r; its last pass also stores A z (double) and the partials of
+ * GGBK_FIN_CHEB* (one per level-0 up tile).  The update's stores are gated by s->active,
+ * the V-cycle's by s->vc.  r_out must not alias r, nor Ap_out Ap_in. */
+typedef struct {
+	int update;                   /* apply the PCG update in the first pass */
+	int first;                    /* the update is the first: p = z, Ap = Az */
+	double *psi;
+	const double *r;              /* the residual before the update (update) or to precondition */
+	double *r_out;                /* update: r - alpha Ap */
+	const double *p_in;           /* the previous direction (may alias p_out) */
+	double *p_out;
+	const double *Ap_in;          /* A p of the previous direction (NULL before the first) */
+	double *Ap_out;
+	double *Az;                   /* in: the previous V-cycle's A z (update); out: this one's */
+	const float *z;               /* the previous V-cycle's z (update) */
+} ggbk_cg_cheb;
+/* The PCG update alone, as ggbk_vcycle_cheb's first pass applies it: for the update the
+ * host expects to be the last of the solve. */
+void ggbk_cg_update_cheb(ggbk_grid g, const ggbk_cg_cheb *c, const ggbk_scalars *s);
+void ggbk_vcycle_cheb(const ggbk_level *levels, int nlev, const ggbk_cg_cheb *c, const void *r, void *zout,
+		      ggbk_scalars *s, ggbk_red red);
 
 /* ---- velocity (reference fluid.c:360-376) + max speed (fluid.c:480-491) ---- */
 /* [psi_out = psi_in - mean] vel = p x grad(psi_out); vmax partial (slot VMAX) */

@@ -321,10 +321,12 @@ __global__ __launch_bounds__(BS) void k_mg_smooth(ggbk_grid g, const TB *b, cons
 // DENSE_UNROLL at a time), then the workgroup tree; a fixed order for a given n.  A row per
 // workgroup keeps enough rows in flight to cover the load latency of this small pass.
 #define DENSE_UNROLL 8
-template <typename TM, typename TD, typename TB>
+template <typename TM, typename TD, typename TB, bool VC = false>
 __global__ __launch_bounds__(DENSE_BS) void k_mg_dense(const TD *M, int n, const TB *b, TM *x, const ggbk_scalars *s)
 {
-	int act = s->active;
+	int act = VC ? s->vc : s->active;
+	if (VC && !act)
+		return;
 	int row = blockIdx.x;
 	const TD *m = M + (size_t) row * n;
 	double acc = 0;
@@ -535,16 +537,8 @@ extern "C" int ggbk_vcycle_update(const ggbk_level *levels, int nlev, const ggbk
 	return 0;
 }
 
-// float restriction and dense bottom for the mixed-precision V-cycle (mgt_k.hip)
-extern "C" void ggbk_mg_restrict_f32(ggbk_xfer xf, const float *rf, float *bc, const ggbk_scalars *s)
+// The dense bottom of the mixed-precision V-cycle, whose stores s->vc gates
+extern "C" void ggbk_mg_dense_vc_f32(const float *M, int n, const float *b, float *x, const ggbk_scalars *s)
 {
-	if (xf.rpad == 18)
-		GGBK_LAUNCH((k_mg_restrict<float, 18>), NB(xf.nc), BS, 0, xf, rf, bc, s);
-	else
-		GGBK_LAUNCH((k_mg_restrict<float, GGBK_RMAX>), NB(xf.nc), BS, 0, xf, rf, bc, s);
-}
-
-extern "C" void ggbk_mg_dense_f32(const float *M, int n, const float *b, float *x, const ggbk_scalars *s)
-{
-	GGBK_LAUNCH((k_mg_dense<float, float, float>), (unsigned) n, DENSE_BS, 0, M, n, b, x, s);
+	GGBK_LAUNCH((k_mg_dense<float, float, float, true>), (unsigned) n, DENSE_BS, 0, M, n, b, x, s);
 }

@@ -18,14 +18,18 @@
 // between its barriers.  Per level:
 //   down   x = D steps from 0 on b, r = b - K x; stores r (for the restriction) and x;
 //   up     x = x_pre + P x_coarse, then D steps; stores x.
-// Level 0's down pass also applies the PCG update (psi += alpha p, r_out = r_in - alpha Ap,
-// the r.r partial) and its up pass writes z and the r.z partial.
+// In the PCG, level 0's down pass also applies the update (p = z + beta p, Ap = Az + beta Ap,
+// psi += alpha p, r_out = r_in - alpha Ap) and its up pass, one ring wider, also forms A z
+// in double from the float z and the partials of the fold that yields the next alpha, so
+// the solve needs no separate A p pass.  The passes after the last update are skipped
+// (s->vc).
 //
 // Tiles.  An interior tile's rings lie on its face and every stencil there uses the regular
 // offset codes, so its ext cells form a (16 + 2D)^2 square addressed by arithmetic.  A tile
 // near a face seam reads host-built tables: ext cells ring by ring (BFS over the stencil
 // graph) and each stencil neighbour's slot in that list.  Storage and arithmetic are float.
 #include <hip/hip_runtime.h>
+#include <string.h>
 
 #include "dev.h"
 #include "ggbk.h"
@@ -212,24 +216,26 @@ __device__ __forceinline__ int tile_setup(const ggbk_grid &g, const ggbk_tiles &
 // around (D + 1 rings of an interior tile's square, D around a graph tile's support), the
 // fine residual r = b - K x on that support, and each coarse cell's b = sum of its row of
 // R times r (float, in row order).  x is stored for the fine tile (the up pass starts from
-// it).  UPD: level 0 of a PCG iteration, where b = r_in - alpha Ap is the updated CG
-// residual (stored as r_out on the tile, with psi += alpha p and the r.r partial).
+// it).  UPD: level 0 of a PCG iteration, where b = r - alpha Ap' is the updated CG
+// residual for the new direction p' = z + beta p, Ap' = Az + beta Ap (first: z, Az); the
+// tile stores r_out = b, p', Ap' and psi += alpha p' (when s->active).  The smoothing's
+// stores are skipped when no V-cycle follows the update (s->vc == 0).
 template <int D, typename TB, bool UPD, int PER>
 __global__ __launch_bounds__(MGT_BS) void k_mgt_down(ggbk_grid g, ggbk_tiles tl, ggbk_xfer xf, const TB *b, float *bc,
-						       float *xpre, double *psi, const double *r_in, double *r_out,
-						       const double *p, const double *Ap, const ggbk_scalars *s,
-						       ggbk_red red)
+						       float *xpre, ggbk_cg_cheb cg, const ggbk_scalars *s)
 {
 	constexpr int CAP = PER * MGT_BS, HS = D + 1, W = MGT_T + 2 * HS;
 	__shared__ float xa[CAP], xb[CAP];
-	int act = s->active;
-	double alpha = UPD ? s->alpha : 0.0;
+	int upd = s->active, act = s->vc; // the update; the smoothing after it
+	if (!(UPD && upd) && !act)
+		return;
+	double alpha = UPD ? s->alpha : 0.0, beta = UPD ? s->beta : 0.0;
+	bool first = cg.first;
 	int tile = xblock(), kind, off, ring_end[D + 1];
 	int next = tile_setup<HS, D>(g, tl, tile, kind, off, ring_end);
 	int hk = kind < 0 ? HS : D; // rings of this tile's ext set
 	ExtCell xc[PER];
 	float bv[PER], dv[PER];
-	double rr = 0;
 #pragma unroll
 	for (int m = 0; m < PER; m++) {
 		int e = threadIdx.x + m * MGT_BS;
@@ -241,14 +247,18 @@ __global__ __launch_bounds__(MGT_BS) void k_mgt_down(ggbk_grid g, ggbk_tiles tl,
 			bool own = kind < 0 ? xc[m].ring == 0 : e < MGT_T * MGT_T;
 			double bd;
 			if (UPD) {
-				bd = r_in[c] - alpha * Ap[c];
+				// p = z + beta p, Ap = Az + beta Ap (first: p = z, Ap = Az); r -= alpha Ap
+				double apv = first ? cg.Az[c] : cg.Az[c] + beta * cg.Ap_in[c];
+				bd = cg.r[c] - alpha * apv;
 				if (own) {
-					double pv = psi[c] + alpha * p[c];
-					if (act) {
-						psi[c] = pv;
-						r_out[c] = bd;
+					double pv = first ? (double) cg.z[c] : (double) cg.z[c] + beta * cg.p_in[c];
+					double ps = cg.psi[c] + alpha * pv;
+					if (upd) {
+						cg.psi[c] = ps;
+						cg.p_out[c] = pv;
+						cg.Ap_out[c] = apv;
+						cg.r_out[c] = bd;
 					}
-					rr += bd * bd;
 				}
 			} else {
 				bd = (double) b[c];
@@ -258,6 +268,8 @@ __global__ __launch_bounds__(MGT_BS) void k_mgt_down(ggbk_grid g, ggbk_tiles tl,
 			xa[xc[m].slot] = dv[m];
 		}
 	}
+	if (!act)
+		return; // the update was the last: no V-cycle follows
 	__syncthreads();
 	float *xs = xa, *xn = xb;
 #pragma unroll
@@ -317,10 +329,25 @@ __global__ __launch_bounds__(MGT_BS) void k_mgt_down(ggbk_grid g, ggbk_tiles tl,
 		if (act)
 			bc[cc] = acc;
 	}
-	if (UPD && act) {
-		double bsum = block_sum<MGT_BS>(rr);
-		if (threadIdx.x == 0)
-			red.part[(size_t) GGBK_SL_RR * red.stride + tile] = bsum;
+}
+
+// The PCG update alone (ggbk_cg_update_cheb): the first pass's update without the
+// smoothing, for the iteration the host expects to be the last of the solve.
+__global__ __launch_bounds__(MGT_BS) void k_cg_update_cheb(int n, ggbk_cg_cheb cg, const ggbk_scalars *s)
+{
+	int upd = s->active;
+	double alpha = s->alpha, beta = s->beta;
+	int c = xblock() * MGT_BS + threadIdx.x;
+	if (c >= n || !upd)
+		return;
+	double apv = cg.first ? cg.Az[c] : cg.Az[c] + beta * cg.Ap_in[c];
+	double pv = cg.first ? (double) cg.z[c] : (double) cg.z[c] + beta * cg.p_in[c];
+	double rv = cg.r[c] - alpha * apv, ps = cg.psi[c] + alpha * pv;
+	if (upd) {
+		cg.psi[c] = ps;
+		cg.p_out[c] = pv;
+		cg.Ap_out[c] = apv;
+		cg.r_out[c] = rv;
 	}
 }
 
@@ -345,31 +372,28 @@ __device__ __forceinline__ float prolong_at(const ggbk_xfer &xf, const float *xc
 }
 
 // Post-smoothing pass (up): x = x_pre + P x_coarse, then D Chebyshev steps on b.
-// RZ: level 0, where the result is z and the r.z partial is stored.
-template <int D, typename TB, bool RZ>
-__global__ __launch_bounds__(MGT_BS) void k_mgt_up(ggbk_grid g, ggbk_tiles tl, ggbk_xfer xf, const TB *b,
+template <int D>
+__global__ __launch_bounds__(MGT_BS) void k_mgt_up(ggbk_grid g, ggbk_tiles tl, ggbk_xfer xf, const float *b,
 						     const float *xpre, const float *xcoarse, float *xout,
-						     const ggbk_scalars *s, ggbk_red red)
+						     const ggbk_scalars *s)
 {
 	__shared__ float xa[MGT_CAP], xb[MGT_CAP];
-	int act = s->active;
+	int act = s->vc;
+	if (!act)
+		return;
 	int tile = xblock(), kind, off, ring_end[D + 1];
 	int next = tile_setup<D, D>(g, tl, tile, kind, off, ring_end);
 	ExtCell xc[MGT_PER];
 	float bv[MGT_PER], dv[MGT_PER];
-	double bz[MGT_PER];
 #pragma unroll
 	for (int m = 0; m < MGT_PER; m++) {
 		ext_cell<D, D>(g, tl, tile, kind, off, threadIdx.x + m * MGT_BS, next, ring_end, xc[m]);
 		bv[m] = 0.0f;
 		dv[m] = 0.0f;
-		bz[m] = 0.0;
 		if (xc[m].c >= 0) {
 			int c = xc[m].c;
-			if (xc[m].ring < D) {
-				bz[m] = (double) b[c];
-				bv[m] = (float) bz[m];
-			}
+			if (xc[m].ring < D)
+				bv[m] = b[c];
 			xa[xc[m].slot] = xpre[c] + prolong_at(xf, xcoarse, c);
 		}
 	}
@@ -387,8 +411,6 @@ __global__ __launch_bounds__(MGT_BS) void k_mgt_up(ggbk_grid g, ggbk_tiles tl, g
 					xn[xc[m].slot] = xv;
 				else if (act)
 					xout[xc[m].c] = xv;
-				if (k == D - 1)
-					bz[m] *= (double) xv;
 			}
 		}
 		if (k < D - 1) {
@@ -398,73 +420,184 @@ __global__ __launch_bounds__(MGT_BS) void k_mgt_up(ggbk_grid g, ggbk_tiles tl, g
 			xn = t;
 		}
 	}
-	if (RZ && act && red.part) {
-		double v = 0;
+}
+
+// Level 0's post-smoothing pass, one ring wider than k_mgt_up: x = x_pre + P x_coarse on
+// D + 1 rings, D Chebyshev steps on b (the last on rings <= 1), z stored on the tile.  With
+// Az != NULL (the PCG) also A z on the tile in double from the float z (stencil slots in
+// order, as the FP64 operator sums them) and the fold's partials over the tile: r.z,
+// z.Az, z.Ap, r.Az, Az.Az, Az.Ap, r.Ap, Ap.Ap and r.r (slots RZ, ZAZ .. APAP, RR), Ap the
+// current direction's image (zero when NULL), r = b.
+template <int D>
+__global__ __launch_bounds__(MGT_BS) void k_mgt_up_cg(ggbk_grid g, ggbk_tiles tl, ggbk_xfer xf, const double *b,
+							const float *xpre, const float *xcoarse, float *zout, double *Az,
+							const double *Ap, const ggbk_scalars *s, ggbk_red red)
+{
+	constexpr int H = D + 1;
+	__shared__ float xa[MGT_CAP], xb[MGT_CAP];
+	__shared__ double wsum[9][MGT_BS / 64];
+	int act = s->vc;
+	if (!act)
+		return;
+	int tile = xblock(), kind, off, ring_end[H + 1];
+	int next = tile_setup<H, H>(g, tl, tile, kind, off, ring_end);
+	ExtCell xc[MGT_PER];
+	float bv[MGT_PER], dv[MGT_PER];
+	double rd[MGT_PER];
 #pragma unroll
-		for (int m = 0; m < MGT_PER; m++)
-			if (xc[m].c >= 0 && xc[m].ring == 0)
-				v += bz[m];
-		double bsum = block_sum<MGT_BS>(v);
-		if (threadIdx.x == 0)
-			red.part[(size_t) GGBK_SL_RZ * red.stride + tile] = bsum;
+	for (int m = 0; m < MGT_PER; m++) {
+		ext_cell<H, H>(g, tl, tile, kind, off, threadIdx.x + m * MGT_BS, next, ring_end, xc[m]);
+		bv[m] = 0.0f;
+		dv[m] = 0.0f;
+		rd[m] = 0.0;
+		if (xc[m].c >= 0) {
+			int c = xc[m].c;
+			if (xc[m].ring < H) {
+				rd[m] = b[c];
+				bv[m] = (float) rd[m];
+			}
+			xa[xc[m].slot] = xpre[c] + prolong_at(xf, xcoarse, c);
+		}
+	}
+	__syncthreads();
+	float *xs = xa, *xn = xb;
+#pragma unroll
+	for (int k = 0; k < D; k++) {
+#pragma unroll
+		for (int m = 0; m < MGT_PER; m++) {
+			if (xc[m].c >= 0 && xc[m].ring <= H - 1 - k) {
+				float res = bv[m] - kx_lds(xc[m], xs);
+				dv[m] = cheb_cd(k) * dv[m] + cheb_cr(k) * xc[m].dinv * res;
+				xn[xc[m].slot] = xs[xc[m].slot] + dv[m];
+			}
+		}
+		__syncthreads();
+		float *t = xs;
+		xs = xn;
+		xn = t;
+	}
+	// z is in xs on rings <= 1
+	double v[9];
+#pragma unroll
+	for (int q = 0; q < 9; q++)
+		v[q] = 0.0;
+#pragma unroll
+	for (int m = 0; m < MGT_PER; m++) {
+		if (xc[m].c >= 0 && xc[m].ring == 0) {
+			int c = xc[m].c;
+			float zf = xs[xc[m].slot];
+			if (act)
+				zout[c] = zf;
+			if (Az) {
+				double zi = (double) zf, az = 0.0;
+#pragma unroll
+				for (int k = 0; k < 8; k++)
+					az += (double) xc[m].w[k] * (zi - (double) xs[xc[m].nb[k]]);
+				double ap = Ap ? Ap[c] : 0.0, r = rd[m];
+				if (act)
+					Az[c] = az;
+				v[0] += r * zi;
+				v[1] += zi * az;
+				v[2] += zi * ap;
+				v[3] += r * az;
+				v[4] += az * az;
+				v[5] += az * ap;
+				v[6] += r * ap;
+				v[7] += ap * ap;
+				v[8] += r * r;
+			}
+		}
+	}
+	if (Az && act && red.part) {
+		// nine block sums in a fixed order, one barrier
+		int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+		for (int q = 0; q < 9; q++) {
+			double t = wave_sum(v[q]);
+			if (lane == 0)
+				wsum[q][wid] = t;
+		}
+		__syncthreads();
+		if (threadIdx.x < 9) {
+			int q = threadIdx.x;
+			double t = 0.0;
+#pragma unroll
+			for (int w = 0; w < MGT_BS / 64; w++)
+				t += wsum[q][w];
+			int slot = q == 0 ? GGBK_SL_RZ : q == 8 ? GGBK_SL_RR : GGBK_SL_ZAZ + q - 1;
+			red.part[(size_t) slot * red.stride + tile] = t;
+		}
 	}
 }
 
-// Dense bottom (mg_k.hip)
-extern "C" void ggbk_mg_dense_f32(const float *M, int n, const float *b, float *x, const ggbk_scalars *s);
+// Dense bottom (mg_k.hip), gated by s->vc
+extern "C" void ggbk_mg_dense_vc_f32(const float *M, int n, const float *b, float *x, const ggbk_scalars *s);
 
 // down pass of level L into level C's b
 template <int D, int PER>
 static void down_level(const ggbk_level &L, const ggbk_level &C, const float *b, ggbk_scalars *s)
 {
-	ggbk_red none = { nullptr, 0 };
+	ggbk_cg_cheb none;
+	memset(&none, 0, sizeof(none));
 	GGBK_LAUNCH((k_mgt_down<D, float, false, PER>), L.td.ntiles, MGT_BS, 0, L.g, L.td, L.xf, b, (float *) C.b,
-		    (float *) L.xpre, nullptr, nullptr, nullptr, nullptr, nullptr, s, none);
+		    (float *) L.xpre, none, s);
 }
 
 template <int D>
 static void up_level(const ggbk_level &L, const float *b, const float *xc, float *x, ggbk_scalars *s)
 {
-	ggbk_red none = { nullptr, 0 };
-	GGBK_LAUNCH((k_mgt_up<D, float, false>), L.tl.ntiles, MGT_BS, 0, L.g, L.tl, L.xf, b, (const float *) L.xpre, xc,
-		    x, s, none);
+	GGBK_LAUNCH((k_mgt_up<D>), L.tl.ntiles, MGT_BS, 0, L.g, L.tl, L.xf, b, (const float *) L.xpre, xc, x, s);
 }
 
-extern "C" void ggbk_vcycle_cheb(const ggbk_level *lv, int nlev, const ggbk_cg_fuse *u, const void *r, void *r_out,
-				 void *zout, ggbk_scalars *s, ggbk_red red)
+extern "C" void ggbk_cg_update_cheb(ggbk_grid g, const ggbk_cg_cheb *c, const ggbk_scalars *s)
 {
+	ggbk_probe_begin(GGBK_PROBE_CG_UPDATE);
+	GGBK_LAUNCH(k_cg_update_cheb, (unsigned) ((g.n + MGT_BS - 1) / MGT_BS), MGT_BS, 0, g.n, *c, s);
+	ggbk_probe_end(GGBK_PROBE_CG_UPDATE);
+}
+
+extern "C" void ggbk_vcycle_cheb(const ggbk_level *lv, int nlev, const ggbk_cg_cheb *c, const void *r, void *zout,
+				 ggbk_scalars *s, ggbk_red red)
+{
+	(void) nlev;
 	const ggbk_level &L0 = lv[0];
-	// level 0 down (degree 2) into level 1, fused with the PCG update when u is given
-	if (u) {
+	ggbk_cg_cheb cg;
+	memset(&cg, 0, sizeof(cg));
+	if (c)
+		cg = *c;
+	else
+		cg.r = (const double *) r;
+	// level 0 down (degree 2) into level 1, fused with the PCG update when asked
+	const double *b0;
+	if (cg.update) {
 		ggbk_probe_begin(GGBK_PROBE_CG_UPDATE);
 		GGBK_LAUNCH((k_mgt_down<2, double, true, 2>), L0.td.ntiles, MGT_BS, 0, L0.g, L0.td, L0.xf,
-			    (const double *) nullptr, (float *) lv[1].b, (float *) L0.xpre, (double *) u->psi,
-			    (const double *) u->r_in, (double *) r_out, (const double *) u->p, (const double *) u->Ap, s,
-			    red);
+			    (const double *) nullptr, (float *) lv[1].b, (float *) L0.xpre, cg, s);
 		ggbk_probe_end(GGBK_PROBE_CG_UPDATE);
-		ggbk_fin(GGBK_FIN_RR, s, red, L0.td.ntiles, 0, 0, 0);
+		b0 = cg.r_out;
 	} else {
 		ggbk_probe_begin(GGBK_PROBE_MG_DOWN0);
-		ggbk_red none = { nullptr, 0 };
-		GGBK_LAUNCH((k_mgt_down<2, double, false, 2>), L0.td.ntiles, MGT_BS, 0, L0.g, L0.td, L0.xf, (const double *) r,
-			    (float *) lv[1].b, (float *) L0.xpre, nullptr, nullptr, nullptr, nullptr, nullptr, s, none);
+		GGBK_LAUNCH((k_mgt_down<2, double, false, 2>), L0.td.ntiles, MGT_BS, 0, L0.g, L0.td, L0.xf, cg.r,
+			    (float *) lv[1].b, (float *) L0.xpre, cg, s);
 		ggbk_probe_end(GGBK_PROBE_MG_DOWN0);
+		b0 = cg.r;
 	}
-	const double *b0 = (const double *) (u ? r_out : r);
 	// smooth and restrict down to the dense level
 	int l = 1;
 	while (!lv[l].M) {
 		down_level<3, 3>(lv[l], lv[l + 1], (const float *) lv[l].b, s);
 		l++;
 	}
-	ggbk_mg_dense_f32((const float *) lv[l].M, lv[l].g.n, (const float *) lv[l].b, (float *) lv[l].x, s);
+	ggbk_mg_dense_vc_f32((const float *) lv[l].M, lv[l].g.n, (const float *) lv[l].b, (float *) lv[l].x, s);
 	// back up
 	for (l = l - 1; l >= 1; l--) {
 		const ggbk_level &L = lv[l];
 		up_level<3>(L, (const float *) L.b, (const float *) lv[l + 1].x, (float *) L.x, s);
 	}
+	// the current direction's image: just formed by the update, else the caller's
+	const double *ap = cg.update ? cg.Ap_out : cg.Ap_in;
 	ggbk_probe_begin(GGBK_PROBE_MG_UP0);
-	GGBK_LAUNCH((k_mgt_up<2, double, true>), L0.tl.ntiles, MGT_BS, 0, L0.g, L0.tl, L0.xf, b0, (const float *) L0.xpre,
-		    (const float *) lv[1].x, (float *) zout, s, red);
+	GGBK_LAUNCH((k_mgt_up_cg<2>), L0.tl.ntiles, MGT_BS, 0, L0.g, L0.tl, L0.xf, b0, (const float *) L0.xpre,
+		    (const float *) lv[1].x, (float *) zout, c ? cg.Az : nullptr, ap, s, c ? red : ggbk_red{ nullptr, 0 });
 	ggbk_probe_end(GGBK_PROBE_MG_UP0);
 }
