@@ -430,9 +430,13 @@ static int transfer_is_analytic(const CubeSphere *fine, const CubeSphere *coarse
 
 /* Bilinear transfer from the coarse grid to the fine grid: taps of every fine centre on
  * the coarse grid, and R = P^T as CSR with fine indices ascending (fluid.c:64-96).  The
- * host tables are left in *hl. */
+ * host tables are left in *hl.  With exact, the face-interior cells (whose taps follow the
+ * bilinear pattern) take the exact bilinear weights in equiangular coordinates, products
+ * of 1/4 and 3/4 by the cell's parity, instead of those evaluated at the float-rounded
+ * cell centres (which differ by up to ~1e-5): the mixed-precision preconditioner's own
+ * transfer, whose device passes compute these weights instead of reading them. */
 static ggbk_xfer build_transfer(struct ggb_fluid_dev *D, const CubeSphere *fine, const CubeSphere *coarse,
-				HostLevel *hl)
+				HostLevel *hl, int exact)
 {
 	int nf = fine->ncells, nc = coarse->ncells;
 	int *ptap = malloc(sizeof(int) * 4 * (size_t) nf);
@@ -451,6 +455,24 @@ static ggbk_xfer build_transfer(struct ggb_fluid_dev *D, const CubeSphere *fine,
 		for (int k = 0; k < 4; k++) {
 			ptap[4 * i + k] = idx[k];
 			pw[4 * i + k] = (float) wt[k];
+		}
+	}
+	if (exact && fine->N == 2 * coarse->N) {
+		int Nf = fine->N, Nc = coarse->N;
+		#pragma omp parallel for schedule(static)
+		for (int i = 0; i < nf; i++) {
+			int f = i / (Nf * Nf), y = (i / Nf) % Nf, x = i % Nf;
+			if (x < 1 || x > Nf - 2 || y < 1 || y > Nf - 2)
+				continue;
+			int b = (f * Nc + ((y - 1) >> 1)) * Nc + ((x - 1) >> 1);
+			const int *t = ptap + 4 * (size_t) i;
+			if (t[0] != b || t[1] != b + 1 || t[2] != b + Nc || t[3] != b + Nc + 1)
+				continue;
+			double ax = (x & 1) ? 0.75 : 0.25, ay = (y & 1) ? 0.75 : 0.25;
+			pw[4 * (size_t) i + 0] = (float) (ax * ay);
+			pw[4 * (size_t) i + 1] = (float) ((1.0 - ax) * ay);
+			pw[4 * (size_t) i + 2] = (float) (ax * (1.0 - ay));
+			pw[4 * (size_t) i + 3] = (float) ((1.0 - ax) * (1.0 - ay));
 		}
 	}
 	for (int i = 0; i < 4 * nf; i++)
@@ -499,15 +521,19 @@ static ggbk_xfer build_transfer(struct ggb_fluid_dev *D, const CubeSphere *fine,
 	/* computed indices pay off where the tables stream from HBM; the small levels' tables
 	 * stay in L2, and there the index arithmetic costs more than the loads it saves */
 	x.rana = transfer_is_analytic(fine, coarse, ptap, rstart, ridx, rw, pw);
-	x.analytic = nc >= 65536 && x.rana;
+	/* with exact weights nothing is read for the face-interior cells: computed at every size */
+	x.exact = exact && x.rana;
+	x.analytic = x.rana && (x.exact || nc >= 65536);
 	void *d;
 	d = xbuf(D, sizeof(int) * 8 * (size_t) nf, "transfer");
 	upload(d, pt, sizeof(int) * 8 * (size_t) nf);
 	x.pt = d;
-	d = xbuf(D, sizeof(float) * 4 * (size_t) nf, "transfer");
-	upload(d, pw, sizeof(float) * 4 * (size_t) nf);
-	x.pw = d;
-	if (x.rana) {
+	if (!x.exact) {
+		d = xbuf(D, sizeof(float) * 4 * (size_t) nf, "transfer");
+		upload(d, pw, sizeof(float) * 4 * (size_t) nf);
+		x.pw = d;
+	}
+	if (x.rana && !x.exact) {
 		/* the weights of the face-interior rows alone, coalesced per entry group */
 		float *w16 = calloc(16 * (size_t) nc, sizeof(float));
 		if (!w16)
@@ -929,23 +955,36 @@ static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, c
 	return rc;
 }
 
+/* The level whose sub-V-cycle is tabulated as a dense operator: the first coarse level of
+ * at most DENSE_MAX cells (none in reference-precision mode, which keeps the reference's
+ * float sweeps bit for bit), or -1. */
+static int dense_level_of(int prec, int N, int nlev)
+{
+	if (prec == GGB_PREC_REF32)
+		return -1;
+	for (int l = 1; l < nlev; l++)
+		if (6 * (N >> l) * (N >> l) <= DENSE_MAX)
+			return l;
+	return -1;
+}
+
 /* The mixed-precision preconditioner (degree 2 on level 0, 3 below) needs a dense level
  * below level 0 and every level above it a multiple of 16 cells wide; GGB_MIXED_CHEB=0 in
  * the environment selects the reference V(1,1) cycle instead.  setup_cheb builds the
  * tiles of those levels. */
-static int cheb_eligible(const struct ggb_fluid_dev *D, const CubeSphere *cs)
+static int cheb_eligible(int prec, int N, int nlev)
 {
+	if (prec != GGB_PREC_MIXED)
+		return 0;
 	const char *e = getenv("GGB_MIXED_CHEB");
 	if (e && !strcmp(e, "0"))
 		return 0;
-	int L = D->dense_level;
+	int L = dense_level_of(prec, N, nlev);
 	if (L < 1)
 		return 0;
-	for (int l = 0; l < L; l++) {
-		const CubeSphere *c = l == 0 ? cs : &D->coarse[l];
-		if (c->N % MGT_TILE)
+	for (int l = 0; l < L; l++)
+		if ((N >> l) % MGT_TILE)
 			return 0;
-	}
 	return 1;
 }
 
@@ -1048,13 +1087,14 @@ int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
 		D->levels[0].g = D->g0;
 		D->levels[0].r = xbuf(D, D->tmsz * n, "multigrid");
 		D->hl[0].cs = cs;
+		int cheb = cheb_eligible(D->prec, cs->N, D->nlev);
 		for (int l = 1; l < D->nlev; l++) {
 			if (cs_init(&D->coarse[l], cs->N >> l))
 				ggb_die("fluid_init: coarse grid allocation failed");
 			const CubeSphere *fine = l == 1 ? cs : &D->coarse[l - 1];
 			D->hl[l].cs = &D->coarse[l];
 			D->levels[l].g = ggb_grid_dev(&D->coarse[l]);
-			D->levels[l - 1].xf = build_transfer(D, fine, &D->coarse[l], &D->hl[l - 1]);
+			D->levels[l - 1].xf = build_transfer(D, fine, &D->coarse[l], &D->hl[l - 1], cheb);
 			size_t nl = D->coarse[l].ncells;
 			D->levels[l].x = xbuf(D, D->tmsz * nl, "multigrid");
 			D->levels[l].b = xbuf(D, D->tmsz * nl, "multigrid");
@@ -1073,14 +1113,9 @@ int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
 				D->hl[l].dinv[i] = (float) (1.0 / sw);
 			}
 		}
-		/* the bottom of the V-cycle as one dense operator (not in reference-precision
-		 * mode, which keeps the reference's float sweeps bit for bit) */
-		int L = D->prec == GGB_PREC_REF32 ? D->nlev : -1;
-		for (int l = 1; l < D->nlev && L < 0; l++)
-			if (D->coarse[l].ncells <= DENSE_MAX)
-				L = l;
-		D->dense_level = L > 0 && L < D->nlev ? L : -1;
-		int cheb = D->prec == GGB_PREC_MIXED && cheb_eligible(D, cs);
+		/* the bottom of the V-cycle as one dense operator */
+		int L = dense_level_of(D->prec, cs->N, D->nlev);
+		D->dense_level = L;
 		if (D->dense_level > 0)
 			build_dense(D, L, cheb ? 3 : 1);
 		if (cheb) {

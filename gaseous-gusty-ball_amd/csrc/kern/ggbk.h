@@ -47,7 +47,10 @@ typedef struct {
 	int nf, nc, rmax, rpad;  /* rpad: padded row width, 18 or GGBK_RMAX */
 	int Nc, analytic;  /* coarse face size; indices of face-interior rows follow the
 	                    * bilinear pattern (only the weights are read for those) */
-	int rana, pad_;    /* the pattern holds (rw16 is then tabulated), whatever the size */
+	int rana;          /* the pattern holds (rw16 is then tabulated), whatever the size */
+	int exact;         /* the pattern holds and the face-interior weights are the exact
+	                    * bilinear ones, products of 1/4 and 3/4 by the cells' parity: they are
+	                    * computed, and pw / rw16 are not stored (NULL) */
 	const float *pw;   /* [nf][4] weights of each fine cell's taps */
 	const float *rw16; /* [4][nc][4] weights of the 16 entries of face-interior rows of R, in
 	                    * row order (entry e in element e / 4, lane e % 4) */

@@ -307,13 +307,20 @@ __global__ __launch_bounds__(MGT_BS) void k_mgt_down(ggbk_grid g, ggbk_tiles tl,
 		float acc = 0.0f;
 		if (kind < 0) {
 			// the 4x4 fine block x in [2I - 1, 2I + 2], y in [2J - 1, 2J + 2], row-major,
-			// with the weights of the row as four 16-byte groups
+			// with the weights of the row as four 16-byte groups (exact: the products of
+			// 1/4, 3/4, 3/4, 1/4 along each axis)
 			const float4 *wq = (const float4 *) xf.rw16;
 			size_t nc = (size_t) xf.nc;
 			int sy = 2 * J - 1 - (J0 - HS), sx = 2 * I - 1 - (I0 - HS);
 #pragma unroll
 			for (int q = 0; q < 4; q++) {
-				float4 wv = wq[q * nc + cc];
+				float4 wv;
+				if (xf.exact) {
+					float wy = (q == 0 || q == 3) ? 0.25f : 0.75f;
+					wv = make_float4(wy * 0.25f, wy * 0.75f, wy * 0.75f, wy * 0.25f);
+				} else {
+					wv = wq[q * nc + cc];
+				}
 				const float *row = xn + (sy + q) * W + sx;
 				acc += wv.x * row[0];
 				acc += wv.y * row[1];
@@ -361,7 +368,16 @@ __device__ __forceinline__ float prolong_at(const ggbk_xfer &xf, const float *xc
 		if (x >= 1 && x <= Nf - 2 && y >= 1 && y <= Nf - 2) {
 			int Nc = xf.Nc;
 			int t = (f * Nc + ((y - 1) >> 1)) * Nc + ((x - 1) >> 1);
-			float4 w = ((const float4 *) xf.pw)[c];
+			float4 w;
+			if (xf.exact) {
+				// odd x sits a quarter cell from its first tap column: 3/4 there, 1/4 on
+				// the next; even x the other way round (the same for y and rows)
+				float ax = (x & 1) ? 0.75f : 0.25f, ay = (y & 1) ? 0.75f : 0.25f;
+				float bx = 1.0f - ax, by = 1.0f - ay;
+				w = make_float4(ax * ay, bx * ay, ax * by, bx * by);
+			} else {
+				w = ((const float4 *) xf.pw)[c];
+			}
 			return w.x * xc[t] + w.y * xc[t + 1] + w.z * xc[t + Nc] + w.w * xc[t + Nc + 1];
 		}
 	}

@@ -14,6 +14,39 @@ import numpy as np
 import scipy.sparse as sp
 
 
+def chebyshev_hierarchy(N):
+    """Whether the library builds the Chebyshev V-cycle for a grid of face size N: a first
+    coarse level of at most 1536 cells (the dense bottom), every level above it a multiple
+    of the 16-cell tile (csrc/host/fluid.c, cheb_eligible)."""
+    nlev = 1
+    while (N >> nlev) >= 4 and (N >> (nlev - 1)) % 2 == 0:
+        nlev += 1
+    L = next((l for l in range(1, nlev) if 6 * (N >> l) ** 2 <= 1536), -1)
+    return L >= 1 and all((N >> l) % 16 == 0 for l in range(L))
+
+
+def exact_weights(taps, weights, Nf):
+    """The Chebyshev hierarchy's transfer: fine cells (f, x, y) with 1 <= x, y <= Nf - 2 whose
+    taps follow the bilinear pattern (coarse ((x - 1) >> 1, (y - 1) >> 1) and the cells right,
+    up and up-right of it) take the exact bilinear weights in equiangular coordinates -- the
+    centre sits a quarter cell from its nearer tap column, so 3/4 there and 1/4 on the other
+    (odd x: the first column), the same for rows -- instead of those evaluated at the
+    float-rounded centres; other cells keep theirs."""
+    Nc = Nf // 2
+    taps = np.asarray(taps).reshape(-1, 4)
+    w = np.array(weights, np.float64).reshape(-1, 4)
+    i = np.arange(taps.shape[0])
+    f, y, x = i // (Nf * Nf), (i // Nf) % Nf, i % Nf
+    inner = (x >= 1) & (x <= Nf - 2) & (y >= 1) & (y <= Nf - 2)
+    b = (f * Nc + ((y - 1) >> 1)) * Nc + ((x - 1) >> 1)
+    pat = inner & (taps[:, 0] == b) & (taps[:, 1] == b + 1) & (taps[:, 2] == b + Nc) & (taps[:, 3] == b + Nc + 1)
+    ax = np.where(x & 1, 0.75, 0.25)
+    ay = np.where(y & 1, 0.75, 0.25)
+    ex = np.stack([ax * ay, (1 - ax) * ay, ax * (1 - ay), (1 - ax) * (1 - ay)], 1)
+    w[pat] = ex[pat]
+    return taps, w
+
+
 def kmat(g):
     """K x = sum_k w_k (x_i - x_nb(k)) as a sparse matrix (float weights, float64 sums)."""
     n = g.n

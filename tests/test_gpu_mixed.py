@@ -3,10 +3,11 @@ smoothing of degree 2 on level 0 and 3 below, fused tile passes, float storage) 
 its float64 restatement (tests/mixed_precond.py, pinned by test_mixed_precond.py), and
 its effect on the PCG:
 
-  * the V-cycle applied through fluid_vcycle_f64 matches the restatement to float
-    rounding (2e-6 relative), at sizes with interior tiles and at sizes made only of seam
-    tiles; grids whose levels are not multiples of the 16-cell tile keep the reference
-    V(1,1) cycle in float (the restatement with degree 1);
+  * the V-cycle applied through fluid_vcycle_f64 matches the restatement (with the exact
+    bilinear face-interior transfer weights of the Chebyshev hierarchy) to float rounding
+    (2e-6 relative), at sizes with interior tiles and at sizes made only of seam tiles;
+    grids whose levels are not multiples of the 16-cell tile keep the reference V(1,1)
+    cycle in float and its transfer (the restatement with degree 1);
   * every tile through the graph tables gives the same bits as the interior tiles'
     index arithmetic;
   * the V-cycle is symmetric and positive to float rounding;
@@ -19,7 +20,7 @@ import numpy as np
 import pytest
 
 from conftest import levels_for
-from mixed_precond import MixedVcycle
+from mixed_precond import MixedVcycle, chebyshev_hierarchy, exact_weights
 
 pytestmark = pytest.mark.gpu
 
@@ -30,7 +31,10 @@ def restatement(ggb, orc, N):
     tr = []
     for l in range(len(levels) - 1):
         pt, pw = O.transfer(l)
-        tr.append((np.asarray(pt).reshape(-1, 4), np.asarray(pw).reshape(-1, 4)))
+        pt, pw = np.asarray(pt).reshape(-1, 4), np.asarray(pw).reshape(-1, 4)
+        if chebyshev_hierarchy(N):
+            pt, pw = exact_weights(pt, pw, levels[l].N)
+        tr.append((pt, pw))
     return levels, MixedVcycle(levels, tr)
 
 

@@ -7,13 +7,17 @@ mixed_precond.py), the checker of the GPU's Chebyshev V-cycle:
     q_k(lam) = T_k((5/2 - 2 lam) / (3/2)) / T_k(5/3), evaluated through an
     eigendecomposition of D^-1 K;
   * the V-cycle with the default degrees is symmetric (the same polynomial before and
-    after the coarse correction).
+    after the coarse correction);
+  * the exact bilinear face-interior transfer weights of the Chebyshev hierarchy are the
+    reference's own taps at the exact cell centres.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
 from conftest import levels_for
-from mixed_precond import MixedVcycle, kmat
+from mixed_precond import MixedVcycle, chebyshev_hierarchy, exact_weights, kmat
 
 
 def make(ggb, orc, N):
@@ -68,3 +72,54 @@ def test_vcycle_symmetric(ggb, orc, N):
     a, b = M(x) @ y, x @ M(y)
     assert abs(a - b) <= 1e-12 * np.linalg.norm(M(x)) * np.linalg.norm(y)
     assert M(x) @ x > 0
+
+
+def _ref_centre(orc):
+    f = orc.rlib("ref").cs_face_ab_to_dir
+    f.restype = orc.RVec3
+    f.argtypes = [C.c_int, C.c_double, C.c_double]
+    return f
+
+
+def test_exact_weights_are_the_reference_taps_at_exact_centres(ggb, orc, ref):
+    """The Chebyshev hierarchy's face-interior transfer weights (exact_weights, the same
+    formula as csrc/host/fluid.c and mgt_k.hip) equal the reference's own cs_taps
+    (cubesphere.c) evaluated at the exact double-precision cell centres (its
+    cs_face_ab_to_dir at the equiangular cell midpoints, before the rounding to float
+    positions) -- so tap order and the 3/4 - 1/4 assignment are the reference's -- and
+    differ from the taps at the float positions by that rounding only."""
+    Nf, Nc = 64, 32
+    centre = _ref_centre(orc)
+    rc = orc.RefGrid(Nc)
+    g = ggb.Grid(Nf)
+    h = (np.pi / 2) / Nf
+    rng = np.random.default_rng(7)
+    cells = rng.choice(6 * Nf * Nf, 400, replace=False)
+    taps = np.zeros((6 * Nf * Nf, 4), np.int64)
+    wts = np.zeros((6 * Nf * Nf, 4))
+    exact_at = {}
+    for i in cells:
+        f, y, x = i // (Nf * Nf), (i // Nf) % Nf, i % Nf
+        d = centre(int(f), -np.pi / 4 + (x + 0.5) * h, -np.pi / 4 + (y + 0.5) * h)
+        idx, wt = rc.taps((d.x, d.y, d.z))
+        exact_at[i] = (idx, wt)
+        taps[i], wts[i] = rc.taps(g.pos[i].astype(np.float64))
+    _, snapped = exact_weights(taps, wts, Nf)
+    checked = 0
+    for i, (idx, wt) in exact_at.items():
+        x, y = i % Nf, (i // Nf) % Nf
+        if not (1 <= x <= Nf - 2 and 1 <= y <= Nf - 2):
+            assert np.array_equal(snapped[i], wts[i])  # seam cells keep their weights
+            continue
+        assert np.array_equal(idx, taps[i])
+        assert np.abs(snapped[i] - wt).max() < 1e-12
+        assert 0 < np.abs(wts[i] - wt).max() < 5e-5
+        checked += 1
+    assert checked > 300
+
+
+def test_chebyshev_hierarchy_sizes():
+    """Grids with the tiled Chebyshev V-cycle: every level above the dense one (the first
+    of at most 1536 cells) a multiple of 16 wide."""
+    assert [N for N in (16, 24, 32, 48, 64, 96, 128, 256, 512, 1024) if chebyshev_hierarchy(N)] == \
+        [16, 32, 64, 128, 256, 512, 1024]
