@@ -312,7 +312,6 @@ struct ggb_fluid_dev {
 	int nxbufs;
 	void *omega, *omega_band, *psi, *psi_alt, *vel, *q0, *q1, *q2, *dep_b, *dep_f, *rbuf[2], *pbuf[2], *Ap, *z;
 	double *Az;             /* mixed-precision PCG: A z of the last V-cycle */
-	int nbt;                /* blocks of level 0's up pass (the fold of the mixed PCG) */
 	ggbk_scalars *s;        /* device */
 	ggbk_scalars *hs;       /* pinned host copy */
 	ggbk_red red;
@@ -1008,7 +1007,6 @@ static void setup_cheb(struct ggb_fluid_dev *D, const CubeSphere *cs)
 		D->levels[l].xpre = xbuf(D, sizeof(float) * (size_t) c->ncells, "multigrid");
 	}
 	D->Az = xbuf(D, sizeof(double) * (size_t) cs->ncells, "fluid");
-	D->nbt = D->levels[0].tl.ntiles;
 }
 
 int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
@@ -1070,6 +1068,8 @@ int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
 	D->nb0 = ggbk_nblocks((int) n);
 	D->red.stride = D->nb0;
 	D->red.part = xbuf(D, sizeof(double) * GGBK_NSLOTS * (size_t) D->nb0, "fluid");
+	D->red.cnt = xbuf(D, sizeof(unsigned) * GGBK_NCNT, "fluid");
+	ggbk_zero(D->red.cnt, sizeof(unsigned) * GGBK_NCNT);
 	ggbk_zero(D->s, sizeof(ggbk_scalars));
 	ggbk_zero(D->omega, ts * n);
 	ggbk_zero(D->omega_band, ts * n);
@@ -1351,8 +1351,8 @@ static void read_scalars(struct ggb_fluid_dev *D)
 }
 
 /* The mixed-precision PCG, the same iteration as the reference's (fluid.c:288-358) with the
- * work regrouped so that each iteration is one V-cycle and one fold: the V-cycle of r_k
- * also yields A z_k and the partials from which the fold forms rz, beta, p.Ap =
+ * work regrouped so that each iteration is one V-cycle: the V-cycle of r_k also yields
+ * A z_k and the sums from which the fold after it (ggbk_cheb_fold) forms rz, beta, p.Ap =
  * z.Az + 2 beta z.Ap + beta^2 p.Ap (A symmetric), alpha and the residual norm after the
  * coming update, ||r - alpha Ap'||^2; the next V-cycle's first pass applies that update
  * (p' = z + beta p, Ap' = Az + beta Ap).  The stopping test thus sees each update's
@@ -1370,7 +1370,6 @@ static void solve_cheb(struct ggb_fluid_dev *D, void *psi, int maxit)
 	c.z = D->z;
 	c.r = D->rbuf[0];
 	ggbk_vcycle_cheb(D->levels, D->nlev, &c, NULL, D->z, D->s, D->red);
-	ggbk_fin(GGBK_FIN_CHEB_FIRST, D->s, D->red, D->nbt, 0, 0, 0);
 	int batch = D->predict > 1 ? D->predict : 1;
 	int k = 0;
 	for (;;) {
@@ -1386,7 +1385,6 @@ static void solve_cheb(struct ggb_fluid_dev *D, void *psi, int maxit)
 				ggbk_fin(GGBK_FIN_CHEB_END, D->s, D->red, 0, 0, 0, 0);
 			} else {
 				ggbk_vcycle_cheb(D->levels, D->nlev, &c, NULL, D->z, D->s, D->red);
-				ggbk_fin(GGBK_FIN_CHEB, D->s, D->red, D->nbt, 0, 0, 0);
 			}
 		}
 		read_scalars(D);
@@ -1396,7 +1394,6 @@ static void solve_cheb(struct ggb_fluid_dev *D, void *psi, int maxit)
 		c.r = D->rbuf[k & 1];
 		c.Ap_in = ap[k & 1];
 		ggbk_vcycle_cheb(D->levels, D->nlev, &c, NULL, D->z, D->s, D->red);
-		ggbk_fin(GGBK_FIN_CHEB, D->s, D->red, D->nbt, 0, 0, 0);
 		batch = batch < 64 ? 2 * batch : 64;
 	}
 }
@@ -1630,7 +1627,7 @@ int fluid_vcycle_f64(Fluid *F, const double *b, double *x)
 	hs.vc = 1;
 	ggbk_scalars *ds = ggb_dmalloc(sizeof(ggbk_scalars), "vcycle");
 	upload(ds, &hs, sizeof(hs));
-	ggbk_red none = { NULL, 0 };
+	ggbk_red none = { NULL, 0, NULL };
 	if (D->cheb)
 		ggbk_vcycle_cheb(D->levels, D->nlev, NULL, db, dz, ds, none);
 	else
@@ -1715,8 +1712,7 @@ void fluid_remove_mean(const CubeSphere *cs, float *f)
 	int nb = ggbk_nblocks((int) n);
 	float *d = ggb_dmalloc(4 * n, "remove_mean");
 	ggbk_scalars *s = ggb_dmalloc(sizeof(ggbk_scalars), "remove_mean");
-	ggbk_red red;
-	red.stride = nb;
+	ggbk_red red = { NULL, nb, NULL };
 	red.part = ggb_dmalloc(sizeof(double) * GGBK_NSLOTS * (size_t) nb, "remove_mean");
 	ggbk_zero(s, sizeof(ggbk_scalars));
 	upload(d, f, 4 * n);

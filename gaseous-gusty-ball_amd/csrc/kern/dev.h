@@ -291,6 +291,62 @@ __device__ __forceinline__ double block_max(double v)
 	return r;
 }
 
+// The mixed-precision PCG's scalar update after a V-cycle, from its nine sums v = r.z,
+// z.Az, z.Ap, r.Az, Az.Az, Az.Ap, r.Ap, Ap.Ap, r.r (Ap the current direction's image): rz,
+// beta = rz / rz_old, p.Ap = z.Az + 2 beta z.Ap + beta^2 p.Ap_old (A symmetric), alpha, and the
+// residual after the coming update, r.r - 2 alpha r.Ap' + alpha^2 Ap'.Ap' with
+// Ap' = Az + beta Ap; the coming update is the last (stop) when that meets the tolerance
+// or the iteration cap.  first: no previous direction (p = z, Ap = Az).  r.r is the sum
+// over r itself: the prediction loses about two digits to cancellation, and chaining
+// predictions would compound that each iteration.
+__device__ __forceinline__ void ggbk_cheb_fold(ggbk_scalars *s, const double *v, int first)
+{
+	if (!s->active)
+		return;
+	if (s->stop) { // the last update has been applied
+		s->active = 0;
+		s->vc = 0;
+		return;
+	}
+	double rz = v[0], zAz = v[1], zAp = v[2], rAz = v[3], AzAz = v[4], AzAp = v[5], rAp0 = v[6], ApAp0 = v[7];
+	double rr = v[8], beta, pAp, rAp, ApAp;
+	if (first) {
+		beta = 0.0;
+		pAp = zAz;
+		rAp = rAz;
+		ApAp = AzAz;
+	} else {
+		beta = rz / s->rz;
+		pAp = zAz + 2.0 * beta * zAp + beta * beta * s->pAp;
+		rAp = rAz + beta * rAp0;
+		ApAp = AzAz + 2.0 * beta * AzAp + beta * beta * ApAp0;
+	}
+	s->rz = rz;
+	s->beta = beta;
+	s->pAp = pAp;
+	if (!(pAp > 0)) {
+		s->status = 1;
+		s->active = 0;
+		s->vc = 0;
+		return;
+	}
+	double alpha = rz / pAp;
+	s->alpha = alpha;
+	double rrn = rr - 2.0 * alpha * rAp + alpha * alpha * ApAp;
+	rrn = rrn > 0.0 ? rrn : 0.0;
+	s->rr = rrn;
+	s->it = s->it + 1;
+	if (rrn <= s->tol2) {
+		s->stop = 1;
+		s->vc = 0;
+		s->status = 0;
+	} else if (s->it >= s->maxit) {
+		s->stop = 1;
+		s->vc = 0;
+		s->status = 3;
+	}
+}
+
 // Logical block index with each XCD's share of the grid contiguous.  The dispatcher
 // deals workgroups to the 8 XCDs round-robin (workgroup k runs on XCD k mod 8) and each
 // XCD has its own L2, so with the plain index the cells of rows i +- N, which a stencil

@@ -73,53 +73,38 @@ __device__ __forceinline__ double fold(const double *p, int nb)
 __device__ __forceinline__ double fold_sum(const double *p, int nb) { return fold<false>(p, nb); }
 __device__ __forceinline__ double fold_max(const double *p, int nb) { return fold<true>(p, nb); }
 
-// K slots folded together, each in the same fixed order as fold_sum (thread t adds
-// p[t], p[t + FIN_BS], ... in turn): groups of G slots with all FOLD_BATCH loads of each in
-// flight at once, so a fold of up to FOLD_BATCH FIN_BS partials takes ceil(K / G) rounds
-// of memory latency.  Results valid in thread 0.
-template <int K, int G = 3>
-__device__ __forceinline__ void fold_sums(const double *p, size_t stride, const int (&slot)[K], int nb, double *out)
+// The mixed-precision PCG's fold (ggbk_fold_cheb): workgroup q folds the partials of sum q
+// (fold_sum's fixed order) into slot G0 + q; the last workgroup to finish, told by a ticket
+// taken after a device-scope fence, reads the nine sums and updates the scalars.  The ticket
+// counter is reset by that workgroup; nobody waits.
+__global__ __launch_bounds__(FIN_BS) void k_fold_cheb(ggbk_scalars *s, ggbk_red red, int nb, int first)
 {
-	__shared__ double wsum[K][FIN_BS / 64];
-	double acc[K];
-#pragma unroll
-	for (int k = 0; k < K; k++)
-		acc[k] = 0;
-	for (int i = threadIdx.x; i < nb; i += FOLD_BATCH * FIN_BS) {
-#pragma unroll
-		for (int k0 = 0; k0 < K; k0 += G) {
-			double v[G][FOLD_BATCH];
-#pragma unroll
-			for (int g = 0; g < G; g++)
-#pragma unroll
-				for (int q = 0; q < FOLD_BATCH; q++)
-					if (k0 + g < K)
-						v[g][q] = p[(size_t) slot[k0 + g] * stride + min(i + q * FIN_BS, nb - 1)];
-#pragma unroll
-			for (int g = 0; g < G; g++)
-#pragma unroll
-				for (int q = 0; q < FOLD_BATCH; q++)
-					if (k0 + g < K && i + q * FIN_BS < nb)
-						acc[k0 + g] += v[g][q];
-		}
-	}
-	int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-	for (int k = 0; k < K; k++) {
-		double v = wave_sum(acc[k]);
-		if (lane == 0)
-			wsum[k][wid] = v;
-	}
+	__shared__ int last;
+	const int q = blockIdx.x;
+	const int slot = q == 0 ? GGBK_SL_RZ : q == 8 ? GGBK_SL_RR : GGBK_SL_ZAZ + q - 1;
+	const size_t st = (size_t) red.stride;
+	double v = fold_sum(red.part + slot * st, nb);
+	if (threadIdx.x == 0)
+		red.part[(GGBK_SL_G0 + q) * st] = v;
+	__threadfence();
 	__syncthreads();
-	if (threadIdx.x == 0) {
+	if (threadIdx.x == 0)
+		last = atomicAdd(&red.cnt[0], 1u) == 8u;
+	__syncthreads();
+	if (!last || threadIdx.x != 0)
+		return;
+	__threadfence();
+	double vv[9];
 #pragma unroll
-		for (int k = 0; k < K; k++) {
-			double r = 0;
-			for (int w = 0; w < FIN_BS / 64; w++)
-				r += wsum[k][w];
-			out[k] = r;
-		}
-	}
+	for (int k = 0; k < 9; k++)
+		vv[k] = red.part[(GGBK_SL_G0 + k) * st];
+	red.cnt[0] = 0;
+	ggbk_cheb_fold(s, vv, first);
+}
+
+extern "C" void ggbk_fold_cheb(ggbk_scalars *s, ggbk_red red, int nb, int first)
+{
+	GGBK_LAUNCH(k_fold_cheb, 9, FIN_BS, 0, s, red, nb, first);
 }
 
 __global__ __launch_bounds__(FIN_BS) void k_fin(int op, ggbk_scalars *s, ggbk_red red, int nb, double tol, int maxit,
@@ -132,68 +117,6 @@ __global__ __launch_bounds__(FIN_BS) void k_fin(int op, ggbk_scalars *s, ggbk_re
 		if (threadIdx.x == 0 && s->active && s->stop) {
 			s->active = 0;
 			s->vc = 0;
-		}
-		return;
-	}
-	if (op == GGBK_FIN_CHEB_FIRST || op == GGBK_FIN_CHEB) {
-		// slots RZ, ZAZ .. APAP and RR (nine sums)
-		int act = s->active, stop = s->stop;
-		double rz_old = s->rz, pAp_old = s->pAp, tol2 = s->tol2;
-		int it = s->it, smaxit = s->maxit;
-		double v[9];
-		const int slots[9] = { GGBK_SL_RZ,   GGBK_SL_ZAZ,  GGBK_SL_ZAP, GGBK_SL_RAZ, GGBK_SL_AZAZ,
-				       GGBK_SL_AZAP, GGBK_SL_RAP,  GGBK_SL_APAP, GGBK_SL_RR };
-		fold_sums<9>(red.part, red.stride, slots, nb, v);
-		if (threadIdx.x != 0 || !act)
-			return;
-		if (stop) {
-			// the last update has been applied
-			s->active = 0;
-			s->vc = 0;
-			return;
-		}
-		// with Ap the previous direction's image: r.Ap and Ap.Ap are summed directly (v[6],
-		// v[7]); only p.Ap carries over from the previous fold
-		double rz = v[0], zAz = v[1], zAp = v[2], rAz = v[3], AzAz = v[4], AzAp = v[5], rAp0 = v[6], ApAp0 = v[7];
-		// r.r summed from r itself: the prediction below loses about two digits to
-		// cancellation, and chaining predictions would compound that each iteration
-		double rr = v[8];
-		double beta, pAp, rAp, ApAp;
-		if (op == GGBK_FIN_CHEB_FIRST) {
-			beta = 0.0;
-			pAp = zAz;
-			rAp = rAz;
-			ApAp = AzAz;
-		} else {
-			beta = rz / rz_old;
-			pAp = zAz + 2.0 * beta * zAp + beta * beta * pAp_old;
-			rAp = rAz + beta * rAp0;
-			ApAp = AzAz + 2.0 * beta * AzAp + beta * beta * ApAp0;
-		}
-		s->rz = rz;
-		s->beta = beta;
-		s->pAp = pAp;
-		if (pAp <= 0) {
-			s->status = 1;
-			s->active = 0;
-			s->vc = 0;
-			return;
-		}
-		double alpha = rz / pAp;
-		s->alpha = alpha;
-		// the residual after the coming update, r' = r - alpha Ap'
-		double rrn = fmax(rr - 2.0 * alpha * rAp + alpha * alpha * ApAp, 0.0);
-		s->rr = rrn;
-		it = it + 1;
-		s->it = it;
-		if (rrn <= tol2) {
-			s->stop = 1;
-			s->vc = 0;
-			s->status = 0;
-		} else if (it >= smaxit) {
-			s->stop = 1;
-			s->vc = 0;
-			s->status = 3;
 		}
 		return;
 	}
@@ -741,9 +664,8 @@ extern "C" int ggbk_remove_mean_noalloc(ggbk_grid g, float *f)
 		return -1;
 	unsigned char *base = (unsigned char *) g_scratch_dev[dev];
 	ggbk_scalars *s = (ggbk_scalars *) base;
-	ggbk_red red;
-	red.part = (double *) (base + NOALLOC_HEAD);
-	red.stride = 0; // the MEAN slot only: every slot at the same place
+	ggbk_red red = { (double *) (base + NOALLOC_HEAD), 0, nullptr };
+	// stride 0: the MEAN slot only, every slot at the same place
 	// every partial the fold reads is written first; the scalars start cleared
 	if (ggbk_zero(s, sizeof(ggbk_scalars)))
 		return -1;

@@ -132,17 +132,21 @@ void ggbk_applyK(ggbk_grid g, const void *x, double *y, int T);
 /* ---- reductions ----
  * Reduction kernels store one partial per block in slot `slot` of a partials array
  * (slot stride ggbk_red.stride doubles); a one-workgroup fold kernel then combines the
- * partials in block order and updates ggbk_scalars.  Deterministic, no cross-block
- * synchronisation inside a kernel. */
+ * partials in block order and updates ggbk_scalars.  Deterministic.  The mixed-precision
+ * PCG's fold of nine sums runs as nine workgroups, one per sum (into slot G0 + q); the
+ * last to finish (a ticket counter in cnt, no waiting) updates the scalars. */
 typedef struct {
 	double *part;
 	int stride;
+	unsigned *cnt;  /* ticket counters (zero between launches), or NULL */
 } ggbk_red;
 enum { GGBK_SL_BB = 0, GGBK_SL_RR0, GGBK_SL_PAP, GGBK_SL_RR, GGBK_SL_RZ, GGBK_SL_MEAN, GGBK_SL_VMAX,
        /* the mixed-precision PCG's fold after each V-cycle (z its output, r the residual it
         * preconditioned, Ap the current search direction's image) */
        GGBK_SL_ZAZ, GGBK_SL_ZAP, GGBK_SL_RAZ, GGBK_SL_AZAZ, GGBK_SL_AZAP, GGBK_SL_RAP, GGBK_SL_APAP,
-       GGBK_NSLOTS };
+       /* the nine sums RZ, ZAZ .. APAP, RR of its fold, in that order (element 0 of each) */
+       GGBK_SL_G0, GGBK_NSLOTS = GGBK_SL_G0 + 9 };
+#define GGBK_NCNT 8 /* ticket counters per ggbk_red */
 enum {
 	GGBK_FIN_START = 0, /* bb, rr -> tol2, active, status */
 	GGBK_FIN_PAP,       /* pAp -> alpha, or stop */
@@ -152,13 +156,8 @@ enum {
 	GGBK_FIN_MEAN_F64,  /* mean = sum / total_area */
 	GGBK_FIN_MEAN_F32,  /* mean rounded to float */
 	GGBK_FIN_VMAX,      /* vmax2 = max */
-	/* mixed-precision PCG, after a V-cycle: rz, beta, pAp = z.Az + 2 beta z.Ap + beta^2
-	 * pAp, alpha, and the residual after the coming update, r.r - 2 alpha r.Ap' + alpha^2
-	 * Ap'.Ap' with Ap' = Az + beta Ap; the coming update is the last when that meets the
-	 * tolerance (or the iteration cap).  _FIRST: no previous direction (p = z). */
-	GGBK_FIN_CHEB_FIRST,
-	GGBK_FIN_CHEB,
-	GGBK_FIN_CHEB_END,  /* after an update applied alone: the loop ends if it was the last */
+	GGBK_FIN_CHEB_END,  /* mixed-precision PCG, after an update applied alone: the loop ends
+	                     * if it was the last */
 };
 /* nb = number of blocks that wrote partials */
 void ggbk_fin(int op, ggbk_scalars *s, ggbk_red red, int nb, double tol, int maxit, double total_area);
@@ -251,9 +250,10 @@ int ggbk_vcycle_update(const ggbk_level *levels, int nlev, const ggbk_cg_fuse *u
  * c == NULL: z = M r (stores gated by s->vc).  Otherwise the PCG's V-cycle: with c->update
  * its first pass applies the update (the direction p = z + beta p, Ap = Az + beta Ap, first:
  * p = z, Ap = Az; psi += alpha p; r_out = r - alpha Ap) and preconditions r_out, else it
- * preconditions c->r; its last pass also stores A z (double) and the partials of
- * GGBK_FIN_CHEB* (one per level-0 up tile).  The update's stores are gated by s->active,
- * the V-cycle's by s->vc.  r_out must not alias r, nor Ap_out Ap_in. */
+ * preconditions c->r; its last pass also stores A z (double) and the partials of the
+ * sums of ggbk_cheb_fold, which ggbk_fold_cheb then folds into the scalars.  The update's
+ * stores are gated by s->active, the V-cycle's by s->vc.  r_out must not
+ * alias r, nor Ap_out Ap_in. */
 typedef struct {
 	int update;                   /* apply the PCG update in the first pass */
 	int first;                    /* the update is the first: p = z, Ap = Az */
@@ -267,6 +267,9 @@ typedef struct {
 	double *Az;                   /* in: the previous V-cycle's A z (update); out: this one's */
 	const float *z;               /* the previous V-cycle's z (update) */
 } ggbk_cg_cheb;
+/* The fold after the PCG's V-cycle (nine workgroups; nb partials per sum): ggbk_cheb_fold
+ * (dev.h) on the nine sums.  first: the solve's first V-cycle (no direction yet). */
+void ggbk_fold_cheb(ggbk_scalars *s, ggbk_red red, int nb, int first);
 /* The PCG update alone, as ggbk_vcycle_cheb's first pass applies it: for the update the
  * host expects to be the last of the solve. */
 void ggbk_cg_update_cheb(ggbk_grid g, const ggbk_cg_cheb *c, const ggbk_scalars *s);

@@ -497,7 +497,7 @@ static void vcycle_level(const ggbk_level *lv, int nlev, int l, const TB *b, TM 
 		GGBK_LAUNCH((k_mg_restrict<TM, 18>), NB(L.xf.nc), BS, 0, L.xf, (const TM *) r, (TM *) C.b, s);
 	else
 		GGBK_LAUNCH((k_mg_restrict<TM, GGBK_RMAX>), NB(L.xf.nc), BS, 0, L.xf, (const TM *) r, (TM *) C.b, s);
-	ggbk_red none = { nullptr, 0 };
+	ggbk_red none = { nullptr, 0, nullptr };
 	vcycle_level<TM, TM>(lv, nlev, l + 1, (const TM *) C.b, (TM *) C.x, s, none);
 	if (l == 0)
 		ggbk_probe_begin(GGBK_PROBE_MG_UP0);

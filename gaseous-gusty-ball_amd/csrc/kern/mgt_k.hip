@@ -614,6 +614,10 @@ extern "C" void ggbk_vcycle_cheb(const ggbk_level *lv, int nlev, const ggbk_cg_c
 	const double *ap = cg.update ? cg.Ap_out : cg.Ap_in;
 	ggbk_probe_begin(GGBK_PROBE_MG_UP0);
 	GGBK_LAUNCH((k_mgt_up_cg<2>), L0.tl.ntiles, MGT_BS, 0, L0.g, L0.tl, L0.xf, b0, (const float *) L0.xpre,
-		    (const float *) lv[1].x, (float *) zout, c ? cg.Az : nullptr, ap, s, c ? red : ggbk_red{ nullptr, 0 });
+		    (const float *) lv[1].x, (float *) zout, c ? cg.Az : nullptr, ap, s,
+		    c ? red : ggbk_red{ nullptr, 0, nullptr });
 	ggbk_probe_end(GGBK_PROBE_MG_UP0);
+	// the fold (after the first V-cycle of a solve there is no direction yet)
+	if (c)
+		ggbk_fold_cheb(s, red, L0.tl.ntiles, !cg.update && !cg.Ap_in);
 }
