@@ -64,19 +64,21 @@ def kernel_bytes(name, n, P, E, T, Ts, Tm, Tz, reorder_frac=1.0, cheb=False):
     if name == "k_render_fill":      # endpoints read, sample count, list; on reordering deposits
         return P * (48 + 2) + reorder_frac * P * (20 + 68) + E * 4  # also colour+index, copy out
     if name == "mg_up(level 0)":
-        if cheb:                     # b, x_pre, tap weights, coarse x, stencil record, dinv, z out
-            return n * (Ts + 4 + 16 + 1 + 8 + 4 + 4)
+        if cheb:                     # r, x_pre, coarse x, stencil record, dinv, Ap in; z, A z out
+            return n * (Ts + 4 + 1 + 8 + 4 + Ts + 4 + Ts)
         return n * (Ts + 4 + 8 + 16 + Tm / 4 + Tm)   # b, dinv, stencil record, tap weights, coarse x, z out
     if name == "k_mg_down(level 0)":
-        if cheb:                     # b, stencil record, dinv; x_pre and the coarse b out
+        if cheb:                     # r, stencil record, dinv; x_pre and the coarse b out
             return n * (Ts + 8 + 4 + 4 + 1)
         return n * (Ts + 4 + 8 + Tm)  # b, dinv, stencil record, r1 out
     if name == "k_cg_Ap":            # z, p in, stencil record, p out, Ap out
         return n * (Tz + Ts + 8 + 2 * Ts)
     if name == "k_cg_update_down":
-        if cheb:                     # psi, r, p, Ap, stencil record, dinv in; psi, r, x_pre, coarse b out
-            return n * (6 * Ts + 12 + 4 + 1)
+        if cheb:                     # r, A z, Ap, p, psi, z, stencil record, dinv in; psi, p, Ap, r, x_pre, coarse b out
+            return n * (5 * Ts + 4 + 8 + 4 + 4 * Ts + 4 + 1)
         return n * (6 * Ts + 12 + Tm)  # psi, r, p, Ap, dinv, stencil record in; psi, r, r1 out
+    if name == "k_cg_update_cheb":   # the update alone: r, A z, Ap, p, psi, z in; psi, p, Ap, r out
+        return n * (5 * Ts + 4 + 4 * Ts)
     if name == "k_particles_advect":  # pos in, pos+prev out (FP64), velocity field once
         return P * 72 + n * 3 * Ts
     if name == "k_departures":       # vel, omega, centre in; q0, dep_b, dep_f out
@@ -88,8 +90,9 @@ def fluid_step_bytes(N, its, vcycles, precision, cheb):
     """Algorithmic bytes of one fluid step under the per-pass model (each pass reads its
     inputs and writes its outputs once; gathered neighbours, the stencil's class weights
     and the small transfer tables are cache hits): the fixed passes plus, per V-cycle, the
-    level-0 down and up passes and every coarser level's, and per CG iteration the Ap pass.
-    Returns (bytes, breakdown)."""
+    level-0 down and up passes and every coarser level's, and per CG iteration the Ap pass
+    (FP64 mode).  The mixed PCG runs its iterations' updates inside the V-cycles' first
+    passes (one V-cycle per iteration, the last update alone).  Returns (bytes, breakdown)."""
     n = 6 * N * N
     Ts = 4 if precision == ggb.PREC_REF32 else 8
     Tm = 8 if precision == ggb.PREC_FP64 else 4
@@ -98,7 +101,7 @@ def fluid_step_bytes(N, its, vcycles, precision, cheb):
         "advect + bfecc + limiter": n * (40 + 48 + 64) * Ts // 8,
         "cg_start": n * (3 * Ts + 4 + 8),
         "velocity + mean": n * (2 * Ts + 96 + 32 + 12 + 3 * Ts + 12),
-        "extrapolated start": n * 4 * Ts if precision == ggb.PREC_MIXED else 0,
+        "extrapolated start": n * 5 * Ts if precision == ggb.PREC_MIXED else 0,
     }
     per_cycle = 0.0
     levels = []
@@ -113,19 +116,28 @@ def fluid_step_bytes(N, its, vcycles, precision, cheb):
     for l, nl in enumerate(levels):
         if l == 0:
             continue
-        # a coarse level's down pass (b, stencil record, dinv in; x_pre and the next level's b
-        # out) and up pass (b, x_pre, taps and weights, coarse x, stencil record, dinv in; x out)
-        per_cycle += nl * (4.5 * Tm + 56)
+        if cheb:
+            # a coarse level's down pass (b, stencil record, dinv in; x_pre and the next level's
+            # b out) and up pass (b, x_pre, coarse x, stencil record, dinv in; x out)
+            per_cycle += nl * 46
+        else:
+            # the same with the restriction and prolongation tables read
+            per_cycle += nl * (4.5 * Tm + 56)
     if m <= 1536:
         per_cycle += Tm * m * m + 2 * Tm * m  # the dense bottom operator and its vectors
     up0 = kernel_bytes("mg_up(level 0)", n, 0, 0, 0, Ts, Tm, Tm, cheb=cheb)
     down0 = kernel_bytes("k_mg_down(level 0)", n, 0, 0, 0, Ts, Tm, Tm, cheb=cheb)
     upd0 = kernel_bytes("k_cg_update_down", n, 0, 0, 0, Ts, Tm, Tm, cheb=cheb)
-    ap = kernel_bytes("k_cg_Ap", n, 0, 0, 0, Ts, Tm, Tm)
-    solver = {
-        "V-cycles": vcycles * (per_cycle + up0) + down0 + (vcycles - 1) * upd0,
-        "Ap": its * ap,
-    }
+    if cheb:
+        # its V-cycles (the first without an update), its - 1 fused updates, one alone
+        alone = kernel_bytes("k_cg_update_cheb", n, 0, 0, 0, Ts, Tm, Tm)
+        solver = {"V-cycles": its * (per_cycle + up0) + down0 + (its - 1) * upd0 + alone}
+    else:
+        ap = kernel_bytes("k_cg_Ap", n, 0, 0, 0, Ts, Tm, Tm)
+        solver = {
+            "V-cycles": vcycles * (per_cycle + up0) + down0 + (vcycles - 1) * upd0,
+            "Ap": its * ap,
+        }
     b = dict(fixed)
     b.update(solver)
     return float(sum(b.values())), {k: int(v) for k, v in b.items()}

@@ -74,13 +74,18 @@ def test_fluid_step_byte_model(bench):
     b6, parts6 = bench.fluid_step_bytes(512, 6, 7, ggb.PREC_MIXED, True)
     b7, parts7 = bench.fluid_step_bytes(512, 7, 8, ggb.PREC_MIXED, True)
     n = 6 * 512 * 512
-    # one more iteration adds one Ap pass, one fused update + down pass, one level-0 up pass
-    # and one pass over the coarse levels -- about 160-200 B per cell in mixed precision
+    # one more iteration of the mixed PCG adds one fused update + down pass, one level-0 up
+    # pass (which also forms A z: no separate Ap pass) and one pass over the coarse levels
+    # and the dense bottom -- about 160 B per cell
     extra = (b7 - b6) / n
-    assert 150 < extra < 220, extra
-    assert parts7["Ap"] - parts6["Ap"] == bench.kernel_bytes("k_cg_Ap", n, 0, 0, 0, 8, 4, 4)
+    assert 130 < extra < 190, extra
+    assert "Ap" not in parts6
+    assert extra * n == pytest.approx(bench.kernel_bytes("k_cg_update_down", n, 0, 0, 0, 8, 4, 4, cheb=True)
+                                      + bench.kernel_bytes("mg_up(level 0)", n, 0, 0, 0, 8, 4, 4, cheb=True)
+                                      + 46 * n * (1 / 4 + 1 / 16 + 1 / 64 + 1 / 256) + 4 * 1536 ** 2 + 8 * 1536)
     assert sum(parts6.values()) == int(b6) or abs(sum(parts6.values()) - b6) < len(parts6)
-    # FP64 storage moves more per V-cycle
-    b6d, _ = bench.fluid_step_bytes(512, 6, 7, ggb.PREC_FP64, False)
+    # FP64 storage moves more per V-cycle, and its PCG has the Ap pass
+    b6d, parts6d = bench.fluid_step_bytes(512, 6, 7, ggb.PREC_FP64, False)
     assert b6d > b6
+    assert parts6d["Ap"] == 6 * bench.kernel_bytes("k_cg_Ap", n, 0, 0, 0, 8, 8, 8)
 
