@@ -1758,6 +1758,7 @@ struct FluidAdjoint {
 	double *ckpt;          /* device [nck_cap][2][n]: omega, psi at steps 0, K, 2K, ... */
 	double *tape;          /* device [K][2][n] */
 	double *lam, *lq, *lq1, *lq2, *ldb, *ldf, *lvel, *lpsi, *om, *z, *dk;
+	ggbk_adj_tr tr;        /* entries of the transposes (no floating-point atomics) */
 	unsigned char *rslot;  /* device [maxnb][n]: slot of cell j in the list of its m-th neighbour */
 	double sum_area;
 	int *its;              /* host [nsteps + 1]: CG iterations of the last forward (set-up, steps) */
@@ -1811,6 +1812,16 @@ FluidAdjoint *fluid_adjoint_new(Fluid *F, int checkpoint_every)
 	for (size_t k = 0; k < sizeof(v3) / sizeof(v3[0]); k++)
 		*v3[k] = ggb_dmalloc(sizeof(double) * 3 * n, "adjoint");
 	A->dk = ggb_dmalloc(sizeof(double), "adjoint");
+	/* 10 entries per cell (the departure points' taps), 2n targets (lq and lq2) */
+	A->tr.cap_entries = 10 * (long long) n;
+	A->tr.cap_targets = (int) (2 * n);
+	A->tr.tgt = ggb_dmalloc(sizeof(int) * 10 * n, "adjoint");
+	A->tr.val = ggb_dmalloc(sizeof(double) * 30 * n, "adjoint");
+	A->tr.list = ggb_dmalloc(sizeof(int) * 10 * n, "adjoint");
+	A->tr.cnt = ggb_dmalloc(sizeof(int) * 2 * n, "adjoint");
+	A->tr.cursor = ggb_dmalloc(sizeof(int) * 2 * n, "adjoint");
+	A->tr.start = ggb_dmalloc(sizeof(int) * (2 * n + 1), "adjoint");
+	A->tr.bsum = ggb_dmalloc(sizeof(int) * (2 * n / 1024 + 2), "adjoint");
 	return A;
 }
 
@@ -1820,7 +1831,8 @@ void fluid_adjoint_free(FluidAdjoint *A)
 		return;
 	ggb_sync();
 	void *bufs[] = { A->ckpt, A->tape, A->lam, A->lq, A->lq1, A->lq2, A->ldb, A->ldf,
-			 A->lvel, A->lpsi, A->om, A->z, A->dk, A->rslot };
+			 A->lvel, A->lpsi, A->om, A->z, A->dk, A->rslot, A->tr.tgt, A->tr.val,
+			 A->tr.list, A->tr.cnt, A->tr.cursor, A->tr.start, A->tr.bsum };
 	for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); k++)
 		ggbk_free(bufs[k]);
 	free(A->dts);
@@ -1919,17 +1931,25 @@ static void adj_step(FluidAdjoint *A, double dt, const double *om_s, const doubl
 	ggbk_zero(A->lq1, 8 * n);
 	ggbk_zero(A->lq2, 8 * n);
 	ggbk_zero(A->lvel, 24 * n);
-	ggbk_adj_limiter(g, D->q0, D->q2, D->dep_b, D->omega_band, A->lam, D->s, relax, drelax, c, bfecc, A->lq,
-			 A->lq2, A->ldb, D->red);
+	ggbk_adj_limiter(g, D->q0, D->q2, D->dep_b, D->omega_band, A->lam, D->s, relax, drelax, c, bfecc, &A->tr,
+			 A->ldb, D->red);
 	ggbk_fin(GGBK_FIN_MEAN_F64, D->s, D->red, D->nb0, 0, 0, g.total_area);
 	ggbk_adj_accum(D->s, A->dk);
+	if (ggbk_adj_transpose(&A->tr, 4 * (long long) n, 1, (int) (2 * n), (int) n, A->lq, A->lq2))
+		ggb_die("fluid_adjoint_backward: transpose buffers too small");
 	/* BFECC advections */
 	if (bfecc) {
-		ggbk_adj_bfecc_f(g, A->lq2, D->q1, D->dep_f, A->lq, A->lq1, A->ldf);
-		ggbk_adj_advect_b(g, A->lq1, D->q0, D->dep_b, A->lq, A->ldb);
+		ggbk_adj_bfecc_f(g, A->lq2, D->q1, D->dep_f, A->lq, &A->tr, A->ldf);
+		if (ggbk_adj_transpose(&A->tr, 4 * (long long) n, 1, (int) n, (int) n, A->lq1, NULL))
+			ggb_die("fluid_adjoint_backward: transpose buffers too small");
+		ggbk_adj_advect_b(g, A->lq1, D->q0, D->dep_b, &A->tr, A->ldb);
+		if (ggbk_adj_transpose(&A->tr, 4 * (long long) n, 1, (int) n, (int) n, A->lq, NULL))
+			ggb_die("fluid_adjoint_backward: transpose buffers too small");
 	}
 	/* departure points, velocity, Poisson */
-	ggbk_adj_departures(g, D->vel, A->ldb, A->ldf, dt, bfecc, A->lvel);
+	ggbk_adj_departures(g, D->vel, A->ldb, A->ldf, dt, bfecc, &A->tr);
+	if (ggbk_adj_transpose(&A->tr, 10 * (long long) n, 3, (int) n, (int) n, A->lvel, NULL))
+		ggb_die("fluid_adjoint_backward: transpose buffers too small");
 	ggbk_adj_velocity(g, A->lvel, A->rslot, A->lpsi, D->red);
 	ggbk_fin(GGBK_FIN_MEAN_F64, D->s, D->red, D->nb0, 0, 0, g.total_area);
 	ggbk_adj_rhs(g, A->lpsi, D->s, A->sum_area, A->om);

@@ -9,12 +9,15 @@
 //   velocity u = p x grad psi (:360-376)     k_adj_velocity   (gather over reverse slots)
 //   Poisson solve                            k_adj_rhs + the forward MG-PCG + k_adj_combine
 //
-// The transpose of an interpolation scatters w_k * lambda to the four taps with FP64
-// atomics; the derivative of the bilinear weights with respect to the sample point
-// gives the departure-point adjoint.  Great-circle moves are differentiated in closed
+// The transpose of an interpolation sends w_k * lambda to the four taps; the derivative
+// of the bilinear weights with respect to the sample point gives the departure-point
+// adjoint.  Those transposes run without floating-point atomics: the scatter kernels write
+// per-source entries (target, value), and ggbk_adj_transpose adds each target's entries in
+// source order (below), so gradients repeat bit for bit.  Great-circle moves are differentiated in closed
 // form (Rodrigues step, then normalisation).  Forward intermediates are recomputed by
 // the forward kernels from the taped (omega, psi) of the step.
 #include <hip/hip_runtime.h>
+#include <limits.h>
 
 #include "dev.h"
 #include "ggbk.h"
@@ -82,11 +85,22 @@ __device__ __forceinline__ dv3 gc_move_adj(dv3 p, dv3 v, double s, dv3 lo)
 	return vtan(lt, p);
 }
 
-__device__ __forceinline__ void atomic_add3(double *a, dv3 v)
+// Entry e of a scatter: its target (-1: none) and W values.
+struct TrEntries {
+	int *tgt;
+	double *val;
+};
+
+__device__ __forceinline__ void put1(const TrEntries &t, size_t e, int target, double v)
 {
-	atomicAdd(a, v.x);
-	atomicAdd(a + 1, v.y);
-	atomicAdd(a + 2, v.z);
+	t.tgt[e] = target;
+	t.val[e] = v;
+}
+
+__device__ __forceinline__ void put3(const TrEntries &t, size_t e, int target, dv3 v)
+{
+	t.tgt[e] = target;
+	st3(t.val + 3 * e, v);
 }
 
 // Sum of x (slot MEAN partials; fold with FIN_MEAN_F64: s->sum, s->mean = sum / area).
@@ -98,11 +112,12 @@ __global__ __launch_bounds__(BS) void k_adj_sum(int n, const double *x, ggbk_red
 
 // Relaxation toward the band (fluid.c:466-474) after the mean removal (:475), and the
 // limited final advection (:449-464).  lam = dL/d omega_{n+1}; s->mean = sum(lam) / area.
-// Out: lq (scatter, atomics), lq2 (scatter, atomics), ldb (store), dL/dk partial.
+// Out: four entries per cell for lq (targets [0, n)) and lq2 (targets [n, 2n)), ldb
+// (store), dL/dk partial.
 __global__ __launch_bounds__(BS) void k_adj_limiter(ggbk_grid g, const double *q, const double *q2,
 						     const double *dep_b, const double *band, const double *lam,
 						     const ggbk_scalars *s, double relax, double drelax, double c,
-						     int bfecc, double *lq, double *lq2, double *ldb, ggbk_red red)
+						     int bfecc, TrEntries tr, double *ldb, ggbk_red red)
 {
 	int i = xblock() * BS + threadIdx.x;
 	double dk = 0;
@@ -144,15 +159,17 @@ __global__ __launch_bounds__(BS) void k_adj_limiter(ggbk_grid g, const double *q
 		dk = lpp * (band[i] - w) * drelax;
 		double lqn = (1 - relax) * lpp;
 		dv3 d = d3(0, 0, 0);
-		if (state == 1) {
-			atomicAdd(&lq[t.idx[klo]], lqn);
-		} else if (state == 2) {
-			atomicAdd(&lq[t.idx[khi]], lqn);
+		size_t e = 4 * (size_t) i;
+		if (state == 1 || state == 2) {
+			// a clamped cell passes its adjoint to the extreme tap
+			put1(tr, e, t.idx[state == 1 ? klo : khi], lqn);
+			for (int k = 1; k < 4; k++)
+				put1(tr, e + k, -1, 0.0);
 		} else {
-			double *lt = bfecc ? lq2 : lq;
+			int off = bfecc ? g.n : 0; // lq2 when the source was the BFECC field
 #pragma unroll
 			for (int k = 0; k < 4; k++) {
-				atomicAdd(&lt[t.idx[k]], wt[k] * lqn);
+				put1(tr, e + k, t.idx[k] + off, wt[k] * lqn);
 				d = vadd(d, vscl(dw[k], src[t.idx[k]] * lqn));
 			}
 		}
@@ -162,9 +179,10 @@ __global__ __launch_bounds__(BS) void k_adj_limiter(ggbk_grid g, const double *q
 }
 
 // q2 = q + 0.5 (q - A_f q1) (fluid.c:446-448): lq += 1.5 lq2; then the transpose of
-// A_f applied to -0.5 lq2 scatters into lq1, and its weight derivative gives ldf.
+// A_f applied to -0.5 lq2, as four entries per cell for lq1, and its weight derivative
+// gives ldf.
 __global__ __launch_bounds__(BS) void k_adj_bfecc_f(ggbk_grid g, const double *lq2, const double *q1,
-						     const double *dep_f, double *lq, double *lq1, double *ldf)
+						     const double *dep_f, double *lq, TrEntries tr, double *ldf)
 {
 	int i = xblock() * BS + threadIdx.x;
 	if (i >= g.n)
@@ -179,15 +197,16 @@ __global__ __launch_bounds__(BS) void k_adj_bfecc_f(ggbk_grid g, const double *l
 	dv3 d = d3(0, 0, 0);
 #pragma unroll
 	for (int k = 0; k < 4; k++) {
-		atomicAdd(&lq1[t.idx[k]], wt[k] * l);
+		put1(tr, 4 * (size_t) i + k, t.idx[k], wt[k] * l);
 		d = vadd(d, vscl(dw[k], q1[t.idx[k]] * l));
 	}
 	st3(ldf + 3 * (size_t) i, d);
 }
 
-// q1 = A_b q: transpose into lq, weight derivative added to ldb.
+// q1 = A_b q: the transpose as four entries per cell for lq, the weight derivative added
+// to ldb.
 __global__ __launch_bounds__(BS) void k_adj_advect_b(ggbk_grid g, const double *lq1, const double *q,
-						      const double *dep_b, double *lq, double *ldb)
+						      const double *dep_b, TrEntries tr, double *ldb)
 {
 	int i = xblock() * BS + threadIdx.x;
 	if (i >= g.n)
@@ -200,7 +219,7 @@ __global__ __launch_bounds__(BS) void k_adj_advect_b(ggbk_grid g, const double *
 	dv3 d = ld3(ldb + 3 * (size_t) i);
 #pragma unroll
 	for (int k = 0; k < 4; k++) {
-		atomicAdd(&lq[t.idx[k]], wt[k] * l);
+		put1(tr, 4 * (size_t) i + k, t.idx[k], wt[k] * l);
 		d = vadd(d, vscl(dw[k], q[t.idx[k]] * l));
 	}
 	st3(ldb + 3 * (size_t) i, d);
@@ -210,7 +229,7 @@ __global__ __launch_bounds__(BS) void k_adj_advect_b(ggbk_grid g, const double *
 // with respect to the velocity field: pm = move(p, u_i, s1); r = sum w_k(pm) u[tap_k];
 // um = tan_p(tan_pm(r)); dep = move(p, um, s2).
 __device__ __forceinline__ void departure_adj(const ggbk_grid &g, const double *vel, int i, double s1, double s2,
-					      dv3 ldep, double *lvel)
+					      dv3 ldep, const TrEntries &tr, size_t e)
 {
 	dv3 p = cell_pos(g, i), u = ld3(vel + 3 * (size_t) i);
 	dv3 pm = gc_move(p, u, s1);
@@ -233,21 +252,161 @@ __device__ __forceinline__ void departure_adj(const ggbk_grid &g, const double *
 	dv3 lpm = vsub(vscl(lsv, -vdot(pm, r)), vscl(r, vdot(pm, lsv)));
 #pragma unroll
 	for (int k = 0; k < 4; k++) {
-		atomic_add3(lvel + 3 * (size_t) t.idx[k], vscl(lr, wt[k]));
+		put3(tr, e + k, t.idx[k], vscl(lr, wt[k]));
 		lpm = vadd(lpm, vscl(dw[k], vdot(uk[k], lr)));
 	}
-	atomic_add3(lvel + 3 * (size_t) i, gc_move_adj(p, u, s1, lpm));
+	put3(tr, e + 4, i, gc_move_adj(p, u, s1, lpm));
 }
 
+// Ten entries per cell for lvel: the backward and (BFECC) forward departure points' four
+// taps and the cell itself.
 __global__ __launch_bounds__(BS) void k_adj_departures(ggbk_grid g, const double *vel, const double *ldb,
-							const double *ldf, double dt, int bfecc, double *lvel)
+							const double *ldf, double dt, int bfecc, TrEntries tr)
 {
 	int i = xblock() * BS + threadIdx.x;
 	if (i >= g.n)
 		return;
-	departure_adj(g, vel, i, -0.5 * dt, -dt, ld3(ldb + 3 * (size_t) i), lvel);
-	if (bfecc)
-		departure_adj(g, vel, i, 0.5 * dt, dt, ld3(ldf + 3 * (size_t) i), lvel);
+	size_t e = 10 * (size_t) i;
+	departure_adj(g, vel, i, -0.5 * dt, -dt, ld3(ldb + 3 * (size_t) i), tr, e);
+	if (bfecc) {
+		departure_adj(g, vel, i, 0.5 * dt, dt, ld3(ldf + 3 * (size_t) i), tr, e + 5);
+	} else {
+		for (int k = 5; k < 10; k++)
+			put3(tr, e + k, -1, d3(0, 0, 0));
+	}
+}
+
+// ---------------- transposes without atomics (ggbk_adj_transpose) ----------------
+// Entries per target counted (integer atomics: the counts do not depend on their order),
+// an exclusive scan of the counts into starts, each entry's id placed into its target's
+// bucket (in any order), then each target adds its entries in ascending id -- the
+// sources' order -- to its destination.  The ids of a bucket are taken smallest first by
+// repeated selection (buckets hold about 4 to 10 entries).
+
+__global__ __launch_bounds__(BS) void k_tr_count(long long nent, const int *tgt, int *cnt)
+{
+	long long e = (long long) blockIdx.x * BS + threadIdx.x;
+	if (e < nent && tgt[e] >= 0)
+		atomicAdd(&cnt[tgt[e]], 1);
+}
+
+#define SCAN_BS 1024
+// Exclusive scan of a block's 1024 values (one per thread); *total gets their sum.
+__device__ __forceinline__ int block_excl_scan(int v, int *total)
+{
+	__shared__ int wtot[SCAN_BS / 64];
+	int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	int x = v;
+#pragma unroll
+	for (int off = 1; off < 64; off <<= 1) {
+		int y = __shfl_up(x, off, 64);
+		if (lane >= off)
+			x += y;
+	}
+	if (lane == 63)
+		wtot[w] = x;
+	__syncthreads();
+	if (w == 0) {
+		int t = lane < SCAN_BS / 64 ? wtot[lane] : 0;
+#pragma unroll
+		for (int off = 1; off < SCAN_BS / 64; off <<= 1) {
+			int y = __shfl_up(t, off, 64);
+			if (lane >= off)
+				t += y;
+		}
+		if (lane < SCAN_BS / 64)
+			wtot[lane] = t;
+	}
+	__syncthreads();
+	int r = (w ? wtot[w - 1] : 0) + x - v;
+	*total = wtot[SCAN_BS / 64 - 1];
+	__syncthreads();
+	return r;
+}
+
+// pass 1: each block's exclusive scan into start, its total into bsum
+__global__ __launch_bounds__(SCAN_BS) void k_tr_scan_local(int nt, const int *cnt, int *start, int *bsum)
+{
+	int i = blockIdx.x * SCAN_BS + threadIdx.x, tot;
+	int v = i < nt ? cnt[i] : 0;
+	int x = block_excl_scan(v, &tot);
+	if (i < nt)
+		start[i] = x;
+	if (threadIdx.x == 0)
+		bsum[blockIdx.x] = tot;
+}
+
+// pass 2 (one workgroup): exclusive scan of the nb block totals in place, in chunks of
+// TR_TOP per thread; start[nt] = the grand total
+#define TR_TOP 16
+__global__ __launch_bounds__(SCAN_BS) void k_tr_scan_top(int nb, int nt, int *bsum, int *start)
+{
+	int base = threadIdx.x * TR_TOP, loc = 0, tot;
+	for (int k = 0; k < TR_TOP; k++)
+		loc += base + k < nb ? bsum[base + k] : 0;
+	int x = block_excl_scan(loc, &tot);
+	for (int k = 0; k < TR_TOP; k++)
+		if (base + k < nb) {
+			int v = bsum[base + k];
+			bsum[base + k] = x;
+			x += v;
+		}
+	if (threadIdx.x == 0)
+		start[nt] = tot;
+}
+
+// pass 3: add each block's offset
+__global__ __launch_bounds__(SCAN_BS) void k_tr_scan_add(int nt, const int *bsum, int *start)
+{
+	int i = blockIdx.x * SCAN_BS + threadIdx.x;
+	if (i < nt)
+		start[i] += bsum[blockIdx.x];
+}
+
+__global__ __launch_bounds__(BS) void k_tr_place(long long nent, const int *tgt, const int *start, int *cursor,
+						  int *list)
+{
+	long long e = (long long) blockIdx.x * BS + threadIdx.x;
+	if (e >= nent)
+		return;
+	int t = tgt[e];
+	if (t >= 0)
+		list[start[t] + atomicAdd(&cursor[t], 1)] = (int) e;
+}
+
+// dest (targets [0, split) into d0, [split, nt) into d1, W values each) += the target's
+// entries in ascending id
+template <int W>
+__global__ __launch_bounds__(BS) void k_tr_gather(int nt, int split, const int *start, const int *list,
+						   const double *val, double *d0, double *d1)
+{
+	int j = blockIdx.x * BS + threadIdx.x;
+	if (j >= nt)
+		return;
+	int s = start[j], m = start[j + 1] - s;
+	if (m == 0)
+		return;
+	double acc[W];
+#pragma unroll
+	for (int w = 0; w < W; w++)
+		acc[w] = 0.0;
+	int prev = -1;
+	for (int k = 0; k < m; k++) {
+		int best = INT_MAX;
+		for (int q = 0; q < m; q++) {
+			int e = list[s + q];
+			if (e > prev && e < best)
+				best = e;
+		}
+		prev = best;
+#pragma unroll
+		for (int w = 0; w < W; w++)
+			acc[w] += val[(size_t) best * W + w];
+	}
+	double *d = j < split ? d0 + (size_t) j * W : d1 + (size_t) (j - split) * W;
+#pragma unroll
+	for (int w = 0; w < W; w++)
+		d[w] += acc[w];
 }
 
 // vel_i = p_i x sum_k gc_ik (psi_nbr(i,k) - psi_i): with c_ik = gc_ik . (lvel_i x p_i),
@@ -320,31 +479,55 @@ extern "C" void ggbk_adj_sum(ggbk_grid g, const double *x, ggbk_red red)
 	GGBK_LAUNCH(k_adj_sum, NB(g.n), BS, 0, g.n, x, red);
 }
 
+static TrEntries entries(const ggbk_adj_tr *t) { return TrEntries{ t->tgt, t->val }; }
+
 extern "C" void ggbk_adj_limiter(ggbk_grid g, const double *q, const double *q2, const double *dep_b,
 				 const double *band, const double *lam, const ggbk_scalars *s, double relax,
-				 double drelax, double c, int bfecc, double *lq, double *lq2, double *ldb,
-				 ggbk_red red)
+				 double drelax, double c, int bfecc, const ggbk_adj_tr *tr, double *ldb, ggbk_red red)
 {
-	GGBK_LAUNCH(k_adj_limiter, NB(g.n), BS, 0, g, q, q2, dep_b, band, lam, s, relax, drelax, c, bfecc, lq, lq2,
-		    ldb, red);
+	GGBK_LAUNCH(k_adj_limiter, NB(g.n), BS, 0, g, q, q2, dep_b, band, lam, s, relax, drelax, c, bfecc,
+		    entries(tr), ldb, red);
 }
 
 extern "C" void ggbk_adj_bfecc_f(ggbk_grid g, const double *lq2, const double *q1, const double *dep_f, double *lq,
-				 double *lq1, double *ldf)
+				 const ggbk_adj_tr *tr, double *ldf)
 {
-	GGBK_LAUNCH(k_adj_bfecc_f, NB(g.n), BS, 0, g, lq2, q1, dep_f, lq, lq1, ldf);
+	GGBK_LAUNCH(k_adj_bfecc_f, NB(g.n), BS, 0, g, lq2, q1, dep_f, lq, entries(tr), ldf);
 }
 
-extern "C" void ggbk_adj_advect_b(ggbk_grid g, const double *lq1, const double *q, const double *dep_b, double *lq,
-				  double *ldb)
+extern "C" void ggbk_adj_advect_b(ggbk_grid g, const double *lq1, const double *q, const double *dep_b,
+				  const ggbk_adj_tr *tr, double *ldb)
 {
-	GGBK_LAUNCH(k_adj_advect_b, NB(g.n), BS, 0, g, lq1, q, dep_b, lq, ldb);
+	GGBK_LAUNCH(k_adj_advect_b, NB(g.n), BS, 0, g, lq1, q, dep_b, entries(tr), ldb);
 }
 
 extern "C" void ggbk_adj_departures(ggbk_grid g, const double *vel, const double *ldb, const double *ldf, double dt,
-				    int bfecc, double *lvel)
+				    int bfecc, const ggbk_adj_tr *tr)
 {
-	GGBK_LAUNCH(k_adj_departures, NB(g.n), BS, 0, g, vel, ldb, ldf, dt, bfecc, lvel);
+	GGBK_LAUNCH(k_adj_departures, NB(g.n), BS, 0, g, vel, ldb, ldf, dt, bfecc, entries(tr));
+}
+
+extern "C" int ggbk_adj_transpose(const ggbk_adj_tr *t, long long nent, int W, int nt, int split, double *d0,
+				  double *d1)
+{
+	int nb = (nt + SCAN_BS - 1) / SCAN_BS;
+	if (nt < 1 || nt > t->cap_targets || nent > t->cap_entries || nb > SCAN_BS * TR_TOP || (W != 1 && W != 3))
+		return -1;
+	if (ggbk_zero(t->cnt, sizeof(int) * (size_t) nt) || ggbk_zero(t->cursor, sizeof(int) * (size_t) nt))
+		return -1;
+	unsigned eb = (unsigned) ((nent + BS - 1) / BS);
+	GGBK_LAUNCH(k_tr_count, eb, BS, 0, nent, (const int *) t->tgt, t->cnt);
+	GGBK_LAUNCH(k_tr_scan_local, (unsigned) nb, SCAN_BS, 0, nt, (const int *) t->cnt, t->start, t->bsum);
+	GGBK_LAUNCH(k_tr_scan_top, 1, SCAN_BS, 0, nb, nt, t->bsum, t->start);
+	GGBK_LAUNCH(k_tr_scan_add, (unsigned) nb, SCAN_BS, 0, nt, (const int *) t->bsum, t->start);
+	GGBK_LAUNCH(k_tr_place, eb, BS, 0, nent, (const int *) t->tgt, (const int *) t->start, t->cursor, t->list);
+	if (W == 1)
+		GGBK_LAUNCH((k_tr_gather<1>), NB(nt), BS, 0, nt, split, (const int *) t->start, (const int *) t->list,
+			    (const double *) t->val, d0, d1);
+	else
+		GGBK_LAUNCH((k_tr_gather<3>), NB(nt), BS, 0, nt, split, (const int *) t->start, (const int *) t->list,
+			    (const double *) t->val, d0, d1);
+	return 0;
 }
 
 extern "C" void ggbk_adj_velocity(ggbk_grid g, const double *lvel, const unsigned char *rslot, double *lpsi,

@@ -296,16 +296,36 @@ void ggbk_limited_relax(ggbk_grid g, const void *q2, const void *q, const void *
 
 /* ---- discrete adjoint of the fluid step (FP64 state; adj_k.hip) ---- */
 void ggbk_adj_sum(ggbk_grid g, const double *x, ggbk_red red);     /* slot MEAN partials of sum x */
+/* The transposes of the step's interpolations, without floating-point atomics: the
+ * scatter passes below write per-source entries (target index or -1, W values), and
+ * ggbk_adj_transpose adds to every target the sum of its entries in entry (source) order.
+ * Buffers for up to cap_entries entries and cap_targets targets. */
+typedef struct {
+	int *tgt;         /* [cap_entries] */
+	double *val;      /* [cap_entries][3] */
+	int *list;        /* [cap_entries] entry ids by target */
+	int *cnt, *cursor; /* [cap_targets] */
+	int *start;       /* [cap_targets + 1] */
+	int *bsum;        /* [cap_targets / 1024 + 1] */
+	long long cap_entries;
+	int cap_targets, pad_;
+} ggbk_adj_tr;
+/* d0[t] (t < split) or d1[t - split] (W values each) += the entries of target t in order;
+ * nent entries, nt targets.  -1 when the sizes exceed the buffers. */
+int ggbk_adj_transpose(const ggbk_adj_tr *t, long long nent, int W, int nt, int split, double *d0, double *d1);
+/* 4 entries per cell: targets [0, n) for lq, [n, 2n) for lq2 (W = 1) */
 void ggbk_adj_limiter(ggbk_grid g, const double *q, const double *q2, const double *dep_b,
 		      const double *band, const double *lam, const ggbk_scalars *s, double relax,
-		      double drelax, double c, int bfecc, double *lq, double *lq2, double *ldb,
-		      ggbk_red red);
+		      double drelax, double c, int bfecc, const ggbk_adj_tr *tr, double *ldb, ggbk_red red);
+/* lq += 1.5 lq2; 4 entries per cell for lq1 (W = 1) */
 void ggbk_adj_bfecc_f(ggbk_grid g, const double *lq2, const double *q1, const double *dep_f, double *lq,
-		      double *lq1, double *ldf);
-void ggbk_adj_advect_b(ggbk_grid g, const double *lq1, const double *q, const double *dep_b, double *lq,
-		       double *ldb);
+		      const ggbk_adj_tr *tr, double *ldf);
+/* 4 entries per cell for lq (W = 1) */
+void ggbk_adj_advect_b(ggbk_grid g, const double *lq1, const double *q, const double *dep_b,
+		       const ggbk_adj_tr *tr, double *ldb);
+/* 10 entries per cell for lvel (W = 3) */
 void ggbk_adj_departures(ggbk_grid g, const double *vel, const double *ldb, const double *ldf, double dt,
-			 int bfecc, double *lvel);
+			 int bfecc, const ggbk_adj_tr *tr);
 void ggbk_adj_velocity(ggbk_grid g, const double *lvel, const unsigned char *rslot, double *lpsi, ggbk_red red);
 void ggbk_adj_rhs(ggbk_grid g, const double *lpsi, const ggbk_scalars *s, double sum_area, double *om);
 void ggbk_adj_combine(ggbk_grid g, const double *lq, const double *z, double *lam);
