@@ -659,12 +659,78 @@ static void h_vcycle(const HostLevel *hl, int nlev, int l, const double *b, doub
 		h_smooth(L, b, x, r, 1);
 }
 
+/* the same on the Galerkin hierarchy of the mixed precision (levels l.. of gal, the
+ * transfers of hl): Chebyshev smoothing of degree deg, 60 damped Jacobi sweeps on the
+ * coarsest level */
+static double gK(const GalOp *G, const double *x, int i)
+{
+	const int *nb = G->nbr + (size_t) GGBK_WNB * i;
+	const double *w = G->w + (size_t) GGBK_WNB * i;
+	double xi = x[i], s = 0;
+	for (int k = 0; k < GGBK_WNB; k++)
+		s += w[k] * (xi - x[nb[k]]);
+	return s;
+}
+
+static void g_cheb(const GalOp *G, const double *b, double *x, double *r, double *d, int deg, int jacobi)
+{
+	int n = G->n;
+	double sigma = 5.0 / 3.0, delta = 0.75, rho = 1.0 / sigma;
+	for (int k = 0; k < deg; k++) {
+		double cd = 0.0, cr = 0.8;
+		if (k > 0 && !jacobi) {
+			double rn = 1.0 / (2 * sigma - rho);
+			cd = rn * rho;
+			cr = 2 * rn / delta;
+			rho = rn;
+		}
+		for (int i = 0; i < n; i++)
+			r[i] = b[i] - gK(G, x, i);
+		for (int i = 0; i < n; i++) {
+			d[i] = (k ? cd * d[i] : 0.0) + cr * (double) G->dinv[i] * r[i];
+			x[i] += d[i];
+		}
+	}
+}
+
+static void g_vcycle(const HostLevel *hl, const GalOp *gal, int nlev, int l, const double *b, double *x,
+		     double **bufs, int deg)
+{
+	const GalOp *G = &gal[l];
+	const HostLevel *L = &hl[l];
+	int n = G->n;
+	double *r = bufs[l], *d = bufs[l] + 3 * (size_t) n;
+	memset(x, 0, sizeof(double) * n);
+	if (l == nlev - 1) {
+		g_cheb(G, b, x, r, d, 60, 1);
+		return;
+	}
+	g_cheb(G, b, x, r, d, deg, 0);
+	for (int i = 0; i < n; i++)
+		r[i] = b[i] - gK(G, x, i);
+	int nc = gal[l + 1].n;
+	double *bc = bufs[l + 1] + nc, *xc = bufs[l + 1] + 2 * (size_t) nc;
+	for (int c = 0; c < nc; c++) {
+		double acc = 0;
+		for (int k = L->rstart[c]; k < L->rstart[c + 1]; k++)
+			acc += L->rw[k] * r[L->ridx[k]];
+		bc[c] = acc;
+	}
+	g_vcycle(hl, gal, nlev, l + 1, bc, xc, bufs, deg);
+	for (int i = 0; i < n; i++) {
+		const int *t = L->ptap + 4 * i;
+		const float *w = L->pw + 4 * i;
+		x[i] += w[0] * xc[t[0]] + w[1] * xc[t[1]] + w[2] * xc[t[2]] + w[3] * xc[t[3]];
+	}
+	g_cheb(G, b, x, r, d, deg, 0);
+}
+
 #define DENSE_MAX 1536
 
 /* Tabulate the sub-V-cycle of level L as M[i][j] = (V_L e_j)_i and upload it: the
  * reference's V(1,1) cycle (deg 1), or the mixed precision's Chebyshev cycle (deg 3, as the
- * levels above it smooth). */
-static void build_dense(struct ggb_fluid_dev *D, int L, int deg)
+ * levels above it smooth), or with gal the Chebyshev cycle of the Galerkin hierarchy. */
+static void build_dense(struct ggb_fluid_dev *D, int L, int deg, const GalOp *gal)
 {
 	int n = D->hl[L].cs->ncells;
 	double *M = malloc(sizeof(double) * (size_t) n * n);
@@ -680,7 +746,10 @@ static void build_dense(struct ggb_fluid_dev *D, int L, int deg)
 		for (int j = 0; j < n; j++) {
 			memset(e, 0, sizeof(double) * n);
 			e[j] = 1.0;
-			h_vcycle(D->hl, D->nlev, L, e, x, bufs, deg);
+			if (gal)
+				g_vcycle(D->hl, gal, D->nlev, L, e, x, bufs, deg);
+			else
+				h_vcycle(D->hl, D->nlev, L, e, x, bufs, deg);
 			for (int i = 0; i < n; i++)
 				M[(size_t) i * n + j] = x[i];
 		}
@@ -726,22 +795,27 @@ static int cell_is_regular(const CubeSphere *cs, int i)
  * stencils on rings < deg is interior (no tables); any other gets graph tables: its ext
  * cells ring by ring in BFS order (ring r = the unvisited stencil neighbours of ring r - 1,
  * in discovery order) and each stencil neighbour of rings < deg as a slot of that list.
- * Returns -1 when a graph tile would exceed capmax ext cells. */
-static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, const HostLevel *rl, int rana,
-		       int capmax, ggbk_tiles *out)
+ * With gal the stencil is that Galerkin operator's (two cells wide: an interior tile's
+ * square then has 2 deg rings, one more with the restriction, and its cells on rings
+ * <= hs - 2 must be regular).  Returns -1 when a graph tile would exceed capmax ext cells. */
+static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, const GalOp *gal, int deg, const HostLevel *rl,
+		       int rana, int capmax, ggbk_tiles *out)
 {
 	/* rl != NULL: tiles of a down pass, whose restriction through rl's R into the next
-	 * level is fused; an interior tile then has deg + 1 rings (the rows of R reach one ring
+	 * level is fused; an interior tile then has one more ring (the rows of R reach one ring
 	 * beyond the fine tile) and the bilinear row pattern (rana) */
-	int hs = rl ? deg + 1 : deg;
+	int rad = gal ? 2 : 1, S = gal ? GGBK_WNB : 8;
+	if (deg + 4 > GGBK_TINFO)
+		ggb_die("fluid: %d tile rings exceed the tile record", deg);
+	int hs = rad * deg + (rl ? 1 : 0);
 	int N = cs->N, n = cs->ncells, tpe = N / MGT_TILE, ntiles = 6 * tpe * tpe;
 	unsigned char *reg = malloc((size_t) n);
 	int *kind = malloc(sizeof(int) * (size_t) ntiles);
-	int *tinfo = calloc((size_t) ntiles * 8, sizeof(int));
+	int *tinfo = calloc((size_t) ntiles * GGBK_TINFO, sizeof(int));
 	int *stamp = malloc(sizeof(int) * (size_t) n), *loc = malloc(sizeof(int) * (size_t) n);
 	size_t ecap = (size_t) capmax * 64, ne = 0;
 	int *gcell = malloc(sizeof(int) * ecap);
-	unsigned short *gnbr = malloc(sizeof(unsigned short) * 8 * ecap);
+	unsigned short *gnbr = malloc(sizeof(unsigned short) * S * ecap);
 	int *goff = malloc(sizeof(int) * ((size_t) ntiles + 1));
 	int *gring = malloc(sizeof(int) * (size_t) ntiles * (deg + 1));
 	size_t rcap = rl ? (size_t) 64 * GGBK_TROW * 64 : 1, nr = 0;
@@ -751,7 +825,7 @@ static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, c
 		ggb_die("fluid: out of memory");
 	#pragma omp parallel for schedule(static)
 	for (int i = 0; i < n; i++) {
-		reg[i] = (unsigned char) cell_is_regular(cs, i);
+		reg[i] = gal ? gal->reg[i] : (unsigned char) cell_is_regular(cs, i);
 		stamp[i] = -1;
 	}
 	/* GGB_TILES_GRAPH=1: every tile through the graph tables (both forms give the same bits) */
@@ -763,8 +837,8 @@ static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, c
 		int J0 = (r / tpe) * MGT_TILE, I0 = (r % tpe) * MGT_TILE;
 		int interior = !all_graph && (!rl || rana) && I0 >= hs && J0 >= hs && I0 + MGT_TILE - 1 + hs <= N - 1 &&
 			       J0 + MGT_TILE - 1 + hs <= N - 1;
-		for (int y = J0 - (hs - 1); interior && y <= J0 + MGT_TILE - 1 + (hs - 1); y++)
-			for (int x = I0 - (hs - 1); x <= I0 + MGT_TILE - 1 + (hs - 1); x++)
+		for (int y = J0 - (hs - rad); interior && y <= J0 + MGT_TILE - 1 + (hs - rad); y++)
+			for (int x = I0 - (hs - rad); x <= I0 + MGT_TILE - 1 + (hs - rad); x++)
 				if (!reg[(f * N + y) * N + x]) {
 					interior = 0;
 					break;
@@ -779,7 +853,7 @@ static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, c
 		if (ne + capmax > ecap) {
 			ecap = 2 * ecap + capmax;
 			gcell = realloc(gcell, sizeof(int) * ecap);
-			gnbr = realloc(gnbr, sizeof(unsigned short) * 8 * ecap);
+			gnbr = realloc(gnbr, sizeof(unsigned short) * S * ecap);
 			if (!gcell || !gnbr)
 				ggb_die("fluid: out of memory");
 		}
@@ -827,8 +901,9 @@ static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, c
 			int rend = cnt;
 			for (int e = rs; e < rend && rc == 0; e++) {
 				int c = gcell[base + e];
-				for (int k = 0; k < cs->nnb[c]; k++) {
-					int j = cs->nbr[(size_t) CS_MAXNB * c + k];
+				int nn = gal ? gal->nn[c] : cs->nnb[c];
+				for (int k = 0; k < nn; k++) {
+					int j = gal ? gal->nbr[(size_t) GGBK_WNB * c + k] : cs->nbr[(size_t) CS_MAXNB * c + k];
 					if (stamp[j] == t)
 						continue;
 					if (cnt == capmax) {
@@ -845,20 +920,21 @@ static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, c
 		}
 		for (int e = 0; e < re[deg - 1] && rc == 0; e++) {
 			int c = gcell[base + e];
-			for (int k = 0; k < 8; k++) {
+			int nn = gal ? gal->nn[c] : cs->nnb[c];
+			for (int k = 0; k < S; k++) {
 				int slot = e;
-				if (k < cs->nnb[c]) {
-					int j = cs->nbr[(size_t) CS_MAXNB * c + k];
+				if (k < nn) {
+					int j = gal ? gal->nbr[(size_t) GGBK_WNB * c + k] : cs->nbr[(size_t) CS_MAXNB * c + k];
 					if (stamp[j] != t)
 						ggb_die("fluid: tile %d ring cell %d has a neighbour outside its rings", t, c);
 					slot = loc[j];
 				}
-				gnbr[8 * (base + e) + k] = (unsigned short) slot;
+				gnbr[S * (base + e) + k] = (unsigned short) slot;
 			}
 		}
 		for (int e = re[deg - 1]; e < cnt; e++)
-			for (int k = 0; k < 8; k++)
-				gnbr[8 * (base + e) + k] = (unsigned short) e;
+			for (int k = 0; k < S; k++)
+				gnbr[S * (base + e) + k] = (unsigned short) e;
 		if (rl && rc == 0) {
 			int Nc = N / 2;
 			for (int q = 0; q < 64; q++) {
@@ -890,7 +966,7 @@ static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, c
 			int f = t / (tpe * tpe), r = t - f * tpe * tpe;
 			int J0 = (r / tpe) * MGT_TILE, I0 = (r % tpe) * MGT_TILE;
 			int gi = I0 >= hs && J0 >= hs && I0 + MGT_TILE - 1 + hs <= N - 1 && J0 + MGT_TILE - 1 + hs <= N - 1;
-			int *ti = tinfo + 8 * (size_t) t;
+			int *ti = tinfo + GGBK_TINFO * (size_t) t;
 			ti[0] = kind[t];
 			if (kind[t] >= 0) {
 				int k = kind[t];
@@ -902,16 +978,19 @@ static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, c
 			if (gi != (kind[t] < 0))
 				geo = 0;
 		}
-		/* the weight class of each ext cell, from the device stencil records */
-		ggbk_grid gg = ggb_grid_dev(cs);
-		int *srec = malloc(sizeof(int) * 2 * (size_t) n);
+		/* the weight class of each ext cell, from the device stencil records (a Galerkin
+		 * level's weights are per cell: class 0) */
 		int *gcc = malloc(sizeof(int) * 2 * (ne + 1));
-		if (!gcc || !srec)
+		int *srec = gal ? NULL : malloc(sizeof(int) * 2 * (size_t) n);
+		if (!gcc || (!gal && !srec))
 			ggb_die("fluid: out of memory");
-		download(srec, gg.srec, sizeof(int) * 2 * (size_t) n);
+		if (!gal) {
+			ggbk_grid gg = ggb_grid_dev(cs);
+			download(srec, gg.srec, sizeof(int) * 2 * (size_t) n);
+		}
 		for (size_t e = 0; e < ne; e++) {
 			gcc[2 * e] = gcell[e];
-			gcc[2 * e + 1] = srec[2 * (size_t) gcell[e]] & 0x7fffffff;
+			gcc[2 * e + 1] = gal ? 0 : srec[2 * (size_t) gcell[e]] & 0x7fffffff;
 		}
 		free(srec);
 		memset(out, 0, sizeof(*out));
@@ -921,15 +1000,16 @@ static int build_tiles(struct ggb_fluid_dev *D, const CubeSphere *cs, int deg, c
 		out->ngraph = ngraph;
 		out->geo = geo;
 		out->hs = hs;
-		void *d = xbuf(D, sizeof(int) * 8 * (size_t) ntiles, "multigrid tiles");
-		upload(d, tinfo, sizeof(int) * 8 * (size_t) ntiles);
+		out->nslot = S;
+		void *d = xbuf(D, sizeof(int) * GGBK_TINFO * (size_t) ntiles, "multigrid tiles");
+		upload(d, tinfo, sizeof(int) * GGBK_TINFO * (size_t) ntiles);
 		out->tinfo = d;
 		d = xbuf(D, sizeof(int) * 2 * (ne + 1), "multigrid tiles");
 		upload(d, gcc, sizeof(int) * 2 * ne);
 		out->gcc = d;
 		free(gcc);
-		d = xbuf(D, sizeof(unsigned short) * 8 * (ne + 1), "multigrid tiles");
-		upload(d, gnbr, sizeof(unsigned short) * 8 * ne);
+		d = xbuf(D, sizeof(unsigned short) * S * (ne + 1), "multigrid tiles");
+		upload(d, gnbr, sizeof(unsigned short) * S * ne);
 		out->gnbr = d;
 		if (rl) {
 			d = xbuf(D, sizeof(unsigned short) * (nr + 1), "multigrid tiles");
@@ -967,10 +1047,10 @@ static int dense_level_of(int prec, int N, int nlev)
 	return -1;
 }
 
-/* The mixed-precision preconditioner (degree 2 on level 0, 3 below) needs a dense level
- * below level 0 and every level above it a multiple of 16 cells wide; GGB_MIXED_CHEB=0 in
- * the environment selects the reference V(1,1) cycle instead.  setup_cheb builds the
- * tiles of those levels. */
+/* The mixed-precision preconditioner (Chebyshev smoothing on face tiles, Galerkin coarse
+ * levels) needs a dense level below level 0 and every level above it a multiple of 16
+ * cells wide; GGB_MIXED_CHEB=0 in the environment selects the reference V(1,1) cycle
+ * instead.  setup_cheb builds the tiles of those levels. */
 static int cheb_eligible(int prec, int N, int nlev)
 {
 	if (prec != GGB_PREC_MIXED)
@@ -987,24 +1067,55 @@ static int cheb_eligible(int prec, int N, int nlev)
 	return 1;
 }
 
-/* capacities of the tiled passes' workgroups (mgt_k.hip): 2 ext cells per thread, 3 in
- * the down passes of the coarse levels (degree 3, one ring more) */
-#define MGT_CAP_UP 512
-#define MGT_CAP_DOWN0 512
-#define MGT_CAP_DOWN 768
+/* capacities of the tiled passes' workgroups (mgt_k.hip): 2 ext cells per thread for
+ * level 0 with degree 2, else 3 (level 0 of higher degree, the coarse down passes, the
+ * Galerkin levels' passes) */
+#define MGT_CAP2 512
+#define MGT_CAP3 768
 
-static void setup_cheb(struct ggb_fluid_dev *D, const CubeSphere *cs)
+/* The mixed hierarchy's options: GGB_MIXED_GALERKIN=0 keeps the levels' own Voronoi
+ * operators (degree 3 below level 0, degree 2 on level 0); GGB_MIXED_DEG0 sets level 0's
+ * degree with the Galerkin levels (2-4, default 4: at N = 512 and tol 1e-8, 5 / 4 / 3 CG
+ * iterations per step). */
+static int mixed_galerkin(void)
+{
+	const char *e = getenv("GGB_MIXED_GALERKIN");
+	return !(e && !strcmp(e, "0"));
+}
+
+static int mixed_deg0(void)
+{
+	const char *e = getenv("GGB_MIXED_DEG0");
+	int d = e ? atoi(e) : 4;
+	return d < 2 ? 2 : d > 4 ? 4 : d;
+}
+
+static void setup_cheb(struct ggb_fluid_dev *D, const CubeSphere *cs, const GalOp *gal, int deg0)
 {
 	for (int l = 0; l < D->dense_level; l++) {
 		const CubeSphere *c = l == 0 ? cs : &D->coarse[l];
-		int deg = l == 0 ? 2 : 3, capd = l == 0 ? MGT_CAP_DOWN0 : MGT_CAP_DOWN;
+		const GalOp *G = gal && l > 0 ? &gal[l] : NULL;
+		int deg = l == 0 ? deg0 : G ? 2 : 3;
+		int capu = (l == 0 && deg0 == 2) || (l > 0 && !G) ? MGT_CAP2 : MGT_CAP3;
+		int capd = l == 0 && deg0 == 2 ? MGT_CAP2 : MGT_CAP3;
 		/* level 0's up pass takes one ring more (A z on the tile from z on ring 1) */
 		int up_rings = l == 0 ? deg + 1 : deg;
-		if (build_tiles(D, c, up_rings, NULL, 0, MGT_CAP_UP, &D->levels[l].tl))
-			ggb_die("fluid_init: a multigrid tile of level %d exceeds %d cells", l, MGT_CAP_UP);
-		if (build_tiles(D, c, deg, &D->hl[l], D->levels[l].xf.rana, capd, &D->levels[l].td))
+		if (build_tiles(D, c, G, up_rings, NULL, 0, capu, &D->levels[l].tl))
+			ggb_die("fluid_init: a multigrid tile of level %d exceeds %d cells", l, capu);
+		if (build_tiles(D, c, G, deg, &D->hl[l], D->levels[l].xf.rana, capd, &D->levels[l].td))
 			ggb_die("fluid_init: a multigrid tile of level %d exceeds %d cells", l, capd);
 		D->levels[l].xpre = xbuf(D, sizeof(float) * (size_t) c->ncells, "multigrid");
+		if (G) {
+			ggbk_wide *wd = &D->levels[l].wd;
+			wd->n = G->n;
+			wd->N = G->N;
+			void *d = xbuf(D, sizeof(unsigned short) * GGBK_WNB * (size_t) G->n, "multigrid");
+			upload(d, G->wh, sizeof(unsigned short) * GGBK_WNB * (size_t) G->n);
+			wd->w = d;
+			d = xbuf(D, sizeof(float) * (size_t) G->n, "multigrid");
+			upload(d, G->dinv, sizeof(float) * (size_t) G->n);
+			wd->dinv = d;
+		}
 	}
 	D->Az = xbuf(D, sizeof(double) * (size_t) cs->ncells, "fluid");
 }
@@ -1113,14 +1224,36 @@ int fluid_init(Fluid *F, const CubeSphere *cs, const FluidParams *prm)
 				D->hl[l].dinv[i] = (float) (1.0 / sw);
 			}
 		}
+		/* the mixed hierarchy's Galerkin operators K_l = R K_(l-1) P, levels 1.. */
+		GalOp *gal = NULL;
+		if (cheb && mixed_galerkin()) {
+			gal = calloc((size_t) D->nlev, sizeof(GalOp));
+			if (!gal)
+				ggb_die("fluid: out of memory");
+			for (int l = 1; l < D->nlev && gal; l++) {
+				const HostLevel *h = &D->hl[l - 1];
+				if (ggb_galerkin(cs, l == 1 ? NULL : &gal[l - 1], h->ptap, h->pw, h->rstart, h->ridx, h->rw,
+						 cs->N >> l, &gal[l])) {
+					for (int q = 1; q < l; q++)
+						ggb_galop_free(&gal[q]);
+					free(gal);
+					gal = NULL;
+				}
+			}
+		}
 		/* the bottom of the V-cycle as one dense operator */
 		int L = dense_level_of(D->prec, cs->N, D->nlev);
 		D->dense_level = L;
 		if (D->dense_level > 0)
-			build_dense(D, L, cheb ? 3 : 1);
+			build_dense(D, L, cheb ? (gal ? 2 : 3) : 1, gal);
 		if (cheb) {
-			setup_cheb(D, cs);
+			setup_cheb(D, cs, gal, gal ? mixed_deg0() : 2);
 			D->cheb = 1;
+		}
+		if (gal) {
+			for (int l = 1; l < D->nlev; l++)
+				ggb_galop_free(&gal[l]);
+			free(gal);
 		}
 		for (int l = 0; l < D->nlev; l++)
 			host_level_free(&D->hl[l]);

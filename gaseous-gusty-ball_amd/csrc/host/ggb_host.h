@@ -40,4 +40,28 @@ const void *ggb_fluid_dev_vel(const Fluid *F, int *T);
 void ggb_particles_dev_parts(const Particles *P, ggbk_parts *cur, ggbk_parts *nxt);
 void ggb_particles_dev_swap(const Particles *P);
 
+/* A Galerkin coarse operator of the mixed-precision hierarchy on the host (galerkin.c):
+ * (K x)_i = sum_k w[i][k] (x_i - x[nbr[i][k]]), GGBK_WNB slots per cell.  A regular cell
+ * has its slots at the fixed 5x5 offsets (ggbk_wide); any other lists its nn neighbours by
+ * ascending index, self-padded. */
+typedef struct {
+	int n, N;
+	int *nbr;            /* [n][GGBK_WNB] */
+	double *w;           /* [n][GGBK_WNB] the stored half floats' values, zero-padded */
+	unsigned short *wh;  /* [n][GGBK_WNB] their bits */
+	unsigned char *nn;   /* [n] slots in use (GGBK_WNB for a regular cell) */
+	unsigned char *reg;  /* [n] */
+	float *dinv;         /* [n] 1 / sum_k w[i][k] */
+} GalOp;
+
+/* K_c = R K_f P into *out for the coarse grid of face size Nc: K_f the Voronoi operator
+ * of cs (gf == NULL) or the Galerkin operator gf; P given per fine cell by 4 taps and
+ * weights, R = P^T as rows (rstart, ridx, rw).  Returns -1 (nothing allocated) when a
+ * row does not fit the slots. */
+int ggb_galerkin(const CubeSphere *cs, const GalOp *gf, const int *ptap, const float *pw, const int *rstart,
+		 const int *ridx, const float *rw, int Nc, GalOp *out);
+void ggb_galop_free(GalOp *g);
+unsigned short ggb_f16_bits(double v);
+double ggb_f16_value(unsigned short h);
+
 #endif

@@ -208,12 +208,16 @@ typedef struct {
 	int geo;                        /* 1: the interior tiles are exactly those whose hs
 	                                 * rings lie within their face (the kernel then tells
 	                                 * them by their index alone) */
-	int hs, pad_[2];                /* rings of an interior tile */
-	const int *tinfo;               /* [ntiles][8] kind (-1 interior, else graph index),
-	                                 * first ext cell, ext count, ext cells in rings 0..r */
+	int hs;                         /* rings of an interior tile */
+	int nslot;                      /* stencil slots per ext cell in gnbr: 8 (Voronoi) or
+	                                 * GGBK_WNB (a Galerkin level) */
+	int pad_;
+	const int *tinfo;               /* [ntiles][GGBK_TINFO] kind (-1 interior, else graph
+	                                 * index), first ext cell, ext count, ext cells in rings
+	                                 * 0..r (r <= 5) */
 	const int *gcc;                 /* [ext][2] cell index and stencil weight class of each
 	                                 * ext cell of the graph tiles, ring by ring */
-	const unsigned short *gnbr;     /* [ext][8] slot of each stencil neighbour, self-padded */
+	const unsigned short *gnbr;     /* [ext][nslot] slot of each stencil neighbour, self-padded */
 	/* tiles of a down pass with the restriction fused: ring 0 of a graph tile is the fine
 	 * support of the rows of R of the tile's 8x8 coarse cells (its own 16x16 cells first),
 	 * and each row is listed by slot; an interior tile's rows follow the bilinear pattern */
@@ -221,9 +225,25 @@ typedef struct {
 	const float *grw;               /* [ngraph][64][GGBK_TROW] their weights (zero-padded) */
 } ggbk_tiles;
 #define GGBK_TROW GGBK_RMAX
+#define GGBK_TINFO 12
+
+/* A Galerkin coarse operator of the mixed-precision hierarchy, K_c = R K_f P, in the
+ * form (K x)_i = sum_k w_ik (x_i - x_nb(k)) with w_ik = -(R K_f P)_ij (zero row sums).  A
+ * regular cell (its neighbours on its own face within two cells each way) has its slots
+ * at the fixed offsets of the 5x5 square around it, row-major without the centre; any
+ * other cell lists its neighbours by ascending index (in the graph tiles' tables).
+ * Weights are stored as IEEE half floats. */
+#define GGBK_WNB 24
+typedef struct {
+	int n, N;
+	const unsigned short *w;  /* [n][GGBK_WNB] f16 bits of w_ik, slot order, zero-padded */
+	const float *dinv;        /* [n] 1 / sum_k w_ik */
+} ggbk_wide;
 
 typedef struct {
 	ggbk_grid g;
+	ggbk_wide wd;     /* the Galerkin operator of a coarse level of the mixed hierarchy
+	                   * (wd.w == NULL: the level's own Voronoi stencil) */
 	ggbk_xfer xf;     /* transfer to the next coarser level (unused on the coarsest) */
 	void *x, *b, *r;  /* TM scratch */
 	const void *M;    /* dense sub-V-cycle operator, or NULL */
@@ -243,10 +263,12 @@ typedef struct {
 } ggbk_cg_fuse;
 int ggbk_vcycle_update(const ggbk_level *levels, int nlev, const ggbk_cg_fuse *u, void *r_out, int T,
 		       void *zout, int TM, ggbk_scalars *s, ggbk_red red);
-/* The mixed-precision preconditioner: a V-cycle with Chebyshev smoothing (degree 2 on
- * level 0, 3 below; 1/2 <= lambda(D^-1 K) <= 2, the Gershgorin bound) on the tiled levels,
- * float storage and arithmetic, down to the first level with a dense operator (levels[0..L-1]
- * tiled, levels[L].M set, L >= 1; level 0's up tiles have 3 rings).
+/* The mixed-precision preconditioner: a V-cycle with Chebyshev smoothing (level 0 of
+ * degree levels[0].td.deg, 2-4; the coarse levels of degree 2 on their Galerkin operators
+ * levels[l].wd, or of degree 3 on their own Voronoi operators; 1/2 <= lambda(D^-1 K) <= 2,
+ * the Gershgorin bound) on the tiled levels, float storage and arithmetic, down to the
+ * first level with a dense operator (levels[0..L-1] tiled, levels[L].M set, L >= 1; level
+ * 0's up tiles have one ring more than its degree).
  * c == NULL: z = M r (stores gated by s->vc).  Otherwise the PCG's V-cycle: with c->update
  * its first pass applies the update (the direction p = z + beta p, Ap = Az + beta Ap, first:
  * p = z, Ap = Az; psi += alpha p; r_out = r - alpha Ap) and preconditions r_out, else it

@@ -3,12 +3,17 @@ infrastructure: the checker of the GPU's Chebyshev V-cycle (csrc/kern/mgt_k.hip)
 
 The V-cycle follows the reference's (fluid.c:113-146: zero start, pre-smooth, residual,
 restriction R = P^T, coarse correction, prolongation, post-smooth, 60 damped Jacobi
-sweeps on the coarsest level) with one change the mixed mode makes: the smoother is the
-Chebyshev iteration on D^-1 K for the interval [1/2, 2], degree 2 on level 0 and 3 below
-(the library tabulates the cycle below its first level of at most 1536 cells as a dense
-matrix; that changes only the rounding).  With degree 1 everywhere the Chebyshev step is
-the damped Jacobi step, so the restatement is then the reference V-cycle, which the CPU
-tests check against the oracle's.
+sweeps on the coarsest level) with the changes the mixed mode makes:
+  * the smoother is the Chebyshev iteration on D^-1 K for the interval [1/2, 2], of
+    degree 4 on level 0 and 2 below;
+  * the coarse levels carry Galerkin operators K_(l+1) = R_l K_l P_l instead of their own
+    Voronoi Laplacians, in the form (K x)_i = sum_j w_ij (x_i - x_j) with the symmetrised
+    off-diagonal weights w_ij = -(K_ij + K_ji) / 2 rounded to half precision (the
+    library's storage), each level formed from the rounded level above.
+The library tabulates the cycle below its first level of at most 1536 cells as a dense
+matrix; that changes only the rounding.  With the rediscretised levels and degree 1
+everywhere the Chebyshev step is the damped Jacobi step, so the restatement is then the
+reference V-cycle, which the CPU tests check against the oracle's.
 """
 import numpy as np
 import scipy.sparse as sp
@@ -73,12 +78,27 @@ def cheb_coefficients(deg):
     return out
 
 
+def galerkin(R, K, P, half=True):
+    """R K P as (K x)_i = sum_j w_ij (x_i - x_j): symmetrised off-diagonal weights
+    -(G_ij + G_ji) / 2, rounded to half precision when half (the library's storage)."""
+    G = (R @ K @ P).tocoo()
+    off = G.row != G.col
+    W = sp.csr_matrix((-G.data[off], (G.row[off], G.col[off])), shape=G.shape)
+    W = ((W + W.T) * 0.5).tocsr()
+    if half:
+        W.data = W.data.astype(np.float16).astype(np.float64)
+    W.eliminate_zeros()
+    return (sp.diags(np.asarray(W.sum(axis=1)).ravel()) - W).tocsr()
+
+
 class MixedVcycle:
-    def __init__(self, levels, transfers):
-        """levels: ggb.Grid per level; transfers[l] = (taps [nf, 4], weights [nf, 4])."""
+    def __init__(self, levels, transfers, galerkin_levels=True, deg0=4):
+        """levels: ggb.Grid per level; transfers[l] = (taps [nf, 4], weights [nf, 4]).
+        galerkin_levels: coarse operators R K P (else each level's own Voronoi operator,
+        degree 3 below level 0 and deg0 = 2: the hierarchy without them)."""
         self.levels = levels
-        self.K = [kmat(g) for g in levels]
-        self.dinv = [np.float32(1.0 / K.diagonal()).astype(np.float64) for K in self.K]
+        self.galerkin = galerkin_levels
+        self.deg0 = deg0 if galerkin_levels else 2
         self.P = []
         for l, (pt, pw) in enumerate(transfers):
             nf, nc = levels[l].n, levels[l + 1].n
@@ -86,6 +106,11 @@ class MixedVcycle:
                                          (np.repeat(np.arange(nf), 4), np.asarray(pt).reshape(nf, 4).ravel())),
                                         shape=(nf, nc)))
         self.R = [P.T.tocsr() for P in self.P]
+        self.K = [kmat(levels[0])]
+        for l in range(1, len(levels)):
+            self.K.append(galerkin(self.R[l - 1], self.K[l - 1], self.P[l - 1]) if galerkin_levels
+                          else kmat(levels[l]))
+        self.dinv = [np.float32(1.0 / K.diagonal()).astype(np.float64) for K in self.K]
 
     def smooth(self, l, b, x, deg):
         """deg Chebyshev steps from x on K x = b."""
@@ -102,8 +127,12 @@ class MixedVcycle:
         return x
 
     def __call__(self, b, l=0, degrees=None):
-        """z = M b; degrees(l) gives the smoothing degree (default 2 on level 0, 3 below)."""
-        deg = (2 if l == 0 else 3) if degrees is None else degrees(l)
+        """z = M b; degrees(l) gives the smoothing degree (default deg0 on level 0, below it
+        2 with the Galerkin levels, 3 without)."""
+        if degrees is None:
+            deg = self.deg0 if l == 0 else (2 if self.galerkin else 3)
+        else:
+            deg = degrees(l)
         n = self.levels[l].n
         if l == len(self.levels) - 1:
             return self.jacobi(l, b, np.zeros(n), 60)

@@ -1,7 +1,7 @@
 """The mixed-precision preconditioner on the GPU (csrc/kern/mgt_k.hip: Chebyshev
-smoothing of degree 2 on level 0 and 3 below, fused tile passes, float storage) against
-its float64 restatement (tests/mixed_precond.py, pinned by test_mixed_precond.py), and
-its effect on the PCG:
+smoothing of degree 4 on level 0 and 2 on the Galerkin coarse levels below, fused tile
+passes, float storage) against its float64 restatement (tests/mixed_precond.py, pinned by
+test_mixed_precond.py), and its effect on the PCG:
 
   * the V-cycle applied through fluid_vcycle_f64 matches the restatement (with the exact
     bilinear face-interior transfer weights of the Chebyshev hierarchy) to float rounding
@@ -35,7 +35,7 @@ def restatement(ggb, orc, N):
         if chebyshev_hierarchy(N):
             pt, pw = exact_weights(pt, pw, levels[l].N)
         tr.append((pt, pw))
-    return levels, MixedVcycle(levels, tr)
+    return levels, MixedVcycle(levels, tr, galerkin_levels=chebyshev_hierarchy(N))
 
 
 def rel(a, b):

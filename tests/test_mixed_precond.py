@@ -1,11 +1,18 @@
 """Pins the float64 restatement of the mixed-precision preconditioner (tests/
 mixed_precond.py), the checker of the GPU's Chebyshev V-cycle:
 
-  * with degree 1 everywhere its Chebyshev step is the damped Jacobi step, and the
-    V-cycle equals the oracle's reference V-cycle (fluid.c:113-146);
+  * with the levels' own Voronoi operators and degree 1 everywhere its Chebyshev step is
+    the damped Jacobi step, and the V-cycle equals the oracle's reference V-cycle
+    (fluid.c:113-146);
   * its smoother's error propagator is the Chebyshev polynomial for [1/2, 2],
     q_k(lam) = T_k((5/2 - 2 lam) / (3/2)) / T_k(5/3), evaluated through an
     eigendecomposition of D^-1 K;
+  * its Galerkin coarse operators: symmetric with zero row sums, their half-precision
+    weights within half a unit in the last place of the products, a 5x5 stencil on the
+    face interior, the same energy as the level's own Voronoi operator on smooth fields,
+    and -- what defines them -- the two-grid coarse correction I - P K_c^+ R K_f they
+    give is a K_f-orthogonal projection (it is not with the Voronoi coarse operator, nor
+    with a planted mistake in the product's diagonal);
   * the V-cycle with the default degrees is symmetric (the same polynomial before and
     after the coarse correction);
   * the exact bilinear face-interior transfer weights of the Chebyshev hierarchy are the
@@ -15,24 +22,25 @@ import ctypes as C
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 from conftest import levels_for
-from mixed_precond import MixedVcycle, chebyshev_hierarchy, exact_weights, kmat
+from mixed_precond import MixedVcycle, chebyshev_hierarchy, exact_weights, galerkin, kmat
 
 
-def make(ggb, orc, N):
+def make(ggb, orc, N, galerkin_levels=True):
     levels = levels_for(ggb, orc, N)
     O = orc.OracleFluid("f64", levels)
     tr = []
     for l in range(len(levels) - 1):
         pt, pw = O.transfer(l)
         tr.append((np.asarray(pt).reshape(-1, 4), np.asarray(pw).reshape(-1, 4)))
-    return levels, O, MixedVcycle(levels, tr)
+    return levels, O, MixedVcycle(levels, tr, galerkin_levels)
 
 
 @pytest.mark.parametrize("N", [16, 32])
 def test_degree_one_is_the_reference_vcycle(ggb, orc, N):
-    levels, O, M = make(ggb, orc, N)
+    levels, O, M = make(ggb, orc, N, galerkin_levels=False)
     b = np.random.default_rng(N).standard_normal(levels[0].n)
     a, ref = M(b, degrees=lambda l: 1), O.vcycle(b)
     assert np.linalg.norm(a - ref) <= 1e-12 * np.linalg.norm(ref)
@@ -62,6 +70,72 @@ def test_smoother_is_the_chebyshev_polynomial(ggb, orc, deg):
     # on [1/2, 2] the polynomial stays within 1/T_k(5/3)
     hi = lam >= 0.5
     assert np.max(np.abs(q[hi])) <= 1.0 / t(deg, np.array(5.0 / 3.0)) + 1e-12
+
+
+def _exact_transfer(ggb, orc, N):
+    levels = levels_for(ggb, orc, N)
+    O = orc.OracleFluid("f64", levels)
+    pt, pw = O.transfer(0)
+    pt, pw = exact_weights(np.asarray(pt).reshape(-1, 4), np.asarray(pw).reshape(-1, 4), N)
+    nf, nc = levels[0].n, levels[1].n
+    P = sp.csr_matrix((pw.ravel(), (np.repeat(np.arange(nf), 4), pt.ravel())), shape=(nf, nc))
+    return levels, P
+
+
+@pytest.mark.parametrize("N", [32, 64])
+def test_galerkin_weights(ggb, orc, N):
+    levels, P = _exact_transfer(ggb, orc, N)
+    K = kmat(levels[0])
+    G = galerkin(P.T.tocsr(), K, P)
+    Gx = galerkin(P.T.tocsr(), K, P, half=False)
+    # symmetric, zero row sums, non-positive off-diagonal part up to rounding
+    assert abs(G - G.T).max() == 0.0
+    assert np.abs(np.asarray(G.sum(axis=1))).max() <= 1e-12 * G.diagonal().max()
+    # half-precision storage: each weight within half an ulp of the product's
+    W, Wx = (sp.diags(G.diagonal()) - G).tocoo(), (sp.diags(Gx.diagonal()) - Gx).tocsr()
+    ex = np.asarray(Wx[W.row, W.col]).ravel()
+    assert np.all(np.abs(W.data - ex) <= 2.0 ** -11 * np.abs(ex) + 2.0 ** -25)
+    # a 5x5 stencil on the face interior
+    Nc = N // 2
+    i = np.arange(G.shape[0])
+    f, y, x = i // (Nc * Nc), (i // Nc) % Nc, i % Nc
+    Gc = G.tocoo()
+    inner = (x[Gc.row] >= 2) & (x[Gc.row] <= Nc - 3) & (y[Gc.row] >= 2) & (y[Gc.row] <= Nc - 3)
+    d = Gc.col[inner] - Gc.row[inner]
+    assert np.all(f[Gc.col[inner]] == f[Gc.row[inner]])
+    assert np.all(np.abs(np.round(d / Nc)) <= 2) and np.all(np.abs(d - np.round(d / Nc) * Nc) <= 2)
+    # on smooth fields its energy is the coarse grid's own Voronoi operator's
+    Kc = kmat(levels[1])
+    pos = np.asarray(levels[1].pos, np.float64).reshape(-1, 3)
+    for v in (pos[:, 0], pos[:, 1] * pos[:, 2], pos[:, 2] ** 2):
+        v = v - v.mean()
+        assert abs((v @ (G @ v)) / (v @ (Kc @ v)) - 1.0) < 0.01
+
+
+def test_galerkin_coarse_correction_is_a_projection(ggb, orc):
+    levels, P = _exact_transfer(ggb, orc, 16)
+    K = kmat(levels[0]).toarray()
+    Pd = P.toarray()
+    R = Pd.T
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((K.shape[0], 3))
+
+    def defects(Kc):
+        E = np.eye(K.shape[0]) - Pd @ np.linalg.pinv(Kc) @ R @ K
+        proj = np.linalg.norm(E @ (E @ X) - E @ X) / np.linalg.norm(E @ X)
+        sym = np.linalg.norm(K @ E - (K @ E).T) / np.linalg.norm(K @ E)
+        return proj, sym
+
+    proj, sym = defects(galerkin(sp.csr_matrix(R), sp.csr_matrix(K), P, half=False).toarray())
+    assert proj < 1e-7 and sym < 1e-7
+    # the rediscretised coarse operator, and a product with a planted mistake, are not
+    # (K E stays symmetric for any symmetric coarse operator: only the projection tells)
+    proj, sym = defects(kmat(levels[1]).toarray())
+    assert proj > 1e-3
+    Kbad = R @ K @ Pd
+    Kbad = Kbad + 0.05 * np.diag(np.diag(Kbad))  # a planted mistake in the diagonal
+    proj, sym = defects(Kbad)
+    assert proj > 1e-3
 
 
 @pytest.mark.parametrize("N", [16, 32])
