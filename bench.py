@@ -86,13 +86,14 @@ def kernel_bytes(name, n, P, E, T, Ts, Tm, Tz, reorder_frac=1.0, cheb=False):
     raise KeyError(name)
 
 
-def fluid_step_bytes(N, its, vcycles, precision, cheb):
+def fluid_step_bytes(N, its, vcycles, precision, cheb, galerkin=True):
     """Algorithmic bytes of one fluid step under the per-pass model (each pass reads its
     inputs and writes its outputs once; gathered neighbours, the stencil's class weights
     and the small transfer tables are cache hits): the fixed passes plus, per V-cycle, the
     level-0 down and up passes and every coarser level's, and per CG iteration the Ap pass
     (FP64 mode).  The mixed PCG runs its iterations' updates inside the V-cycles' first
-    passes (one V-cycle per iteration, the last update alone).  Returns (bytes, breakdown)."""
+    passes (one V-cycle per iteration, the last update alone); its coarse levels read their
+    Galerkin operators' 24 half-precision weights per cell.  Returns (bytes, breakdown)."""
     n = 6 * N * N
     Ts = 4 if precision == ggb.PREC_REF32 else 8
     Tm = 8 if precision == ggb.PREC_FP64 else 4
@@ -116,7 +117,11 @@ def fluid_step_bytes(N, its, vcycles, precision, cheb):
     for l, nl in enumerate(levels):
         if l == 0:
             continue
-        if cheb:
+        if cheb and galerkin:
+            # a Galerkin level's down pass (b, 48 B of weights, dinv in; x_pre and the next
+            # level's b out) and up pass (b, x_pre, coarse x, weights, dinv in; x out)
+            per_cycle += nl * 126
+        elif cheb:
             # a coarse level's down pass (b, stencil record, dinv in; x_pre and the next level's
             # b out) and up pass (b, x_pre, coarse x, stencil record, dinv in; x out)
             per_cycle += nl * 46
@@ -537,7 +542,8 @@ def main():
         traffic = tjson[dominant].get("bytes_per_launch")
     cheb = cfg["precision"] == ggb.PREC_MIXED and os.environ.get("GGB_MIXED_CHEB") != "0" and cfg["N"] % 16 == 0
     its = acc["its"] / K
-    fbytes, fparts = fluid_step_bytes(cfg["N"], its, its + 1, cfg["precision"], cheb)
+    galerkin = cheb and os.environ.get("GGB_MIXED_GALERKIN") != "0"
+    fbytes, fparts = fluid_step_bytes(cfg["N"], its, its + 1, cfg["precision"], cheb, galerkin)
     fach = fbytes / (ms_fluid / 1000.0)
     W = 8 if cfg["precision"] != ggb.PREC_REF32 else 4
     survey = (47 * W + 44 + 16 * 6 + (13 * W + 24 + (4.0 / 3.0) * (14.5 * (4 if cfg["precision"] != ggb.PREC_FP64

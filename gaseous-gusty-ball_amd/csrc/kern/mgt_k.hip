@@ -97,10 +97,16 @@ struct ExtCell {
 	int c;          // cell index (-1: none)
 	int slot;       // LDS slot
 	int ring;
-	unsigned short nb[8];
+	unsigned nb[4];  // stencil neighbours' LDS slots, two 16-bit slots per word
 	float w[8];
 	float dinv;
 };
+
+// the LDS slot of stencil neighbour k (two 16-bit slots per word)
+__device__ __forceinline__ int slot_at(const unsigned *p, int k)
+{
+	return (int) ((p[k >> 1] >> (16 * (k & 1))) & 0xffffu);
+}
 
 // Sets up ext cell e of tile `tile`: an interior tile's HS-ring square, or a graph tile's
 // list (rings 0..HG, from ext cell `off`); cells on rings < its tile's ring count get
@@ -129,9 +135,10 @@ __device__ __forceinline__ void ext_cell(const ggbk_grid &g, const ggbk_tiles &t
 			cls = rc.x & 0x7fffffff;
 			unsigned code = (unsigned) rc.y;
 #pragma unroll
-			for (int k = 0; k < 8; k++) {
-				int dy = (int) ((code >> (4 * k + 2)) & 3u) - 1, dx = (int) ((code >> (4 * k)) & 3u) - 1;
-				x.nb[k] = (unsigned short) (x.slot + dy * W + dx);
+			for (int k = 0; k < 8; k += 2) {
+				int dy0 = (int) ((code >> (4 * k + 2)) & 3u) - 1, dx0 = (int) ((code >> (4 * k)) & 3u) - 1;
+				int dy1 = (int) ((code >> (4 * k + 6)) & 3u) - 1, dx1 = (int) ((code >> (4 * k + 4)) & 3u) - 1;
+				x.nb[k >> 1] = (unsigned) (x.slot + dy0 * W + dx0) | ((unsigned) (x.slot + dy1 * W + dx1) << 16);
 			}
 		}
 	} else {
@@ -148,14 +155,10 @@ __device__ __forceinline__ void ext_cell(const ggbk_grid &g, const ggbk_tiles &t
 		if (ring < HG) {
 			const uint4 *q = (const uint4 *) (tl.gnbr + 8 * (size_t) (off + e));
 			uint4 v = *q;
-			x.nb[0] = (unsigned short) (v.x & 0xffff);
-			x.nb[1] = (unsigned short) (v.x >> 16);
-			x.nb[2] = (unsigned short) (v.y & 0xffff);
-			x.nb[3] = (unsigned short) (v.y >> 16);
-			x.nb[4] = (unsigned short) (v.z & 0xffff);
-			x.nb[5] = (unsigned short) (v.z >> 16);
-			x.nb[6] = (unsigned short) (v.w & 0xffff);
-			x.nb[7] = (unsigned short) (v.w >> 16);
+			x.nb[0] = v.x;
+			x.nb[1] = v.y;
+			x.nb[2] = v.z;
+			x.nb[3] = v.w;
 		}
 	}
 	x.c = c;
@@ -181,7 +184,7 @@ __device__ __forceinline__ float kx_lds(const ExtCell &x, const float *xs)
 	float xi = xs[x.slot], s = 0.0f;
 #pragma unroll
 	for (int k = 0; k < 8; k++)
-		s += x.w[k] * (xi - xs[x.nb[k]]);
+		s += x.w[k] * (xi - xs[slot_at(x.nb, k)]);
 	return s;
 }
 
@@ -273,11 +276,11 @@ __device__ __forceinline__ void restrict_tile(const ggbk_grid &g, const ggbk_til
 // residual for the new direction p' = z + beta p, Ap' = Az + beta Ap (first: z, Az); the
 // tile stores r_out = b, p', Ap' and psi += alpha p' (when s->active).  The smoothing's
 // stores are skipped when no V-cycle follows the update (s->vc == 0).
-template <int D, typename TB, bool UPD, int PER>
-__global__ __launch_bounds__(MGT_BS) void k_mgt_down(ggbk_grid g, ggbk_tiles tl, ggbk_xfer xf, const TB *b, float *bc,
+template <int D, typename TB, bool UPD, int PER, int BS = MGT_BS>
+__global__ __launch_bounds__(BS) void k_mgt_down(ggbk_grid g, ggbk_tiles tl, ggbk_xfer xf, const TB *b, float *bc,
 						       float *xpre, ggbk_cg_cheb cg, const ggbk_scalars *s)
 {
-	constexpr int CAP = PER * MGT_BS, HS = D + 1, W = MGT_T + 2 * HS;
+	constexpr int CAP = PER * BS, HS = D + 1, W = MGT_T + 2 * HS;
 	__shared__ float xa[CAP], xb[CAP];
 	int upd = s->active, act = s->vc; // the update; the smoothing after it
 	if (!(UPD && upd) && !act)
@@ -291,7 +294,7 @@ __global__ __launch_bounds__(MGT_BS) void k_mgt_down(ggbk_grid g, ggbk_tiles tl,
 	float bv[PER], dv[PER];
 #pragma unroll
 	for (int m = 0; m < PER; m++) {
-		int e = threadIdx.x + m * MGT_BS;
+		int e = threadIdx.x + m * BS;
 		ext_cell<HS, D>(g, tl, tile, kind, off, e, next, ring_end, xc[m]);
 		bv[m] = 0.0f;
 		dv[m] = 0.0f;
@@ -345,7 +348,7 @@ __global__ __launch_bounds__(MGT_BS) void k_mgt_down(ggbk_grid g, ggbk_tiles tl,
 	for (int m = 0; m < PER; m++) {
 		if (xc[m].c >= 0 && xc[m].ring <= hk - D) {
 			xn[xc[m].slot] = bv[m] - kx_lds(xc[m], xs);
-			bool own = kind < 0 ? xc[m].ring == 0 : threadIdx.x + m * MGT_BS < MGT_T * MGT_T;
+			bool own = kind < 0 ? xc[m].ring == 0 : threadIdx.x + m * BS < MGT_T * MGT_T;
 			if (own && act)
 				xpre[xc[m].c] = xs[xc[m].slot];
 		}
@@ -460,14 +463,14 @@ __global__ __launch_bounds__(MGT_BS) void k_mgt_up(ggbk_grid g, ggbk_tiles tl, g
 // order, as the FP64 operator sums them) and the fold's partials over the tile: r.z,
 // z.Az, z.Ap, r.Az, Az.Az, Az.Ap, r.Ap, Ap.Ap and r.r (slots RZ, ZAZ .. APAP, RR), Ap the
 // current direction's image (zero when NULL), r = b.
-template <int D, int PER>
-__global__ __launch_bounds__(MGT_BS) void k_mgt_up_cg(ggbk_grid g, ggbk_tiles tl, ggbk_xfer xf, const double *b,
+template <int D, int PER, int BS = MGT_BS>
+__global__ __launch_bounds__(BS) void k_mgt_up_cg(ggbk_grid g, ggbk_tiles tl, ggbk_xfer xf, const double *b,
 							const float *xpre, const float *xcoarse, float *zout, double *Az,
 							const double *Ap, const ggbk_scalars *s, ggbk_red red)
 {
-	constexpr int H = D + 1, CAP = PER * MGT_BS;
+	constexpr int H = D + 1, CAP = PER * BS;
 	__shared__ float xa[CAP], xb[CAP];
-	__shared__ double wsum[9][MGT_BS / 64];
+	__shared__ double wsum[9][BS / 64];
 	int act = s->vc;
 	if (!act)
 		return;
@@ -478,7 +481,7 @@ __global__ __launch_bounds__(MGT_BS) void k_mgt_up_cg(ggbk_grid g, ggbk_tiles tl
 	double rd[PER];
 #pragma unroll
 	for (int m = 0; m < PER; m++) {
-		ext_cell<H, H>(g, tl, tile, kind, off, threadIdx.x + m * MGT_BS, next, ring_end, xc[m]);
+		ext_cell<H, H>(g, tl, tile, kind, off, threadIdx.x + m * BS, next, ring_end, xc[m]);
 		bv[m] = 0.0f;
 		dv[m] = 0.0f;
 		rd[m] = 0.0;
@@ -524,7 +527,7 @@ __global__ __launch_bounds__(MGT_BS) void k_mgt_up_cg(ggbk_grid g, ggbk_tiles tl
 				double zi = (double) zf, az = 0.0;
 #pragma unroll
 				for (int k = 0; k < 8; k++)
-					az += (double) xc[m].w[k] * (zi - (double) xs[xc[m].nb[k]]);
+					az += (double) xc[m].w[k] * (zi - (double) xs[slot_at(xc[m].nb, k)]);
 				double ap = Ap ? Ap[c] : 0.0, r = rd[m];
 				if (act)
 					Az[c] = az;
@@ -554,7 +557,7 @@ __global__ __launch_bounds__(MGT_BS) void k_mgt_up_cg(ggbk_grid g, ggbk_tiles tl
 			int q = threadIdx.x;
 			double t = 0.0;
 #pragma unroll
-			for (int w = 0; w < MGT_BS / 64; w++)
+			for (int w = 0; w < BS / 64; w++)
 				t += wsum[q][w];
 			int slot = q == 0 ? GGBK_SL_RZ : q == 8 ? GGBK_SL_RR : GGBK_SL_ZAZ + q - 1;
 			red.part[(size_t) slot * red.stride + tile] = t;
@@ -674,11 +677,11 @@ __device__ __forceinline__ float kxw(const ExtCellW &x, const float *xs, bool sq
 // restriction's support and its restriction into the next level (as k_mgt_down).  x_k on
 // rho <= 1 + 2 (D + 1 - k) of an interior tile's square (HS = 2 D + 1 rings), on BFS
 // rings <= D + 1 - k of a graph tile; the residual on rho <= 1 / ring 0.
-template <int D, int PER>
-__global__ __launch_bounds__(MGT_BS) void k_mgw_down(ggbk_grid g, ggbk_wide wd, ggbk_tiles tl, ggbk_xfer xf,
+template <int D, int PER, int BS = MGT_BS>
+__global__ __launch_bounds__(BS) void k_mgw_down(ggbk_grid g, ggbk_wide wd, ggbk_tiles tl, ggbk_xfer xf,
 						       const float *b, float *bc, float *xpre, const ggbk_scalars *s)
 {
-	constexpr int CAP = PER * MGT_BS, HS = 2 * D + 1, W = MGT_T + 2 * HS;
+	constexpr int CAP = PER * BS, HS = 2 * D + 1, W = MGT_T + 2 * HS;
 	__shared__ float xa[CAP], xb[CAP];
 	int act = s->vc;
 	if (!act)
@@ -691,7 +694,7 @@ __global__ __launch_bounds__(MGT_BS) void k_mgw_down(ggbk_grid g, ggbk_wide wd, 
 	float bv[PER], dv[PER];
 #pragma unroll
 	for (int m = 0; m < PER; m++) {
-		ext_cell_w<HS, D>(g, wd, tl, tile, kind, off, threadIdx.x + m * MGT_BS, next, ring_end, xc[m]);
+		ext_cell_w<HS, D>(g, wd, tl, tile, kind, off, threadIdx.x + m * BS, next, ring_end, xc[m]);
 		bv[m] = 0.0f;
 		dv[m] = 0.0f;
 		if (xc[m].c >= 0) {
@@ -722,7 +725,7 @@ __global__ __launch_bounds__(MGT_BS) void k_mgw_down(ggbk_grid g, ggbk_wide wd, 
 	for (int m = 0; m < PER; m++) {
 		if (xc[m].c >= 0 && xc[m].ring <= base) {
 			xn[xc[m].slot] = bv[m] - kxw<W>(xc[m], xs, sq);
-			bool own = sq ? xc[m].ring == 0 : threadIdx.x + m * MGT_BS < MGT_T * MGT_T;
+			bool own = sq ? xc[m].ring == 0 : threadIdx.x + m * BS < MGT_T * MGT_T;
 			if (own)
 				xpre[xc[m].c] = xs[xc[m].slot];
 		}
@@ -734,12 +737,12 @@ __global__ __launch_bounds__(MGT_BS) void k_mgw_down(ggbk_grid g, ggbk_wide wd, 
 // Up pass of a Galerkin level: x = x_pre + P x_coarse, then D Chebyshev steps on b; x_k on
 // rho <= 2 (D - k) of an interior tile's square (HS = 2 D rings), on BFS rings <= D - k of
 // a graph tile; x_D stored on the tile.
-template <int D, int PER>
-__global__ __launch_bounds__(MGT_BS) void k_mgw_up(ggbk_grid g, ggbk_wide wd, ggbk_tiles tl, ggbk_xfer xf,
+template <int D, int PER, int BS = MGT_BS>
+__global__ __launch_bounds__(BS) void k_mgw_up(ggbk_grid g, ggbk_wide wd, ggbk_tiles tl, ggbk_xfer xf,
 						     const float *b, const float *xpre, const float *xcoarse, float *xout,
 						     const ggbk_scalars *s)
 {
-	constexpr int CAP = PER * MGT_BS, HS = 2 * D, W = MGT_T + 2 * HS;
+	constexpr int CAP = PER * BS, HS = 2 * D, W = MGT_T + 2 * HS;
 	__shared__ float xa[CAP], xb[CAP];
 	int act = s->vc;
 	if (!act)
@@ -752,7 +755,7 @@ __global__ __launch_bounds__(MGT_BS) void k_mgw_up(ggbk_grid g, ggbk_wide wd, gg
 	float bv[PER], dv[PER];
 #pragma unroll
 	for (int m = 0; m < PER; m++) {
-		ext_cell_w<HS, D>(g, wd, tl, tile, kind, off, threadIdx.x + m * MGT_BS, next, ring_end, xc[m]);
+		ext_cell_w<HS, D>(g, wd, tl, tile, kind, off, threadIdx.x + m * BS, next, ring_end, xc[m]);
 		bv[m] = 0.0f;
 		dv[m] = 0.0f;
 		if (xc[m].c >= 0) {
@@ -791,11 +794,16 @@ __global__ __launch_bounds__(MGT_BS) void k_mgw_up(ggbk_grid g, ggbk_wide wd, gg
 // Dense bottom (mg_k.hip), gated by s->vc
 extern "C" void ggbk_mg_dense_vc_f32(const float *M, int n, const float *b, float *x, const ggbk_scalars *s);
 
+// The Galerkin levels' passes: a workgroup of MGW_BS threads holds a tile's ext cells, one
+// per thread (676 in a down pass, 576 in an up pass): measured faster than 256 threads with
+// three cells each (whose 110 registers allow 4 waves per SIMD) or 512 with two.
+#define MGW_BS 768
+
 // down pass of level L into level C's b: Galerkin levels degree 2, Voronoi levels 3
 static void down_level(const ggbk_level &L, const ggbk_level &C, const float *b, ggbk_scalars *s)
 {
 	if (L.wd.w) {
-		GGBK_LAUNCH((k_mgw_down<2, 3>), L.td.ntiles, MGT_BS, 0, L.g, L.wd, L.td, L.xf, b, (float *) C.b,
+		GGBK_LAUNCH((k_mgw_down<2, 1, MGW_BS>), L.td.ntiles, MGW_BS, 0, L.g, L.wd, L.td, L.xf, b, (float *) C.b,
 			    (float *) L.xpre, s);
 		return;
 	}
@@ -808,32 +816,33 @@ static void down_level(const ggbk_level &L, const ggbk_level &C, const float *b,
 static void up_level(const ggbk_level &L, const float *b, const float *xc, float *x, ggbk_scalars *s)
 {
 	if (L.wd.w) {
-		GGBK_LAUNCH((k_mgw_up<2, 3>), L.tl.ntiles, MGT_BS, 0, L.g, L.wd, L.tl, L.xf, b, (const float *) L.xpre,
-			    xc, x, s);
+		GGBK_LAUNCH((k_mgw_up<2, 1, MGW_BS>), L.tl.ntiles, MGW_BS, 0, L.g, L.wd, L.tl, L.xf, b,
+			    (const float *) L.xpre, xc, x, s);
 		return;
 	}
 	GGBK_LAUNCH((k_mgt_up<3>), L.tl.ntiles, MGT_BS, 0, L.g, L.tl, L.xf, b, (const float *) L.xpre, xc, x, s);
 }
 
-// level 0's passes for its smoothing degree D (2 ext cells per thread for D = 2, else 3)
-template <int D, int PER>
+// level 0's passes for its smoothing degree D (2 ext cells per thread for D = 2, else 3;
+// 384 or 512 threads with two cells each measured slower)
+template <int D, int PER, int BS = MGT_BS>
 static void level0_down(const ggbk_level *lv, const ggbk_cg_cheb &cg, ggbk_scalars *s)
 {
 	const ggbk_level &L0 = lv[0];
 	if (cg.update)
-		GGBK_LAUNCH((k_mgt_down<D, double, true, PER>), L0.td.ntiles, MGT_BS, 0, L0.g, L0.td, L0.xf,
+		GGBK_LAUNCH((k_mgt_down<D, double, true, PER, BS>), L0.td.ntiles, BS, 0, L0.g, L0.td, L0.xf,
 			    (const double *) nullptr, (float *) lv[1].b, (float *) L0.xpre, cg, s);
 	else
-		GGBK_LAUNCH((k_mgt_down<D, double, false, PER>), L0.td.ntiles, MGT_BS, 0, L0.g, L0.td, L0.xf, cg.r,
+		GGBK_LAUNCH((k_mgt_down<D, double, false, PER, BS>), L0.td.ntiles, BS, 0, L0.g, L0.td, L0.xf, cg.r,
 			    (float *) lv[1].b, (float *) L0.xpre, cg, s);
 }
 
-template <int D, int PER>
+template <int D, int PER, int BS = MGT_BS>
 static void level0_up(const ggbk_level *lv, const double *b0, void *zout, double *Az, const double *ap,
 		      ggbk_scalars *s, ggbk_red red)
 {
 	const ggbk_level &L0 = lv[0];
-	GGBK_LAUNCH((k_mgt_up_cg<D, PER>), L0.tl.ntiles, MGT_BS, 0, L0.g, L0.tl, L0.xf, b0, (const float *) L0.xpre,
+	GGBK_LAUNCH((k_mgt_up_cg<D, PER, BS>), L0.tl.ntiles, BS, 0, L0.g, L0.tl, L0.xf, b0, (const float *) L0.xpre,
 		    (const float *) lv[1].x, (float *) zout, Az, ap, s, red);
 }
 

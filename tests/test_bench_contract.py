@@ -75,14 +75,20 @@ def test_fluid_step_byte_model(bench):
     b7, parts7 = bench.fluid_step_bytes(512, 7, 8, ggb.PREC_MIXED, True)
     n = 6 * 512 * 512
     # one more iteration of the mixed PCG adds one fused update + down pass, one level-0 up
-    # pass (which also forms A z: no separate Ap pass) and one pass over the coarse levels
-    # and the dense bottom -- about 160 B per cell
+    # pass (which also forms A z: no separate Ap pass) and one pass over the Galerkin coarse
+    # levels (126 B per cell, 48 of them the half-precision weights) and the dense bottom --
+    # about 180 B per fine cell
     extra = (b7 - b6) / n
-    assert 130 < extra < 190, extra
+    assert 150 < extra < 210, extra
     assert "Ap" not in parts6
+    coarse = n * (1 / 4 + 1 / 16 + 1 / 64 + 1 / 256)
     assert extra * n == pytest.approx(bench.kernel_bytes("k_cg_update_down", n, 0, 0, 0, 8, 4, 4, cheb=True)
                                       + bench.kernel_bytes("mg_up(level 0)", n, 0, 0, 0, 8, 4, 4, cheb=True)
-                                      + 46 * n * (1 / 4 + 1 / 16 + 1 / 64 + 1 / 256) + 4 * 1536 ** 2 + 8 * 1536)
+                                      + 126 * coarse + 4 * 1536 ** 2 + 8 * 1536)
+    # the rediscretised coarse levels (no weights per cell)
+    b6v, _ = bench.fluid_step_bytes(512, 6, 7, ggb.PREC_MIXED, True, galerkin=False)
+    b7v, _ = bench.fluid_step_bytes(512, 7, 8, ggb.PREC_MIXED, True, galerkin=False)
+    assert (b7v - b6v) - (b7 - b6) == pytest.approx((46 - 126) * coarse)
     assert sum(parts6.values()) == int(b6) or abs(sum(parts6.values()) - b6) < len(parts6)
     # FP64 storage moves more per V-cycle, and its PCG has the Ap pass
     b6d, parts6d = bench.fluid_step_bytes(512, 6, 7, ggb.PREC_FP64, False)
