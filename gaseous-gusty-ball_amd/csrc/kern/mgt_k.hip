@@ -54,42 +54,18 @@ __device__ __forceinline__ float cheb_cr(int k)
 	return k == 0 ? 0.8f : k == 1 ? 40.0f / 41.0f : k == 2 ? 328.0f / 365.0f : k == 3 ? 2920.0f / 3281.0f : 0.0f;
 }
 
-// Ext cell e of an interior tile with D rings, ring by ring: ring 0 the tile row-major,
-// then each ring's top row, bottom row, left column, right column.  Returns the position
-// (ey, ex) in the (16 + 2D)^2 square and the ring.
+// Ext cell e of an interior tile with D rings: the (16 + 2D)^2 square row-major, so that a
+// wave's lanes read runs of consecutive cells (listing the rings one after the other made
+// the lanes of a ring's columns read cells a grid row apart).  Returns the position
+// (ey, ex) in the square and the ring (the Chebyshev distance to the tile).
 template <int D>
 __device__ __forceinline__ void square_pos(int e, int &ey, int &ex, int &ring)
 {
-	if (e < MGT_T * MGT_T) {
-		ey = D + e / MGT_T;
-		ex = D + e % MGT_T;
-		ring = 0;
-		return;
-	}
-	int m = e - MGT_T * MGT_T;
-#pragma unroll
-	for (int r = 1; r <= D; r++) {
-		int s = MGT_T + 2 * r, cnt = 4 * (s - 1);
-		if (m < cnt || r == D) {
-			int o = D - r; // square coordinate of the ring's first row / column
-			if (m < s) {
-				ey = o;
-				ex = o + m;
-			} else if (m < 2 * s) {
-				ey = o + s - 1;
-				ex = o + m - s;
-			} else if (m < 2 * s + s - 2) {
-				ey = o + 1 + (m - 2 * s);
-				ex = o;
-			} else {
-				ey = o + 1 + (m - 3 * s + 2);
-				ex = o + s - 1;
-			}
-			ring = r;
-			return;
-		}
-		m -= cnt;
-	}
+	constexpr int W = MGT_T + 2 * D;
+	ey = e / W;
+	ex = e - ey * W;
+	int dy = max(D - ey, ey - (D + MGT_T - 1)), dx = max(D - ex, ex - (D + MGT_T - 1));
+	ring = max(max(dy, dx), 0);
 }
 
 // Per-thread view of one ext cell.
