@@ -13,6 +13,8 @@ mixed_precond.py), the checker of the GPU's Chebyshev V-cycle:
     and -- what defines them -- the two-grid coarse correction I - P K_c^+ R K_f they
     give is a K_f-orthogonal projection (it is not with the Voronoi coarse operator, nor
     with a planted mistake in the product's diagonal);
+  * the library's own host construction of those operators (csrc/host/galerkin.c, CPU
+    code called through ctypes) equals them bit for bit, level by level;
   * the V-cycle with the default degrees is symmetric (the same polynomial before and
     after the coarse correction);
   * the exact bilinear face-interior transfer weights of the Chebyshev hierarchy are the
@@ -197,3 +199,67 @@ def test_chebyshev_hierarchy_sizes():
     of at most 1536 cells) a multiple of 16 wide."""
     assert [N for N in (16, 24, 32, 48, 64, 96, 128, 256, 512, 1024) if chebyshev_hierarchy(N)] == \
         [16, 32, 64, 128, 256, 512, 1024]
+
+
+class _GalOp(C.Structure):
+    # the library's host Galerkin operator (csrc/host/ggb_host.h: GalOp)
+    _fields_ = [("n", C.c_int), ("N", C.c_int), ("nbr", C.POINTER(C.c_int)), ("w", C.POINTER(C.c_double)),
+                ("wh", C.POINTER(C.c_ushort)), ("nn", C.POINTER(C.c_ubyte)), ("reg", C.POINTER(C.c_ubyte)),
+                ("dinv", C.POINTER(C.c_float))]
+
+
+@pytest.mark.parametrize("N", [32, 64])
+def test_library_galerkin_operators_match_restatement(ggb, orc, N):
+    """The library's host construction of the mixed hierarchy's coarse operators
+    (csrc/host/galerkin.c, CPU code) gives, level by level, exactly the restatement's
+    half-precision Galerkin products, with a regular cell's slots at the fixed 5x5 offsets
+    and dinv = 1 / sum of its weights."""
+    L = ggb.lib()
+    L.ggb_galerkin.restype = C.c_int
+    L.ggb_galop_free.restype = None
+    levels = levels_for(ggb, orc, N)
+    O = orc.OracleFluid("f64", levels)
+    K = kmat(levels[0])
+    keep, prev = [], None
+    for l in range(len(levels) - 1):
+        pt, pw = O.transfer(l)
+        pt, pw = exact_weights(np.asarray(pt).reshape(-1, 4), np.asarray(pw).reshape(-1, 4), levels[l].N)
+        pt = np.ascontiguousarray(pt, np.int32)
+        pw = np.ascontiguousarray(pw, np.float32)
+        nf, nc = levels[l].n, levels[l + 1].n
+        fine = np.repeat(np.arange(nf), 4)
+        P = sp.csr_matrix((pw.ravel().astype(np.float64), (fine, pt.ravel())), shape=(nf, nc))
+        # R = P^T as rows with fine indices ascending
+        order = np.lexsort((fine, pt.ravel()))
+        ridx = np.ascontiguousarray(fine[order], np.int32)
+        rw = np.ascontiguousarray(pw.ravel()[order], np.float32)
+        rstart = np.zeros(nc + 1, np.int32)
+        np.add.at(rstart, pt.ravel()[order] + 1, 1)
+        rstart = np.cumsum(rstart).astype(np.int32)
+        out = _GalOp()
+        ip, fp = C.POINTER(C.c_int), C.POINTER(C.c_float)
+        rc = L.ggb_galerkin(C.byref(levels[0].c), C.byref(prev) if prev is not None else None,
+                            pt.ctypes.data_as(ip), pw.ctypes.data_as(fp), rstart.ctypes.data_as(ip),
+                            ridx.ctypes.data_as(ip), rw.ctypes.data_as(fp), levels[l + 1].N, C.byref(out))
+        assert rc == 0
+        keep.append(out)
+        n = out.n
+        nbr = np.ctypeslib.as_array(out.nbr, (n * 24,)).reshape(n, 24)
+        w = np.ctypeslib.as_array(out.w, (n * 24,)).reshape(n, 24)
+        reg = np.ctypeslib.as_array(out.reg, (n,)).astype(bool)
+        dinv = np.ctypeslib.as_array(out.dinv, (n,))
+        W = sp.csr_matrix((w.ravel(), (np.repeat(np.arange(n), 24), nbr.ravel())), shape=(n, n))
+        W.setdiag(0)
+        W.eliminate_zeros()
+        Klib = (sp.diags(np.asarray(W.sum(axis=1)).ravel()) - W).tocsr()
+        Kref = galerkin(P.T.tocsr(), K, P)
+        assert abs(Klib - Kref).max() == 0.0
+        assert np.array_equal(dinv, np.float32(1.0 / Kref.diagonal()))
+        Nc = levels[l + 1].N
+        i = np.nonzero(reg)[0]
+        for k in range(24):
+            q = k if k < 12 else k + 1
+            assert np.all(nbr[i, k] == i + (q // 5 - 2) * Nc + (q % 5 - 2))
+        K, prev = Kref, out
+    for g in keep:
+        L.ggb_galop_free(C.byref(g))
