@@ -278,6 +278,14 @@ __global__ __launch_bounds__(BS) void k_mgt_down(ggbk_grid g, ggbk_tiles tl, ggb
 			int c = xc[m].c;
 			bool own = kind < 0 ? xc[m].ring == 0 : e < MGT_T * MGT_T;
 			double bd;
+			if (UPD && !act) {
+				// the solve's last update (no V-cycle follows): psi alone
+				if (own && upd) {
+					double pv = first ? (double) cg.z[c] : (double) cg.z[c] + beta * cg.p_in[c];
+					cg.psi[c] = cg.psi[c] + alpha * pv;
+				}
+				continue;
+			}
 			if (UPD) {
 				// p = z + beta p, Ap = Az + beta Ap (first: p = z, Ap = Az); r -= alpha Ap
 				double apv = first ? cg.Az[c] : cg.Az[c] + beta * cg.Ap_in[c];
@@ -337,20 +345,19 @@ __global__ __launch_bounds__(BS) void k_mgt_down(ggbk_grid g, ggbk_tiles tl, ggb
 // smoothing, for the iteration the host expects to be the last of the solve.
 __global__ __launch_bounds__(MGT_BS) void k_cg_update_cheb(int n, ggbk_cg_cheb cg, const ggbk_scalars *s)
 {
-	int upd = s->active;
+	int upd = s->active, last = s->stop;
 	double alpha = s->alpha, beta = s->beta;
 	int c = xblock() * MGT_BS + threadIdx.x;
 	if (c >= n || !upd)
 		return;
-	double apv = cg.first ? cg.Az[c] : cg.Az[c] + beta * cg.Ap_in[c];
 	double pv = cg.first ? (double) cg.z[c] : (double) cg.z[c] + beta * cg.p_in[c];
-	double rv = cg.r[c] - alpha * apv, ps = cg.psi[c] + alpha * pv;
-	if (upd) {
-		cg.psi[c] = ps;
-		cg.p_out[c] = pv;
-		cg.Ap_out[c] = apv;
-		cg.r_out[c] = rv;
-	}
+	cg.psi[c] = cg.psi[c] + alpha * pv;
+	if (last)
+		return; // the solve ends here: r, p and A p are not read again
+	double apv = cg.first ? cg.Az[c] : cg.Az[c] + beta * cg.Ap_in[c];
+	cg.p_out[c] = pv;
+	cg.Ap_out[c] = apv;
+	cg.r_out[c] = cg.r[c] - alpha * apv;
 }
 
 // Prolongation sum_k pw_k xc[tap_k] at fine cell c (interior fine cells of an analytic
