@@ -319,6 +319,9 @@ struct ggb_fluid_dev {
 	int predict;            /* iterations the previous solve needed */
 	int fix_its;            /* >= 0: the next solves run exactly this many iterations */
 	int vmax_valid;         /* s->vmax2 matches vel */
+	unsigned vmax_seq;      /* max-speed folds enqueued */
+	unsigned vmax_read;     /* vmax_seq at the last read of the scalars (hs->vmax2 is current
+	                         * when they are equal) */
 	/* last-synced copies of the host mirrors (detects caller writes) */
 	float *shadow_omega, *shadow_band, *shadow_psi, *shadow_vel;
 };
@@ -1481,6 +1484,36 @@ static void read_scalars(struct ggb_fluid_dev *D)
 {
 	if (ggbk_d2h(D->hs, D->s, sizeof(ggbk_scalars)))
 		ggb_die("device error: %s", ggbk_last_error());
+	D->vmax_read = D->vmax_seq;
+}
+
+static void vmax_fold(struct ggb_fluid_dev *D)
+{
+	ggbk_fin(GGBK_FIN_VMAX, D->s, D->red, D->nb0, 0, 0, 0);
+	D->vmax_seq++;
+}
+
+/* The passes after a solve (fluid.c:354 then :361-376): psi = 0 when b = 0, then
+ * psi - mean(psi) into psi_alt fused into the velocity pass, and max |u|^2.  They read psi
+ * and change no CG state, so a solve enqueues them before it reads its loop flag: the
+ * read then waits for them too, and when the batch was short they run again after the
+ * next one. */
+static void post_solve_enqueue(struct ggb_fluid_dev *D)
+{
+	ggbk_cg_zero_if_bb0(D->g0, D->psi, D->T, D->s);
+	ggbk_mean_partial(D->g0, D->psi, D->T, D->red);
+	ggbk_fin(mean_op(D), D->s, D->red, D->nb0, 0, 0, D->g0.total_area);
+	ggbk_velocity(D->g0, D->psi, D->psi_alt, 1, D->vel, D->T, D->s, D->red);
+	vmax_fold(D);
+}
+
+/* psi_alt (mean removed) becomes psi */
+static void post_solve_finish(struct ggb_fluid_dev *D)
+{
+	void *t = D->psi;
+	D->psi = D->psi_alt;
+	D->psi_alt = t;
+	D->vmax_valid = 1;
 }
 
 /* The mixed-precision PCG, the same iteration as the reference's (fluid.c:288-358) with the
@@ -1492,7 +1525,7 @@ static void read_scalars(struct ggb_fluid_dev *D)
  * residual before the update is applied, and the V-cycle after the last one is skipped.
  * Batches as in solve_on; a batch's last update is applied alone, and when the loop goes
  * on its residual is then preconditioned without an update. */
-static void solve_cheb(struct ggb_fluid_dev *D, void *psi, int maxit)
+static void solve_cheb(struct ggb_fluid_dev *D, void *psi, int maxit, int post)
 {
 	ggbk_cg_cheb c;
 	memset(&c, 0, sizeof(c));
@@ -1520,6 +1553,8 @@ static void solve_cheb(struct ggb_fluid_dev *D, void *psi, int maxit)
 				ggbk_vcycle_cheb(D->levels, D->nlev, &c, NULL, D->z, D->s, D->red);
 			}
 		}
+		if (post)
+			post_solve_enqueue(D);
 		read_scalars(D);
 		if (!D->hs->active || k >= maxit)
 			break;
@@ -1534,7 +1569,9 @@ static void solve_cheb(struct ggb_fluid_dev *D, void *psi, int maxit)
 /* PCG for K psi = -area * omega on device arrays (reference fluid.c:288-358), psi
  * warm-started from its contents.  With sub_mean the pending mean of omega (in s->mean)
  * is subtracted by the first pass.  psi's mean is NOT removed. */
-static int solve_on(Fluid *F, void *omega, void *psi, int maxit, int sub_mean)
+/* post: also the passes after the solve, enqueued before each read of the loop flag (psi
+ * must be the state's psi; post_solve_finish completes them) */
+static int solve_on(Fluid *F, void *omega, void *psi, int maxit, int sub_mean, int post)
 {
 	struct ggb_fluid_dev *D = F->dev;
 	ggbk_grid g = D->g0;
@@ -1548,7 +1585,7 @@ static int solve_on(Fluid *F, void *omega, void *psi, int maxit, int sub_mean)
 	ggbk_cg_start(g, omega, psi, D->rbuf[0], D->T, sub_mean, D->s, D->red);
 	ggbk_fin(GGBK_FIN_START, D->s, D->red, D->nb0, tol, maxit, 0);
 	if (D->cheb) {
-		solve_cheb(D, psi, maxit);
+		solve_cheb(D, psi, maxit, post);
 		goto done;
 	}
 	precondition(D, D->rbuf[0], 1);
@@ -1560,13 +1597,16 @@ static int solve_on(Fluid *F, void *omega, void *psi, int maxit, int sub_mean)
 	for (;;) {
 		for (int b = 0; b < batch && k < maxit; b++, k++)
 			cg_iteration(D, psi, k);
+		if (post)
+			post_solve_enqueue(D);
 		read_scalars(D);
 		if (!D->hs->active || k >= maxit)
 			break;
 		batch = batch < 64 ? 2 * batch : 64;
 	}
 done:
-	ggbk_cg_zero_if_bb0(g, psi, D->T, D->s);
+	if (!post)
+		ggbk_cg_zero_if_bb0(g, psi, D->T, D->s);
 	F->cg_iters = D->hs->it;
 	F->cg_relres = D->hs->bb == 0.0 ? 0.0 : sqrt(D->hs->rr / D->hs->bb);
 	D->predict = D->hs->it > 0 ? D->hs->it : 1;
@@ -1575,28 +1615,26 @@ done:
 
 static int solve_device(Fluid *F, int maxit, int sub_mean)
 {
-	return solve_on(F, F->dev->omega, F->dev->psi, maxit, sub_mean);
+	return solve_on(F, F->dev->omega, F->dev->psi, maxit, sub_mean, 0);
 }
 
-/* psi -= mean(psi) fused into the velocity pass (fluid.c:354 then :361-376) */
+/* psi -= mean(psi) fused into the velocity pass (fluid.c:354 then :361-376), after a
+ * solve that did not enqueue them itself */
 static void psi_mean_and_velocity(Fluid *F)
 {
 	struct ggb_fluid_dev *D = F->dev;
 	ggbk_mean_partial(D->g0, D->psi, D->T, D->red);
 	ggbk_fin(mean_op(D), D->s, D->red, D->nb0, 0, 0, D->g0.total_area);
 	ggbk_velocity(D->g0, D->psi, D->psi_alt, 1, D->vel, D->T, D->s, D->red);
-	ggbk_fin(GGBK_FIN_VMAX, D->s, D->red, D->nb0, 0, 0, 0);
-	void *t = D->psi;
-	D->psi = D->psi_alt;
-	D->psi_alt = t;
-	D->vmax_valid = 1;
+	vmax_fold(D);
+	post_solve_finish(D);
 }
 
 static void velocity_device(Fluid *F)
 {
 	struct ggb_fluid_dev *D = F->dev;
 	ggbk_velocity(D->g0, D->psi, NULL, 0, D->vel, D->T, D->s, D->red);
-	ggbk_fin(GGBK_FIN_VMAX, D->s, D->red, D->nb0, 0, 0, 0);
+	vmax_fold(D);
 	D->vmax_valid = 1;
 }
 
@@ -1695,8 +1733,8 @@ void fluid_step(Fluid *F, double dt)
 		D->dt_prev[0] = dt;
 		D->prev_valid = D->prev_valid < 2 ? D->prev_valid + 1 : 2;
 	}
-	solve_device(F, F->prm.cg_maxit, 1);
-	psi_mean_and_velocity(F);
+	solve_on(F, D->omega, D->psi, F->prm.cg_maxit, 1, 1);
+	post_solve_finish(D);
 	if (ggbk_check("fluid_step"))
 		ggb_die("fluid_step: %s", ggbk_last_error());
 	if (ggb_sync_mirrors())
@@ -1708,10 +1746,12 @@ double fluid_max_speed(const Fluid *F)
 	struct ggb_fluid_dev *D = F->dev;
 	if (!D->vmax_valid) {
 		ggbk_max_speed2(D->g0, D->vel, D->T, D->red);
-		ggbk_fin(GGBK_FIN_VMAX, D->s, D->red, D->nb0, 0, 0, 0);
+		vmax_fold(D);
 		D->vmax_valid = 1;
 	}
-	read_scalars(D);
+	/* the last fluid_step's read of the loop flag came after its max-speed fold */
+	if (D->vmax_read != D->vmax_seq)
+		read_scalars(D);
 	return sqrt(D->hs->vmax2);
 }
 
@@ -2086,7 +2126,7 @@ static void adj_step(FluidAdjoint *A, double dt, const double *om_s, const doubl
 	ggbk_adj_velocity(g, A->lvel, A->rslot, A->lpsi, D->red);
 	ggbk_fin(GGBK_FIN_MEAN_F64, D->s, D->red, D->nb0, 0, 0, g.total_area);
 	ggbk_adj_rhs(g, A->lpsi, D->s, A->sum_area, A->om);
-	solve_on(F, A->om, A->z, F->prm.cg_maxit, 0);
+	solve_on(F, A->om, A->z, F->prm.cg_maxit, 0, 0);
 	ggbk_adj_combine(g, A->lq, A->z, A->lam);
 }
 
