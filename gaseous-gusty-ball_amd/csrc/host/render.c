@@ -94,6 +94,11 @@ int render_init(Render *r, int R, float radius, float alpha)
 		fprintf(stderr, "render: out of device memory\n");
 		return -1;
 	}
+	/* the draw leaves the tile counts at zero for the next deposit's fill: start there */
+	if (ggbk_zero(k->tile_cursor, sizeof(int) * ((size_t) k->ntiles + 1))) {
+		fprintf(stderr, "render: %s\n", ggbk_last_error());
+		return -1;
+	}
 	if (ggbk_render_host_init(k)) {
 		fprintf(stderr, "render: out of pinned host memory\n");
 		return -1;

@@ -304,11 +304,15 @@ __global__ __launch_bounds__(PBS) void k_render_count(ggbk_parts ps, SplatGeo sg
 // Writes each particle's (particle, tile) entries, and its copy in the next storage order.
 // With bucket != NULL the entries go to fixed-size tile buckets and cursor ends as the
 // exact count of every tile (entries past a full bucket are counted, not stored);
-// otherwise to the list at the scanned starts.
+// otherwise to the list at the scanned starts.  cursor starts at zero (the previous
+// deposit's draw cleared it, or the scan did).  The bucket fill also clears the deposit's
+// statistics, which the bucket statistics after it accumulate into.
 __global__ __launch_bounds__(PBS) void k_render_fill(ggbk_parts ps, ggbk_parts out, SplatGeo sg, const int *starts,
 						     int *cursor, const int *hstarts, int *hcursor, int *list,
-						     long long cap, int *bucket, unsigned short *nsamp)
+						     long long cap, int *bucket, unsigned short *nsamp, long long *stats)
 {
+	if (bucket && blockIdx.x == 0 && threadIdx.x < 16)
+		stats[threadIdx.x] = 0;
 	int i = xblock() * PBS + threadIdx.x;
 	bool on = i < ps.n;
 	dv3 a = d3(0, 0, 1), b = d3(0, 0, 1);
@@ -933,6 +937,10 @@ __global__ __launch_bounds__(DBS, 5) void k_render_draw(ggbk_render r, ggbk_part
 	}
 	if (__lane_id() == 0)
 		atomicAdd(&ntouched, cnt);
+	// the tile's count has been read: zero for the next deposit's fill (tiles with no
+	// entries return above with it zero already)
+	if (lid == 0)
+		r.tile_cursor[t] = 0;
 	__syncthreads();
 	// this tile's diagnostics (summed over tiles on request): touched texels, ranges,
 	// serial ranges, sorted, sample / blend / total / window set-up cycles in units of 1024
@@ -1251,13 +1259,11 @@ extern "C" long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_par
 		GGBK_LAUNCH(k_render_scan_fix, sb, SCAN_BS, 0, r->ntiles, r->tile_start, r->home_start,
 			    (const long long *) r->scan_part, r->total);
 		ggbk_probe_end(GGBK_PROBE_RENDER_COUNT);
-	} else if (ggbk_zero(r->tile_cursor, sizeof(int) * (size_t) r->ntiles) ||
-		   ggbk_zero(r->total, sizeof(long long) * 16)) {
-		return -1;
 	}
+	// otherwise the tile counts are zero from the last draw, and the fill clears the stats
 	ggbk_probe_begin(GGBK_PROBE_RENDER_FILL);
 	GGBK_LAUNCH(k_render_fill, pb, PBS, 0, in, out, sg, (const int *) r->tile_start, r->tile_cursor,
-		    (const int *) r->home_start, hcursor, r->list, r->cap, r->bucket, r->nsamp);
+		    (const int *) r->home_start, hcursor, r->list, r->cap, r->bucket, r->nsamp, r->total);
 	GGBK_LAUNCH(k_render_bucket_stats, sb, SCAN_BS, 0, r->ntiles, (const int *) r->tile_cursor, r->total);
 	ggbk_probe_end(GGBK_PROBE_RENDER_FILL);
 	// the copy runs on its own stream, after the binning, beside the draw
@@ -1299,7 +1305,8 @@ extern "C" long long ggbk_render_deposit(ggbk_render *r, ggbk_parts in, ggbk_par
 		GGBK_LAUNCH(k_render_scan_fix, sb, SCAN_BS, 0, r->ntiles, r->tile_start, r->home_start,
 			    (const long long *) r->scan_part, r->total);
 		GGBK_LAUNCH(k_render_fill, pb, PBS, 0, in, out, sg, (const int *) r->tile_start, r->tile_cursor,
-			    (const int *) r->home_start, hcursor, r->list, r->cap, (int *) nullptr, r->nsamp);
+			    (const int *) r->home_start, hcursor, r->list, r->cap, (int *) nullptr, r->nsamp,
+			    r->total);
 		ggbk_probe_begin(GGBK_PROBE_RENDER_DRAW);
 		GGBK_LAUNCH(k_render_draw, (unsigned) r->ntiles, DBS, 0, *r, in, sg, invg);
 		ggbk_probe_end(GGBK_PROBE_RENDER_DRAW);
