@@ -2,7 +2,9 @@
 // Chebyshev polynomials in D^-1 K, each pass fused into one launch on 16x16 face tiles.
 //
 // Smoothing.  D^-1 K (K the Voronoi Laplacian, D its diagonal) has its spectrum in
-// [0, 2] (Gershgorin: each row of D^-1 W sums to one).  The smoother is the Chebyshev
+// [0, 2] (Gershgorin: the weights are non-negative and each row of D^-1 W sums to one).
+// The Galerkin levels below have some negative weights; their spectra, measured, stay
+// below 1.75 (tests/test_mixed_precond.py checks below 2).  The smoother is the Chebyshev
 // iteration for the interval [1/2, 2], which damps the upper three quarters of the spectrum
 // (the error the coarse grid cannot represent): with sigma = 5/3, delta = 3/4,
 //   d_0 = (4/5) D^-1 r_0,   d_k = rho_k rho_{k-1} d_{k-1} + (2 rho_k / delta) D^-1 r_k,

@@ -90,7 +90,7 @@ def test_galerkin_weights(ggb, orc, N):
     K = kmat(levels[0])
     G = galerkin(P.T.tocsr(), K, P)
     Gx = galerkin(P.T.tocsr(), K, P, half=False)
-    # symmetric, zero row sums, non-positive off-diagonal part up to rounding
+    # symmetric with zero row sums (some couplings are positive: see the spectrum test)
     assert abs(G - G.T).max() == 0.0
     assert np.abs(np.asarray(G.sum(axis=1))).max() <= 1e-12 * G.diagonal().max()
     # half-precision storage: each weight within half an ulp of the product's
@@ -112,6 +112,38 @@ def test_galerkin_weights(ggb, orc, N):
     for v in (pos[:, 0], pos[:, 1] * pos[:, 2], pos[:, 2] ** 2):
         v = v - v.mean()
         assert abs((v @ (G @ v)) / (v @ (Kc @ v)) - 1.0) < 0.01
+
+
+@pytest.mark.parametrize("N", [32, 64])
+def test_galerkin_levels_spectrum_inside_smoothing_interval(ggb, orc, N):
+    """The Chebyshev smoother damps [1/2, 2] of the spectrum of D^-1 K.  On level 0 the
+    weights are non-negative and each row of D^-1 W sums to one, so Gershgorin places the
+    spectrum in [0, 2].  The Galerkin levels have negative weights (positive couplings,
+    up to about a quarter of the diagonal summed over a row), where Gershgorin allows up
+    to ~2.5: the bound is checked on their spectra instead (largest eigenvalue 1.55-1.72
+    on hierarchies from N = 64 and 256)."""
+    levels = levels_for(ggb, orc, N)
+    O = orc.OracleFluid("f64", levels)
+    tr = []
+    for l in range(len(levels) - 1):
+        pt, pw = O.transfer(l)
+        pt, pw = np.asarray(pt).reshape(-1, 4), np.asarray(pw).reshape(-1, 4)
+        tr.append(exact_weights(pt, pw, levels[l].N) if chebyshev_hierarchy(N) else (pt, pw))
+    M = MixedVcycle(levels, tr, galerkin_levels=True)
+    W0 = (sp.diags(M.K[0].diagonal()) - M.K[0]).tocoo()
+    assert W0.data.min() >= 0.0
+    negative = 0
+    for l in range(1, len(levels)):
+        K = M.K[l]
+        d = K.diagonal()
+        W = (sp.diags(d) - K).tocoo()
+        negative += int((W.data < 0).sum())
+        s = 1.0 / np.sqrt(d)
+        S = (sp.diags(s) @ K @ sp.diags(s)).toarray()
+        lam = np.linalg.eigvalsh(S)
+        print("level %d (%d cells): lambda(D^-1 K) in [%.2e, %.4f]" % (l, K.shape[0], lam[0], lam[-1]))
+        assert lam[0] > -1e-10 and lam[-1] < 2.0
+    assert negative > 0
 
 
 def test_galerkin_coarse_correction_is_a_projection(ggb, orc):
