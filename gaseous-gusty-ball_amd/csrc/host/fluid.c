@@ -1710,12 +1710,10 @@ void fluid_step(Fluid *F, double dt)
 	ggbk_advect(g, D->q0, D->dep_b, D->q1, T);
 	if (bfecc)
 		ggbk_advect_bfecc(g, D->q1, D->q0, D->dep_f, D->q2, T);
-	ggbk_limited_relax(g, D->q2, D->q0, D->dep_b, D->q1, bfecc, D->omega_band, D->omega, T, twoOmega, relax,
-			   D->red);
-	ggbk_fin(mean_op(D), D->s, D->red, D->nb0, 0, 0, g.total_area);
 	if (D->psi_prev[0] && !D->no_extrap) {
 		/* psi(t + dt) by the Lagrange polynomial through the last three solutions, at times
-		 * t, t - h1, t - h1 - h2 (fewer while the history fills up) */
+		 * t, t - h1, t - h1 - h2 (fewer while the history fills up), formed in the
+		 * limiter's pass (it touches none of psi's arrays) */
 		double x = dt, h1 = D->dt_prev[0], h2 = D->dt_prev[1], c0 = 1.0, c1 = 0.0, c2 = 0.0;
 		if (D->prev_valid >= 2) {
 			c0 = (x + h1) * (x + h1 + h2) / (h1 * (h1 + h2));
@@ -1725,13 +1723,19 @@ void fluid_step(Fluid *F, double dt)
 			c0 = 1.0 + x / h1;
 			c1 = -x / h1;
 		}
-		ggbk_extrapolate(D->n, (double *) D->psi, D->psi_prev[0], D->psi_prev[1], c0, c1, c2);
+		ggbk_limited_relax_extrap(g, D->q2, D->q0, D->dep_b, D->q1, bfecc, D->omega_band, D->omega, twoOmega,
+					  relax, D->red, (double *) D->psi, D->psi_prev[0], D->psi_prev[1], c0, c1, c2);
+		ggbk_fin(mean_op(D), D->s, D->red, D->nb0, 0, 0, g.total_area);
 		double *t = D->psi_prev[0];
 		D->psi_prev[0] = D->psi_prev[1];
 		D->psi_prev[1] = t;
 		D->dt_prev[1] = D->dt_prev[0];
 		D->dt_prev[0] = dt;
 		D->prev_valid = D->prev_valid < 2 ? D->prev_valid + 1 : 2;
+	} else {
+		ggbk_limited_relax(g, D->q2, D->q0, D->dep_b, D->q1, bfecc, D->omega_band, D->omega, T, twoOmega, relax,
+				   D->red);
+		ggbk_fin(mean_op(D), D->s, D->red, D->nb0, 0, 0, g.total_area);
 	}
 	solve_on(F, D->omega, D->psi, F->prm.cg_maxit, 1, 1);
 	post_solve_finish(D);

@@ -520,11 +520,19 @@ __global__ __launch_bounds__(BS) void k_advect_bfecc(ggbk_grid g, const T *q1, c
 template <typename T>
 __global__ __launch_bounds__(BS) void k_limited_relax(ggbk_grid g, const T *q2, const T *q, const T *dep_b, const T *q1,
 						       int bfecc, const T *band, T *omega, double twoOmega, double relax,
-						       ggbk_red red)
+						       ggbk_red red, double *psi, const double *h1, double *h2, double c0,
+						       double c1, double c2)
 {
 	int i = xblock() * BS + threadIdx.x;
 	double part = 0;
 	if (i < g.n) {
+		if (psi) {
+			// the solve's warm start, psi <- c0 psi + c1 h1 + c2 h2 and h2 <- the old psi (this
+			// pass writes nothing it reads)
+			double a = psi[i], b = h1[i], c = h2[i];
+			h2[i] = a;
+			psi[i] = c0 * a + c1 * b + c2 * c;
+		}
 		T qn;
 		if (bfecc) {
 			Taps t = taps_at(g, ld3(dep_b + 3 * (size_t) i));
@@ -736,22 +744,6 @@ extern "C" void ggbk_jacobi_precond(ggbk_grid g, const void *r, void *z, int T, 
 // Extrapolation in time of the stream function, the warm start of the next solve:
 // psi <- c0 psi + c1 h1 + c2 h2 (h1, h2 the two previous steps' psi), and h2 <- the old psi
 // (the host then swaps h1 and h2, so h1 is always the newest)
-__global__ __launch_bounds__(BS) void k_extrapolate(int n, double *psi, const double *h1, double *h2, double c0,
-						    double c1, double c2)
-{
-	int i = xblock() * BS + threadIdx.x;
-	if (i >= n)
-		return;
-	double a = psi[i], b = h1[i], c = h2[i];
-	h2[i] = a;
-	psi[i] = c0 * a + c1 * b + c2 * c;
-}
-
-extern "C" void ggbk_extrapolate(int n, double *psi, const double *h1, double *h2, double c0, double c1, double c2)
-{
-	GGBK_LAUNCH(k_extrapolate, NB(n), BS, 0, n, psi, h1, h2, c0, c1, c2);
-}
-
 extern "C" void ggbk_cg_zero_if_bb0(ggbk_grid g, void *psi, int T, const ggbk_scalars *s)
 {
 	if (T == GGBK_F64)
@@ -819,9 +811,20 @@ extern "C" void ggbk_limited_relax(ggbk_grid g, const void *q2, const void *q, c
 	if (T == GGBK_F64)
 		GGBK_LAUNCH(k_limited_relax<double>, NB(g.n), BS, 0, g, (const double *) q2, (const double *) q,
 			    (const double *) dep_b, (const double *) q1, bfecc, (const double *) band, (double *) omega,
-			    twoOmega, relax, red);
+			    twoOmega, relax, red, (double *) nullptr, (const double *) nullptr, (double *) nullptr, 0.0, 0.0,
+			    0.0);
 	else
 		GGBK_LAUNCH(k_limited_relax<float>, NB(g.n), BS, 0, g, (const float *) q2, (const float *) q,
 			    (const float *) dep_b, (const float *) q1, bfecc, (const float *) band, (float *) omega,
-			    twoOmega, relax, red);
+			    twoOmega, relax, red, (double *) nullptr, (const double *) nullptr, (double *) nullptr, 0.0, 0.0,
+			    0.0);
+}
+
+extern "C" void ggbk_limited_relax_extrap(ggbk_grid g, const double *q2, const double *q, const double *dep_b,
+					  const double *q1, int bfecc, const double *band, double *omega, double twoOmega,
+					  double relax, ggbk_red red, double *psi, const double *h1, double *h2, double c0,
+					  double c1, double c2)
+{
+	GGBK_LAUNCH(k_limited_relax<double>, NB(g.n), BS, 0, g, q2, q, dep_b, q1, bfecc, band, omega, twoOmega, relax,
+		    red, psi, h1, h2, c0, c1, c2);
 }

@@ -187,8 +187,6 @@ void ggbk_cg_update(ggbk_grid g, void *psi, void *r, const void *p, const void *
 		    const ggbk_scalars *s, ggbk_red red);
 /* Jacobi preconditioner z = dinv r and the r.z partial (no multigrid) */
 void ggbk_jacobi_precond(ggbk_grid g, const void *r, void *z, int T, const ggbk_scalars *s, ggbk_red red);
-/* warm start by extrapolation in time: psi <- c0 psi + c1 h1 + c2 h2, h2 <- the old psi */
-void ggbk_extrapolate(int n, double *psi, const double *h1, double *h2, double c0, double c1, double c2);
 /* bb == 0 exit: psi = 0 when status says so */
 void ggbk_cg_zero_if_bb0(ggbk_grid g, void *psi, int T, const ggbk_scalars *s);
 
@@ -315,6 +313,11 @@ void ggbk_advect_bfecc(ggbk_grid g, const void *q1, const void *q, const void *d
  * area-weighted omega partial (slot MEAN) */
 void ggbk_limited_relax(ggbk_grid g, const void *q2, const void *q, const void *dep_b, const void *q1, int bfecc,
 			const void *band, void *omega, int T, double twoOmega, double relax, ggbk_red red);
+/* the same with the solve's warm start by extrapolation in time in the same pass (FP64
+ * state): psi <- c0 psi + c1 h1 + c2 h2, h2 <- the old psi */
+void ggbk_limited_relax_extrap(ggbk_grid g, const double *q2, const double *q, const double *dep_b, const double *q1,
+			       int bfecc, const double *band, double *omega, double twoOmega, double relax, ggbk_red red,
+			       double *psi, const double *h1, double *h2, double c0, double c1, double c2);
 
 /* ---- discrete adjoint of the fluid step (FP64 state; adj_k.hip) ---- */
 void ggbk_adj_sum(ggbk_grid g, const double *x, ggbk_red red);     /* slot MEAN partials of sum x */
