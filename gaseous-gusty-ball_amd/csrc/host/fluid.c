@@ -1500,8 +1500,7 @@ static void vmax_fold(struct ggb_fluid_dev *D)
  * next one. */
 static void post_solve_enqueue(struct ggb_fluid_dev *D)
 {
-	ggbk_cg_zero_if_bb0(D->g0, D->psi, D->T, D->s);
-	ggbk_mean_partial(D->g0, D->psi, D->T, D->red);
+	ggbk_mean_partial_bb0(D->g0, D->psi, D->T, D->s, D->red);
 	ggbk_fin(mean_op(D), D->s, D->red, D->nb0, 0, 0, D->g0.total_area);
 	ggbk_velocity(D->g0, D->psi, D->psi_alt, 1, D->vel, D->T, D->s, D->red);
 	vmax_fold(D);

@@ -288,6 +288,23 @@ __global__ __launch_bounds__(BS) void k_mean_partial(ggbk_grid g, const T *f, gg
 	store_partial(red, GGBK_SL_MEAN, v);
 }
 
+// After a solve: psi = 0 when b was zero (status 2, as k_zero_if_bb0), then psi's mean
+// partial as k_mean_partial, in one pass
+template <typename T>
+__global__ __launch_bounds__(BS) void k_mean_partial_bb0(ggbk_grid g, T *psi, const ggbk_scalars *s, ggbk_red red)
+{
+	int i = xblock() * BS + threadIdx.x;
+	bool zero = s->status == 2;
+	double v = 0.0;
+	if (i < g.n) {
+		if (zero)
+			psi[i] = (T) 0;
+		else
+			v = (double) g.area[i] * (double) psi[i];
+	}
+	store_partial(red, GGBK_SL_MEAN, v);
+}
+
 template <typename T>
 __global__ __launch_bounds__(BS) void k_sub_mean(int n, T *f, const ggbk_scalars *s)
 {
@@ -612,6 +629,14 @@ extern "C" void ggbk_applyK(ggbk_grid g, const void *x, double *y, int T)
 		GGBK_LAUNCH(k_applyK<double>, NB(g.n), BS, 0, g, (const double *) x, y);
 	else
 		GGBK_LAUNCH(k_applyK<float>, NB(g.n), BS, 0, g, (const float *) x, y);
+}
+
+extern "C" void ggbk_mean_partial_bb0(ggbk_grid g, void *psi, int T, const ggbk_scalars *s, ggbk_red red)
+{
+	if (T == GGBK_F64)
+		GGBK_LAUNCH(k_mean_partial_bb0<double>, NB(g.n), BS, 0, g, (double *) psi, s, red);
+	else
+		GGBK_LAUNCH(k_mean_partial_bb0<float>, NB(g.n), BS, 0, g, (float *) psi, s, red);
 }
 
 extern "C" void ggbk_mean_partial(ggbk_grid g, const void *f, int T, ggbk_red red)

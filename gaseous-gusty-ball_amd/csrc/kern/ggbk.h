@@ -165,6 +165,8 @@ int ggbk_nblocks(int n);
 
 /* f's area-weighted sum partials (slot MEAN); fold with FIN_MEAN_* */
 void ggbk_mean_partial(ggbk_grid g, const void *f, int T, ggbk_red red);
+/* after a solve: psi = 0 when s->status says b was zero, then psi's mean partial */
+void ggbk_mean_partial_bb0(ggbk_grid g, void *psi, int T, const ggbk_scalars *s, ggbk_red red);
 void ggbk_sub_mean(int n, void *f, int T, const ggbk_scalars *s);
 /* f -= area-weighted mean, rounded to T as the reference rounds it (fluid.c:264-275) */
 void ggbk_remove_mean(ggbk_grid g, void *f, int T, ggbk_scalars *s, ggbk_red red);
