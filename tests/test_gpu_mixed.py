@@ -138,3 +138,26 @@ def test_post_solve_passes_follow_a_short_batch(ggb, gpu):
     assert np.linalg.norm(vel - u) <= 1e-10 * np.linalg.norm(u)
     vmax = np.sqrt(np.max(np.sum(vel ** 2, axis=1)))
     assert abs(F.max_speed() - vmax) <= 1e-12 * vmax
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_zero_vorticity_gives_zero_streamfunction(ggb, gpu, prec):
+    """A zero right-hand side (b = -A omega = 0: omega, its band, rotation and forcing all
+    zero) stops the solve before its first iteration and sets psi to zero, as the
+    reference's PCG does for bb == 0 (fluid.c:306-311); the zeroing runs in the pass
+    after the solve (with the psi mean), and the extrapolated warm start of the mixed
+    path, formed from a nonzero psi, must not survive it."""
+    g = ggb.Grid(64)
+    P = ggb.PREC_FP64 if prec == "fp64" else ggb.PREC_MIXED
+    F = ggb.Fluid(g, forcing=0.0, rotation=0.0, cg_tol=1e-8, precision=P)
+    z = np.zeros(g.n)
+    F.set(ggb.FIELD_OMEGA, z)
+    F.set(ggb.FIELD_OMEGA_BAND, z)
+    F.set(ggb.FIELD_VEL, np.zeros(3 * g.n))
+    F.set(ggb.FIELD_PSI, np.random.default_rng(4).standard_normal(g.n))
+    for _ in range(3):
+        F.step(0.5 * g.h)
+        assert F.cg_iters == 0
+        assert np.all(F.get(ggb.FIELD_PSI) == 0.0)
+        assert np.all(F.get(ggb.FIELD_VEL) == 0.0)
+        assert F.max_speed() == 0.0
