@@ -161,3 +161,33 @@ def test_zero_vorticity_gives_zero_streamfunction(ggb, gpu, prec):
         assert np.all(F.get(ggb.FIELD_PSI) == 0.0)
         assert np.all(F.get(ggb.FIELD_VEL) == 0.0)
         assert F.max_speed() == 0.0
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_converged_start_takes_no_iteration(ggb, gpu, prec):
+    """A step so short that the warm start already meets the tolerance: the solve stops
+    before its first iteration (the reference tests the residual before its first iteration, fluid.c:311-321),
+    psi stays the warm start, the true relative residual is within the tolerance, and the
+    velocity and max speed are those of that psi."""
+    g = ggb.Grid(64)
+    vel0, vband = g.initial_velocity(seed=5, octaves=4)
+    P = ggb.PREC_FP64 if prec == "fp64" else ggb.PREC_MIXED
+    F = ggb.Fluid(g, forcing=0.5, rotation=10.0, cg_tol=1e-8, precision=P)
+    F.set_state(vel0, vband)
+    for _ in range(3):
+        F.step(0.5 * g.h)
+    F.step(1e-9 * g.h)
+    assert F.cg_iters == 0
+    assert F.true_relres() <= 1e-8
+    psi = F.get(ggb.FIELD_PSI)
+    vel = F.get(ggb.FIELD_VEL).reshape(-1, 3)
+    p = g.pos.astype(np.float64)
+    nnb = g.nnb.astype(int)
+    d = np.zeros((g.n, 3))
+    for k in range(8):
+        m = k < nnb
+        d[m] += g.gc[m, k].astype(np.float64) * (psi[g.nbr[m, k]] - psi[m])[:, None]
+    u = np.cross(p, d)
+    assert np.linalg.norm(vel - u) <= 1e-10 * np.linalg.norm(u)
+    vmax = np.sqrt(np.max(np.sum(vel ** 2, axis=1)))
+    assert abs(F.max_speed() - vmax) <= 1e-12 * vmax
