@@ -166,7 +166,7 @@ def test_zero_vorticity_gives_zero_streamfunction(ggb, gpu, prec):
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_converged_start_takes_no_iteration(ggb, gpu, prec):
     """A step so short that the warm start already meets the tolerance: the solve stops
-    before its first iteration (the reference tests the residual before its first iteration, fluid.c:311-321),
+    before its first iteration (the reference tests the residual first, fluid.c:311-321),
     psi stays the warm start, the true relative residual is within the tolerance, and the
     velocity and max speed are those of that psi."""
     g = ggb.Grid(64)
