@@ -1524,7 +1524,7 @@ static void post_solve_finish(struct ggb_fluid_dev *D)
  * residual before the update is applied, and the V-cycle after the last one is skipped.
  * Batches as in solve_on; a batch's last update is applied alone, and when the loop goes
  * on its residual is then preconditioned without an update. */
-static void solve_cheb(struct ggb_fluid_dev *D, void *psi, int maxit, int post)
+static void solve_cheb(struct ggb_fluid_dev *D, void *psi, int maxit, int post, double tol)
 {
 	ggbk_cg_cheb c;
 	memset(&c, 0, sizeof(c));
@@ -1534,7 +1534,14 @@ static void solve_cheb(struct ggb_fluid_dev *D, void *psi, int maxit, int post)
 	c.Az = D->Az;
 	c.z = D->z;
 	c.r = D->rbuf[0];
+	/* the first V-cycle runs on cg_start's residual before anything is folded (cg_start
+	 * opened it); its fold then starts the solve (bb, rr, the tolerance) and folds it */
+	c.start = 1;
+	c.nb0 = D->nb0;
+	c.tol = tol;
+	c.maxit = maxit;
 	ggbk_vcycle_cheb(D->levels, D->nlev, &c, NULL, D->z, D->s, D->red);
+	c.start = 0;
 	int batch = D->predict > 1 ? D->predict : 1;
 	int k = 0;
 	for (;;) {
@@ -1581,12 +1588,12 @@ static int solve_on(Fluid *F, void *omega, void *psi, int maxit, int sub_mean, i
 		maxit = D->fix_its;
 		D->predict = maxit > 0 ? maxit : 1;
 	}
-	ggbk_cg_start(g, omega, psi, D->rbuf[0], D->T, sub_mean, D->s, D->red);
-	ggbk_fin(GGBK_FIN_START, D->s, D->red, D->nb0, tol, maxit, 0);
+	ggbk_cg_start(g, omega, psi, D->rbuf[0], D->T, sub_mean, D->s, D->red, D->cheb);
 	if (D->cheb) {
-		solve_cheb(D, psi, maxit, post);
+		solve_cheb(D, psi, maxit, post, tol);
 		goto done;
 	}
+	ggbk_fin(GGBK_FIN_START, D->s, D->red, D->nb0, tol, maxit, 0);
 	precondition(D, D->rbuf[0], 1);
 	/* Iterations are enqueued in batches and the host reads the loop flag after each; the
 	 * first batch is the previous solve's count, later batches grow geometrically.  Passes

@@ -73,23 +73,48 @@ __device__ __forceinline__ double fold(const double *p, int nb)
 __device__ __forceinline__ double fold_sum(const double *p, int nb) { return fold<false>(p, nb); }
 __device__ __forceinline__ double fold_max(const double *p, int nb) { return fold<true>(p, nb); }
 
+// A solve's start from its bb and rr (reference fluid.c:306-314): the tolerance, and
+// whether there is anything to iterate (bb == 0 is status 2: psi is zeroed after the solve)
+__device__ __forceinline__ void start_fold(ggbk_scalars *s, double bb, double rr, double tol, int maxit)
+{
+	s->bb = bb;
+	s->rr = rr;
+	s->it = 0;
+	s->maxit = maxit;
+	s->rz = 0;
+	s->stop = 0;
+	if (bb == 0.0) {
+		s->status = 2;
+		s->tol2 = 0;
+		s->active = 0;
+	} else {
+		s->status = 0;
+		s->tol2 = tol * tol * bb;
+		s->active = (rr > s->tol2 && maxit > 0) ? 1 : 0;
+	}
+	s->vc = s->active;
+}
+
 // The mixed-precision PCG's fold (ggbk_fold_cheb): workgroup q folds the partials of sum q
 // (fold_sum's fixed order) into slot G0 + q; the last workgroup to finish, told by a ticket
 // taken after a device-scope fence, reads the nine sums and updates the scalars.  The ticket
-// counter is reset by that workgroup; nobody waits.
-__global__ __launch_bounds__(FIN_BS) void k_fold_cheb(ggbk_scalars *s, ggbk_red red, int nb, int first)
+// counter is reset by that workgroup; nobody waits.  With nb0 > 0 workgroups 9 and 10 fold
+// the start's bb and rr partials, and the last workgroup starts the solve first.
+__global__ __launch_bounds__(FIN_BS) void k_fold_cheb(ggbk_scalars *s, ggbk_red red, int nb, int first, int nb0,
+						      double tol, int maxit)
 {
 	__shared__ int last;
-	const int q = blockIdx.x;
-	const int slot = q == 0 ? GGBK_SL_RZ : q == 8 ? GGBK_SL_RR : GGBK_SL_ZAZ + q - 1;
+	const int q = blockIdx.x, nwg = gridDim.x;
+	const int slot = q == 0 ? GGBK_SL_RZ : q == 8 ? GGBK_SL_RR : q == 9 ? GGBK_SL_BB : q == 10 ? GGBK_SL_RR0
+											: GGBK_SL_ZAZ + q - 1;
 	const size_t st = (size_t) red.stride;
-	double v = fold_sum(red.part + slot * st, nb);
+	double v = fold_sum(red.part + slot * st, q >= 9 ? nb0 : nb);
 	if (threadIdx.x == 0)
 		red.part[(GGBK_SL_G0 + q) * st] = v;
 	__threadfence();
 	__syncthreads();
 	if (threadIdx.x == 0)
-		last = atomicAdd(&red.cnt[0], 1u) == 8u;
+		last = atomicAdd(&red.cnt[0], 1u) == (unsigned) (nwg - 1);
 	__syncthreads();
 	if (!last || threadIdx.x != 0)
 		return;
@@ -99,12 +124,14 @@ __global__ __launch_bounds__(FIN_BS) void k_fold_cheb(ggbk_scalars *s, ggbk_red 
 	for (int k = 0; k < 9; k++)
 		vv[k] = red.part[(GGBK_SL_G0 + k) * st];
 	red.cnt[0] = 0;
+	if (nwg == 11)
+		start_fold(s, red.part[(GGBK_SL_G0 + 9) * st], red.part[(GGBK_SL_G0 + 10) * st], tol, maxit);
 	ggbk_cheb_fold(s, vv, first);
 }
 
-extern "C" void ggbk_fold_cheb(ggbk_scalars *s, ggbk_red red, int nb, int first)
+extern "C" void ggbk_fold_cheb(ggbk_scalars *s, ggbk_red red, int nb, int first, int nb0, double tol, int maxit)
 {
-	GGBK_LAUNCH(k_fold_cheb, 9, FIN_BS, 0, s, red, nb, first);
+	GGBK_LAUNCH(k_fold_cheb, nb0 > 0 ? 11 : 9, FIN_BS, 0, s, red, nb, first, nb0, tol, maxit);
 }
 
 __global__ __launch_bounds__(FIN_BS) void k_fin(int op, ggbk_scalars *s, ggbk_red red, int nb, double tol, int maxit,
@@ -152,26 +179,9 @@ __global__ __launch_bounds__(FIN_BS) void k_fin(int op, ggbk_scalars *s, ggbk_re
 	if (threadIdx.x != 0 || (guarded && !act))
 		return;
 	switch (op) {
-	case GGBK_FIN_START: {
-		double bb = a, rr = b;
-		s->bb = bb;
-		s->rr = rr;
-		s->it = 0;
-		s->maxit = maxit;
-		s->rz = 0;
-		s->stop = 0;
-		if (bb == 0.0) {
-			s->status = 2;
-			s->tol2 = 0;
-			s->active = 0;
-		} else {
-			s->status = 0;
-			s->tol2 = tol * tol * bb;
-			s->active = (rr > s->tol2 && maxit > 0) ? 1 : 0;
-		}
-		s->vc = s->active;
+	case GGBK_FIN_START:
+		start_fold(s, a, b, tol, maxit);
 		break;
-	}
 	case GGBK_FIN_PAP: {
 		double pAp = a;
 		s->pAp = pAp;
@@ -317,10 +327,12 @@ __global__ __launch_bounds__(BS) void k_sub_mean(int n, T *f, const ggbk_scalars
 
 template <typename T>
 __global__ __launch_bounds__(BS) void k_cg_start(ggbk_grid g, T *omega, const T *psi, T *r, int sub_mean,
-						  const ggbk_scalars *s, ggbk_red red)
+						  ggbk_scalars *s, ggbk_red red, int open_vc)
 {
 	int i = xblock() * BS + threadIdx.x;
 	double bbl = 0, rrl = 0;
+	if (open_vc && i == 0)
+		s->vc = 1; // no block of this pass reads it
 	if (i < g.n) {
 		T om = omega[i];
 		if (sub_mean) {
@@ -707,14 +719,14 @@ extern "C" int ggbk_remove_mean_noalloc(ggbk_grid g, float *f)
 }
 
 extern "C" void ggbk_cg_start(ggbk_grid g, void *omega, const void *psi, void *r, int T, int sub_mean,
-			      const ggbk_scalars *s, ggbk_red red)
+			      ggbk_scalars *s, ggbk_red red, int open_vc)
 {
 	if (T == GGBK_F64)
 		GGBK_LAUNCH(k_cg_start<double>, NB(g.n), BS, 0, g, (double *) omega, (const double *) psi, (double *) r,
-			    sub_mean, s, red);
+			    sub_mean, s, red, open_vc);
 	else
 		GGBK_LAUNCH(k_cg_start<float>, NB(g.n), BS, 0, g, (float *) omega, (const float *) psi, (float *) r,
-			    sub_mean, s, red);
+			    sub_mean, s, red, open_vc);
 }
 
 template <typename T, typename TZ>

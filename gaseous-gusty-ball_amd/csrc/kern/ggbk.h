@@ -145,7 +145,7 @@ enum { GGBK_SL_BB = 0, GGBK_SL_RR0, GGBK_SL_PAP, GGBK_SL_RR, GGBK_SL_RZ, GGBK_SL
         * preconditioned, Ap the current search direction's image) */
        GGBK_SL_ZAZ, GGBK_SL_ZAP, GGBK_SL_RAZ, GGBK_SL_AZAZ, GGBK_SL_AZAP, GGBK_SL_RAP, GGBK_SL_APAP,
        /* the nine sums RZ, ZAZ .. APAP, RR of its fold, in that order (element 0 of each) */
-       GGBK_SL_G0, GGBK_NSLOTS = GGBK_SL_G0 + 9 };
+       GGBK_SL_G0, GGBK_NSLOTS = GGBK_SL_G0 + 11 };
 #define GGBK_NCNT 8 /* ticket counters per ggbk_red */
 enum {
 	GGBK_FIN_START = 0, /* bb, rr -> tol2, active, status */
@@ -178,9 +178,11 @@ int ggbk_noalloc_init(void);
 int ggbk_remove_mean_noalloc(ggbk_grid g, float *f);
 
 /* ---- PCG (reference fluid.c:288-358) ---- */
-/* [omega -= mean if sub_mean] r = b - K psi with b = -area*omega; bb, rr partials */
+/* [omega -= mean if sub_mean] r = b - K psi with b = -area*omega; bb, rr partials; with
+ * open_vc it sets s->vc, so that the mixed PCG's first V-cycle runs before its fold starts
+ * the solve */
 void ggbk_cg_start(ggbk_grid g, void *omega, const void *psi, void *r, int T, int sub_mean,
-		   const ggbk_scalars *s, ggbk_red red);
+		   ggbk_scalars *s, ggbk_red red, int open_vc);
 /* p_out = z (first) or z + beta p_in; Ap = K p_out; pAp partial (guarded by active) */
 void ggbk_cg_Ap(ggbk_grid g, const void *z, int TZ, const void *p_in, void *p_out, void *Ap, int T,
 		int first, const ggbk_scalars *s, ggbk_red red);
@@ -288,10 +290,16 @@ typedef struct {
 	double *Ap_out;
 	double *Az;                   /* in: the previous V-cycle's A z (update); out: this one's */
 	const float *z;               /* the previous V-cycle's z (update) */
+	int start;                    /* the solve's first V-cycle, whose fold also starts the solve */
+	int nb0;                      /* start: ggbk_cg_start's partials per sum */
+	int maxit, pad_;              /* start: the iteration cap */
+	double tol;                   /* start: the relative tolerance */
 } ggbk_cg_cheb;
 /* The fold after the PCG's V-cycle (nine workgroups; nb partials per sum): ggbk_cheb_fold
- * (dev.h) on the nine sums.  first: the solve's first V-cycle (no direction yet). */
-void ggbk_fold_cheb(ggbk_scalars *s, ggbk_red red, int nb, int first);
+ * (dev.h) on the nine sums.  first: the solve's first V-cycle (no direction yet).  With
+ * nb0 > 0 it also starts the solve from ggbk_cg_start's bb and rr partials (nb0 each, two
+ * more workgroups), as ggbk_fin GGBK_FIN_START would, before that fold. */
+void ggbk_fold_cheb(ggbk_scalars *s, ggbk_red red, int nb, int first, int nb0, double tol, int maxit);
 /* The PCG update alone, as ggbk_vcycle_cheb's first pass applies it: for the update the
  * host expects to be the last of the solve. */
 void ggbk_cg_update_cheb(ggbk_grid g, const ggbk_cg_cheb *c, const ggbk_scalars *s);

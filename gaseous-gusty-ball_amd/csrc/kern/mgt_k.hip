@@ -888,5 +888,5 @@ extern "C" void ggbk_vcycle_cheb(const ggbk_level *lv, int nlev, const ggbk_cg_c
 	ggbk_probe_end(GGBK_PROBE_MG_UP0);
 	// the fold (after the first V-cycle of a solve there is no direction yet)
 	if (c)
-		ggbk_fold_cheb(s, red, L0.tl.ntiles, !cg.update && !cg.Ap_in);
+		ggbk_fold_cheb(s, red, L0.tl.ntiles, !cg.update && !cg.Ap_in, cg.start ? cg.nb0 : 0, cg.tol, cg.maxit);
 }
